@@ -1,0 +1,206 @@
+/*
+ * tce_t.h -- C-ABI of the MI355X (gfx950) CCSD(T) perturbative-triples library
+ * (libtce_t.so).  Plain C types only: 64-bit `long` for every Fortran INTEGER
+ * (the reference is built with -fdefault-integer-8), `double` for REAL*8.
+ *
+ * Three groups of entry points:
+ *
+ *  1. The reference's own GPU kernel surface (Fortran-mangled names, every
+ *     argument by reference).  NWChem's src/tce/ccsd_t drivers
+ *     (ccsd_t_gpu.F, ccsd_t_singles_gpu.F, ccsd_t_doubles_gpu.F) link against
+ *     these unchanged.  Each replaces the function of the same name in
+ *     src/tce/ccsd_t/sd_t_total.cu, memory.cu and hybrid.c (cited per entry).
+ *
+ *  2. The whole-job API (tce_t_job_*): the tile-sextuple loop of
+ *     ccsd_t_gpu.F:87-241 as one device-resident job -- T1/T2/V2 blocks
+ *     uploaded once through the reference's offset hash tables, every task
+ *     contracted by fused HIP kernels on FP64 MFMA, energies reduced on the
+ *     device.  Returns status codes, never exits.
+ *
+ *  3. Host helpers that mirror the reference's integer bookkeeping
+ *     (tce_tile.F ranges, tce_*_offset.F hash tables, tce_hash.F) and the
+ *     seeded synthetic inputs used by tests and benchmarks.
+ */
+#ifndef TCE_T_H
+#define TCE_T_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status ---- */
+#define TCE_T_OK 0
+#define TCE_T_ERR_ARG (-1)     /* invalid argument or unsupported shape     */
+#define TCE_T_ERR_HIP (-2)     /* HIP runtime error (see tce_t_last_error)  */
+#define TCE_T_ERR_NOMEM (-3)   /* device or host allocation failed          */
+#define TCE_T_ERR_KEY (-4)     /* a block key is missing from a hash table  */
+#define TCE_T_ERR_STATE (-5)   /* call out of order (e.g. run before upload) */
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *tce_t_last_error(void);
+
+/* ================================================================ 1 ===== */
+/* Reference GPU kernel surface.  Dimensions are the ranges of the dispatch
+ * row's tiles; `t3` is ignored (as in the reference): results accumulate in
+ * the library-owned device tiles created by dev_mem_s_/dev_mem_d_. Operand
+ * arrays are host buffers owned by the caller and valid for the call only. */
+
+/* sd_t_total.cu:5435-5526 and siblings -- singles, triplesx_s(L_N) +-=
+ * t1sub(p4,h1) * v2sub(h3,h2,p6,p5). */
+void sd_t_s1_1_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_2_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_3_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_4_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_5_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_6_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_7_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_8_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+void sd_t_s1_9_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *t3, double *t1, double *v2);
+
+/* sd_t_total.cu:261-314 and siblings -- d1, triplesx(L_N) -+=
+ * sum_h7 t2sub(h7,p4,p5,h1) * v2sub(h3,h2,p6,h7). */
+void sd_t_d1_1_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_2_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_3_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_4_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_5_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_6_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_7_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_8_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+void sd_t_d1_9_cuda_(long *h1d, long *h2d, long *h3d, long *h7d, long *p4d, long *p5d, long *p6d, double *t3, double *t2, double *v2);
+
+/* sd_t_total.cu:2831-2888 and siblings -- d2, triplesx(L_N) -+=
+ * sum_p7 t2sub(p7,p4,h1,h2) * v2sub(p7,h3,p6,p5). */
+void sd_t_d2_1_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_2_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_3_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_4_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_5_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_6_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_7_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_8_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+void sd_t_d2_9_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
+
+/* sd_t_total.cu:5373-5379 -- energy of the current task's device tiles:
+ * energy[0] = E[T] part, energy[1] = E(T) part; eval1..6 point at the
+ * orbital energies of tiles h1,h2,h3,p4,p5,p6 (host); host1/host2 unused. */
+void compute_en_(double *factor, double *energy, double *eval1, double *eval2, double *eval3, double *eval4, double *eval5, double *eval6,
+                 long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *host1, double *host2);
+
+/* sd_t_total.cu:5-16, 5382-5396 -- allocate and zero the task's doubles /
+ * singles device tiles (h3 fastest ... p4 slowest). */
+void dev_mem_d_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d);
+void dev_mem_s_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d);
+/* sd_t_total.cu:17-28 -- release both device tiles. */
+void dev_release_(void);
+/* memory.cu:69-71, 164-176 -- device pool lifecycle. */
+void initmemmodule_(void);
+void finalizememmodule_(void);
+/* hybrid.c:25-29 -- 1 if the calling process owns a GPU (local rank < icuda).
+ * The local rank is read from the launcher environment (OMPI_COMM_WORLD_LOCAL_RANK,
+ * MPI_LOCALRANKID, SLURM_LOCALID, LOCAL_RANK; 0 when none is set). */
+int check_device_(long *icuda);
+/* hybrid.c:32-57 -- bind the process to device `local rank`; sets
+ * *cuda_device_number = 30 when fewer than *icuda devices are visible. */
+int device_init_(long *icuda, long *cuda_device_number);
+
+/* ================================================================ 2 ===== */
+/* Tile model: the tce.fh/MA arrays the path reads.  All arrays are indexed by
+ * tile-1; holes are tiles 1..noab, particles noab+1..noab+nvab. */
+typedef struct {
+  long noab, nvab;
+  const long *spin;   /* k_spin       */
+  const long *sym;    /* k_sym        */
+  const long *range;  /* k_range      */
+  const long *offset; /* k_offset     */
+  const long *alpha;  /* k_alpha (used when restricted), may be NULL */
+  const double *evl;  /* k_evl_sorted, length = sum of ranges         */
+  int restricted;
+} tce_t_tiles;
+
+/* Arithmetic of the d1/d2 contractions (s1, denominators and the energy sums
+ * are FP64 in every mode). */
+#define TCE_T_FP64 0        /* FP64 MFMA                                 */
+
+typedef struct tce_t_job tce_t_job;
+
+/* Bind the library to a device (call once per process). */
+int tce_t_init(int device);
+
+/* Plan a job: enumerate the tile sextuples of ccsd_t_gpu.F:87-112 and size
+ * the device tensors.  The tile arrays are copied. */
+int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **job);
+long tce_t_job_num_tasks(const tce_t_job *job);
+/* Device bytes the job holds once uploaded (admission control). */
+long tce_t_job_device_bytes(const tce_t_job *job);
+
+/* Upload T1, T2 and V2 once.  Arguments mirror d_t1/k_t1_offset,
+ * d_t2/k_t2_offset, d_v2/k_v2_offset of ccsd_t_gpu.F:2-3: host copies of the
+ * canonical TCE blocks and their hash tables [L, keys(1..L), offsets(1..L)].
+ * Every block the (T) path reads is fetched through the hash table exactly as
+ * get_hash_block (+ tce_sort_2/4 and their signs) would, and re-laid out on
+ * the device.  TCE_T_ERR_KEY if a needed key is absent. */
+int tce_t_job_upload(tce_t_job *job, const double *t1, const long *k_t1_offset,
+                     const double *t2, const long *k_t2_offset,
+                     const double *v2, const long *k_v2_offset);
+
+/* Run every task; *e_bt = E[T] (energy1), *e_pt = E(T) (energy2) of
+ * ccsd_t_gpu.F:243-244.  Synchronous. */
+int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt);
+/* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
+ * the last run.  out has 2*ntasks doubles. */
+int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);
+/* Host half of tce_t_job_upload, for inspection and tests: the dense tensors
+ * the job places in HBM, built from the same blocks and hash tables.  With
+ * Ep = no. of virtuals and Mp = no. of occupieds rounded up to a multiple of 8:
+ *   T1[nv][no], T2e[no][no][nv][Ep] = t(p e h h'), T2m[no][nv][nv][Mp] =
+ *   t(p p' h m), Wd1[no][no][nv][Mp] = <m p||h h'>, Vd2[no][nv][nv][Ep] =
+ *   <p p'||w e>, S[no][no][nv][nv] = <p p'||h h'>.  Any output may be NULL. */
+int tce_t_pack_dense(const tce_t_tiles *tiles, const double *t1, const long *k_t1_offset,
+                     const double *t2, const long *k_t2_offset, const double *v2,
+                     const long *k_v2_offset, double *T1, double *T2e, double *T2m,
+                     double *Wd1, double *Vd2, double *S);
+
+/* Device time (ms) of the last run's contraction kernel(s), HIP events. */
+double tce_t_job_last_kernel_ms(const tce_t_job *job);
+int tce_t_job_destroy(tce_t_job *job);
+
+/* ================================================================ 3 ===== */
+/* tce_tile.F:330-355 -- tiles of n orbitals, sizes differing by <= 1. */
+long tce_t_tile_ranges(long n, long tilesize, long *ranges);
+/* tce_t1_offset.F, tce_t2_offset.F, tce_mo2e_offset.F (restricted to the
+ * three integral classes the (T) path reads).  hash may be NULL to query L.
+ * Returns L; *size = doubles in the block array. */
+long tce_t_t1_offset(const tce_t_tiles *tiles, long *hash, long *size);
+long tce_t_t2_offset(const tce_t_tiles *tiles, long *hash, long *size);
+long tce_t_v2_offset(const tce_t_tiles *tiles, long *hash, long *size);
+/* tce_hash.F:271-316 -- offset of key, or -1. */
+long tce_t_hash(const long *hash, long key);
+
+/* Seeded synthetic C1 spin-orbital inputs (Philox4x32-10 counters keyed by
+ * global orbital indices, Box-Muller normals):
+ *   t1 ~ N(0,0.01), t2 ~ N(0,0.02) antisymmetrised in both pairs,
+ *   <h p||h h>, <p p||h p>, <p p||h h> ~ N(0,0.05) antisymmetrised in their
+ *   same-space pairs, occupied energies sorted U[-2,-0.3], virtual sorted
+ *   U[0.2,4.0].  Values depend only on (seed, global indices), never on the
+ *   tiling. */
+int tce_t_synth_evl(unsigned long long seed, long no, long nv, double *evl);
+int tce_t_synth_blocks(const tce_t_tiles *tiles, unsigned long long seed,
+                       double *t1, const long *t1_hash,
+                       double *t2, const long *t2_hash,
+                       double *v2, const long *v2_hash);
+/* Dense forms (tests): T1[nv][no], T2[nv][nv][no][no], W[no][nv][no][no]
+ * = <m p||h h'>, V[nv][nv][no][nv] = <p p'||h e>, S[nv][nv][no][no]. */
+int tce_t_synth_dense(unsigned long long seed, long no, long nv, double *T1,
+                      double *T2, double *W, double *V, double *S);
+/* Sticky failure flag of the per-call surface (group 1), which has no return
+ * codes: TCE_T_ERR_HIP if any call failed since the last query, then reset. */
+int tce_t_legacy_status(void);
+
+/* Philox4x32-10 block function (known-answer tests). */
+void tce_t_philox4x32(const unsigned int ctr[4], const unsigned int key[2],
+                      unsigned int out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,12 @@
+"""nwchem_amd -- MI355X-native CCSD(T) perturbative triples (the TCE (T) path
+of NWChem src/tce/ccsd_t) behind the reference's own kernel surface.
+
+Native code: libtce_t.so (nwchem_amd/csrc, hand-written HIP for gfx950).
+Public C-ABI: include/tce_t.h.  Python here is a thin ctypes layer.
+"""
+from ._lib import build, lib  # noqa: F401
+from .job import FP64, TriplesJob, init  # noqa: F401
+from .tiles import Blocks, TileModel, synth_blocks, synth_dense, tile_ranges  # noqa: F401
+
+__all__ = ["build", "lib", "init", "TriplesJob", "FP64", "TileModel", "Blocks", "synth_blocks",
+           "synth_dense", "tile_ranges"]

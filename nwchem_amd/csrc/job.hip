@@ -1,0 +1,801 @@
+// job.hip -- whole-job (T): the tile-sextuple loop of ccsd_t_gpu.F:87-241 as
+// one device-resident job on MI355X.
+//
+// Data path
+//   upload   T1/T2/V2 canonical TCE blocks are read through the reference's
+//            offset hash tables (get_hash_block semantics, tce_hashnsort.F
+//            signs, tce_restricted mapping) and re-laid out once in HBM as
+//            dense tensors whose contraction index is contiguous:
+//              T2e[h][h'][p][e]  = t(p e h h')       (d2 operand, e = p7)
+//              T2m[h][p][p'][m]  = t(p p' h m)       (d1 operand, m = h7)
+//              Vd2[w][p][p'][e]  = <p p'||w e>       (d2 integral)
+//              Wd1[h][h'][p][m]  = <m p||h h'>       (d1 integral)
+//              S  [h][h'][p][p'] = <p p'||h h'>      (s1 integral)
+//              T1 [p][h]         = t(p h)
+//            (contraction extents padded with zeros to a multiple of 8).
+//   run      one persistent kernel; each work item is a 4x4x4 particle cube
+//            by 4x4x4 hole box of one task's t3 tile.  A wavefront keeps the
+//            4096 t3 elements of its item in 64 accumulator registers, runs
+//            all 18 d1/d2 contractions of the item on v_mfma_f64_4x4x4_4b,
+//            then adds the 9 s1 products and the denominator-weighted energy
+//            in FP64 and writes one {E[T],E(T)} partial.  t3 never touches
+//            memory.  A second kernel sums the partials of each task in a
+//            fixed order (bit-reproducible task energies).
+//
+// Register layout of an item (one double per lane per register r):
+//   lane = x + 4y + 16z, register r = 16*ii + 4*jj + kk (hole offsets)
+//   phase 1: (a,b,c) = (z, x, y)    rows a, cols b, blocks c
+//   phase 2: (a,b,c) = (y, x, z)    rows c, cols b, blocks a
+// v_mfma_f64_4x4x4_4b reads A[i][k] of block q from lane i+4q+16k, B[k][j]
+// from lane j+4q+16k and writes D[i][j] to lane j+4q+16i, so a contraction
+// can run on the accumulators iff its row index and column index sit on
+// different operands; the block index may sit on either (the other operand is
+// replicated).  Phase 1 carries the 12 contractions whose lone particle is a
+// or b (d2 N=1..6, d1 N=4..9), phase 2 the 6 whose lone particle is c (d2
+// N=7..9, d1 N=1..3); one lane permutation switches the layouts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+namespace tce_t {
+namespace {
+
+struct TaskDev {
+  int op4, op5, op6;  // particle-space offsets of the task's particle tiles
+  int oh1, oh2, oh3;  // hole offsets
+  int r4, r5, r6, r1, r2, r3;
+  int nb4, nb5, nb6, nb1, nb2, nb3;  // 4-wide boxes per index
+  long item0;                        // first work item of the task
+  double factor;
+};
+
+struct FusedArgs {
+  const double *T1, *T2e, *T2m, *Vd2, *Wd1, *S, *evl;
+  const TaskDev *tasks;
+  int ntasks;
+  long nitems;
+  int no, nv, Ep, Mp;  // orbitals, padded contraction extents
+  unsigned long long *counter;
+  double *partials;
+};
+
+typedef double acc_t;
+
+__device__ __forceinline__ double mfma4(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// One contraction on the accumulators.  PAIR selects the two hole indices
+// the "pair" operand P carries (0: i,j  1: j,k  2: i,k); the third hole
+// indexes the "single" operand Q.  P_IS_R: P supplies the MFMA rows (A).
+// P/Q point at the lane's particle row; hole offsets are added here; the
+// contraction index runs contiguously, two values per lane per load.
+template <int PAIR, bool P_IS_R>
+__device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restrict__ P, long pS,
+                                         const double *__restrict__ Q, long qS, const int (&hI)[4],
+                                         const int (&hJ)[4], const int (&hK)[4], int no, int Kp,
+                                         int z, double sign) {
+  // hole offsets are wave-uniform; the lane's contraction slot goes into
+  // the base pointers
+  long po[4][4], qo[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      int h1 = PAIR == 1 ? hJ[u] : hI[u];
+      int h2 = PAIR == 0 ? hJ[v] : hK[v];
+      po[u][v] = ((long)h1 * no + h2) * pS;
+    }
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    int hs = PAIR == 0 ? hK[w] : (PAIR == 1 ? hI[w] : hJ[w]);
+    qo[w] = (long)hs * qS;
+  }
+  P += 2 * z;
+  Q += 2 * z;
+  for (int k0 = 0; k0 < Kp; k0 += 8) {
+    double2 pf[4][4], qf[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) pf[u][v] = *(const double2 *)(P + po[u][v] + k0);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      double2 t = *(const double2 *)(Q + qo[w] + k0);
+      qf[w].x = sign * t.x;
+      qf[w].y = sign * t.y;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int w = 0; w < 4; ++w) {
+            int r = PAIR == 0 ? (u * 4 + v) * 4 + w : (PAIR == 1 ? (w * 4 + u) * 4 + v : (u * 4 + w) * 4 + v);
+            double pv = t ? pf[u][v].y : pf[u][v].x;
+            double qv = t ? qf[w].y : qf[w].x;
+            acc[r] = P_IS_R ? mfma4(pv, qv, acc[r]) : mfma4(qv, pv, acc[r]);
+          }
+  }
+}
+
+__device__ __forceinline__ long rfl64(long v) {
+  int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffL));
+  int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return ((long)hi << 32) | (unsigned)lo;
+}
+
+__device__ __forceinline__ double perm_lane(double v, int src_lane) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b & 0xffffffffLL));
+  int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__global__ void zero_counter(unsigned long long *c) {
+  if (threadIdx.x == 0) *c = 0ull;
+}
+
+__global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
+  const int lane = threadIdx.x;
+  const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
+  const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
+  const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
+  for (;;) {
+    long item = 0;
+    if (lane == 0) item = (long)atomicAdd(A.counter, 1ull);
+    item = __shfl(item, 0);
+    item = rfl64(item);
+    if (item >= A.nitems) break;
+    // locate the task (tasks sorted by item0)
+    int lo = 0, hi = A.ntasks - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (A.tasks[mid].item0 <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const TaskDev &T = A.tasks[lo];
+    long box = item - T.item0;
+    int bc = box % T.nb6; box /= T.nb6;
+    int bb = box % T.nb5; box /= T.nb5;
+    int ba = box % T.nb4; box /= T.nb4;
+    int bk = box % T.nb3; box /= T.nb3;
+    int bj = box % T.nb2; box /= T.nb2;
+    int bi = (int)box;
+    const int a0 = 4 * ba, b0 = 4 * bb, c0 = 4 * bc;
+    int hI[4], hJ[4], hK[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      hI[u] = T.oh1 + min(4 * bi + u, T.r1 - 1);
+      hJ[u] = T.oh2 + min(4 * bj + u, T.r2 - 1);
+      hK[u] = T.oh3 + min(4 * bk + u, T.r3 - 1);
+    }
+    // lane particle indices (global particle space, clamped into the tile)
+    const long ax = T.op4 + min(a0 + x, T.r4 - 1), ay = T.op4 + min(a0 + y, T.r4 - 1);
+    const long bx = T.op5 + min(b0 + x, T.r5 - 1);
+    const long cx = T.op6 + min(c0 + x, T.r6 - 1), cy = T.op6 + min(c0 + y, T.r6 - 1);
+
+    acc_t acc[64];
+#pragma unroll
+    for (int r = 0; r < 64; ++r) acc[r] = 0.0;
+
+    const long sT2e = nvE;             // stride of (h*no+h') in T2e
+    const long sW = nvM;               // stride of (h*no+h') in Wd1
+    const long sV = (long)nv * nvE;    // stride of w in Vd2
+    const long sT2m = (long)nv * nvM;  // stride of h in T2m
+    // ---- phase 1: lone particle a or b
+    // d2 N=1..3: rows a from t(a e h h'), cols b / blocks c from <b c||h e>
+    contract<0, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    contract<1, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    contract<2, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    // d2 N=4..6: rows a / blocks c from <a c||h e>, cols b from t(b e h h')
+    contract<0, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    contract<1, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    contract<2, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    // d1 N=4..6: rows a from <m a||h h'>, cols b / blocks c from t(b c h m)
+    contract<1, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    contract<2, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    contract<0, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    // d1 N=7..9: rows a / blocks c from t(a c h m), cols b from <m b||h h'>
+    contract<1, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    contract<2, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    contract<0, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    // ---- switch layouts: element (a,b,c) moves from lane b+4c+16a to b+4a+16c
+    {
+      const int src = x + 4 * z + 16 * y;
+#pragma unroll
+      for (int r = 0; r < 64; ++r) acc[r] = perm_lane(acc[r], src);
+    }
+    // ---- phase 2: lone particle c
+    // d2 N=7..9: rows c from t(c e h h'), cols b / blocks a from <a b||h e>
+    contract<0, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    contract<1, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    contract<2, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    // d1 N=1..3: rows c from <m c||h h'>, cols b / blocks a from t(a b h m)
+    contract<1, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    contract<2, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    contract<0, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+
+    // ---- singles, denominators, energy (layout 2: a=y, b=x, c=z)
+    double e4 = 0.0, e5 = 0.0;
+    const int la = a0 + y, lb = b0 + x, lc = c0 + z;
+    if (la < T.r4 && lb < T.r5 && lc < T.r6) {
+      const long pa = T.op4 + la, pb = T.op5 + lb, pc = T.op6 + lc;
+      const double ea = A.evl[no + pa], eb = A.evl[no + pb], ec = A.evl[no + pc];
+      const double denom_0 = -(ea + eb + ec);
+      const double *T1 = A.T1, *S = A.S;
+      const long sSh = (long)nv * nv;  // stride of (h*no+h') in S
+      const long pbc = pb * nv + pc, pac = pa * nv + pc, pab = pa * nv + pb;
+#pragma unroll
+      for (int ii = 0; ii < 4; ++ii) {
+        if (4 * bi + ii >= T.r1) continue;
+        const int hi = hI[ii];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          if (4 * bj + jj >= T.r2) continue;
+          const int hj = hJ[jj];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            if (4 * bk + kk >= T.r3) continue;
+            const int hk = hK[kk];
+            const double d = acc[ii * 16 + jj * 4 + kk];
+            const long jk = ((long)hj * no + hk) * sSh, ik = ((long)hi * no + hk) * sSh,
+                       ij = ((long)hi * no + hj) * sSh;
+            double s = T1[pa * no + hi] * S[jk + pbc];
+            s -= T1[pa * no + hj] * S[ik + pbc];
+            s += T1[pa * no + hk] * S[ij + pbc];
+            s -= T1[pb * no + hi] * S[jk + pac];
+            s += T1[pb * no + hj] * S[ik + pac];
+            s -= T1[pb * no + hk] * S[ij + pac];
+            s += T1[pc * no + hi] * S[jk + pab];
+            s -= T1[pc * no + hj] * S[ik + pab];
+            s += T1[pc * no + hk] * S[ij + pab];
+            const double den = d * T.factor / (A.evl[hi] + A.evl[hj] + A.evl[hk] + denom_0);
+            e4 += den * d;
+            e5 += den * (d + s);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      e4 += __shfl_xor(e4, off);
+      e5 += __shfl_xor(e5, off);
+    }
+    if (lane == 0) {
+      A.partials[2 * item] = e4;
+      A.partials[2 * item + 1] = e5;
+    }
+  }
+}
+
+// Per-task fixed-order reduction of the item partials.
+__global__ void __launch_bounds__(256) task_reduce(const TaskDev *__restrict__ tasks, int ntasks,
+                                                   long nitems, const double *__restrict__ part,
+                                                   double *__restrict__ out) {
+  const int t = blockIdx.x;
+  if (t >= ntasks) return;
+  const long i0 = tasks[t].item0, i1 = (t + 1 < ntasks) ? tasks[t + 1].item0 : nitems;
+  double s4 = 0.0, s5 = 0.0;
+  for (long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    s4 += part[2 * i];
+    s5 += part[2 * i + 1];
+  }
+  __shared__ double r4[256], r5[256];
+  r4[threadIdx.x] = s4;
+  r5[threadIdx.x] = s5;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r4[threadIdx.x] += r4[threadIdx.x + w];
+      r5[threadIdx.x] += r5[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * t] = r4[0];
+    out[2 * t + 1] = r5[0];
+  }
+}
+
+template <class F> void parallel_for(long n, F &&f) {
+  unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  std::atomic<long> next(0);
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (long i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+long round8(long n) { return (n + 7) / 8 * 8; }
+
+}  // namespace
+
+struct DenseHost {
+  std::vector<double> T1, T2e, T2m, Wd1, Vd2, S;
+};
+
+struct Job {
+  Tiles T;
+  std::vector<TaskDev> tasks;
+  std::vector<long> task_tiles;  // 6 per task: p4b,p5b,p6b,h1b,h2b,h3b
+  long nitems = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double *dT1 = nullptr, *dT2e = nullptr, *dT2m = nullptr, *dVd2 = nullptr, *dWd1 = nullptr,
+         *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
+  TaskDev *dtasks = nullptr;
+  unsigned long long *dcounter = nullptr;
+  bool uploaded = false;
+  float last_ms = 0.f;
+  std::vector<double> taskE;
+  long Ep = 0, Mp = 0;
+  int grid = 0;
+};
+
+}  // namespace tce_t
+
+using namespace tce_t;
+
+static bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  set_error(buf);
+  return false;
+}
+#define HIPCHK(x, what)                  \
+  do {                                   \
+    if (!hip_ok((x), what)) return TCE_T_ERR_HIP; \
+  } while (0)
+
+extern "C" int tce_t_init(int device) {
+  HIPCHK(hipSetDevice(device), "hipSetDevice");
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
+  if (!out) return TCE_T_ERR_ARG;
+  auto J = std::make_unique<Job>();
+  if (!J->T.load(tiles)) {
+    set_error("invalid tile description");
+    return TCE_T_ERR_ARG;
+  }
+  const Tiles &T = J->T;
+  long noab = T.noab, nvab = T.nvab, item = 0;
+  // task loop order of ccsd_t_gpu.F:87-92
+  for (long p4b = noab + 1; p4b <= noab + nvab; ++p4b)
+    for (long p5b = p4b; p5b <= noab + nvab; ++p5b)
+      for (long p6b = p5b; p6b <= noab + nvab; ++p6b)
+        for (long h1b = 1; h1b <= noab; ++h1b)
+          for (long h2b = h1b; h2b <= noab; ++h2b)
+            for (long h3b = h2b; h3b <= noab; ++h3b) {
+              if (!task_allowed(T, p4b, p5b, p6b, h1b, h2b, h3b)) continue;
+              TaskDev d;
+              d.op4 = (int)(T.ofs(p4b) - T.no);
+              d.op5 = (int)(T.ofs(p5b) - T.no);
+              d.op6 = (int)(T.ofs(p6b) - T.no);
+              d.oh1 = (int)T.ofs(h1b);
+              d.oh2 = (int)T.ofs(h2b);
+              d.oh3 = (int)T.ofs(h3b);
+              d.r4 = (int)T.rng(p4b);
+              d.r5 = (int)T.rng(p5b);
+              d.r6 = (int)T.rng(p6b);
+              d.r1 = (int)T.rng(h1b);
+              d.r2 = (int)T.rng(h2b);
+              d.r3 = (int)T.rng(h3b);
+              d.nb4 = (d.r4 + 3) / 4;
+              d.nb5 = (d.r5 + 3) / 4;
+              d.nb6 = (d.r6 + 3) / 4;
+              d.nb1 = (d.r1 + 3) / 4;
+              d.nb2 = (d.r2 + 3) / 4;
+              d.nb3 = (d.r3 + 3) / 4;
+              d.item0 = item;
+              d.factor = task_factor(T.restricted, p4b, p5b, p6b, h1b, h2b, h3b);
+              item += (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
+              J->tasks.push_back(d);
+              long tt[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
+              J->task_tiles.insert(J->task_tiles.end(), tt, tt + 6);
+            }
+  J->nitems = item;
+  J->Ep = round8(T.nv);
+  J->Mp = round8(T.no);
+  *out = reinterpret_cast<tce_t_job *>(J.release());
+  return TCE_T_OK;
+}
+
+extern "C" long tce_t_job_num_tasks(const tce_t_job *job) {
+  return job ? (long)reinterpret_cast<const Job *>(job)->tasks.size() : -1;
+}
+
+static long dense_bytes(const Job &J) {
+  long no = J.T.no, nv = J.T.nv;
+  long n = nv * no                       // T1
+           + no * no * nv * J.Ep         // T2e
+           + no * nv * nv * J.Mp         // T2m
+           + no * nv * nv * J.Ep         // Vd2
+           + no * no * nv * J.Mp         // Wd1
+           + no * no * nv * nv           // S
+           + J.T.norb;                   // evl
+  return n * (long)sizeof(double);
+}
+
+extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
+  if (!job) return -1;
+  const Job &J = *reinterpret_cast<const Job *>(job);
+  return dense_bytes(J) + J.nitems * 16 + (long)J.tasks.size() * (sizeof(TaskDev) + 16) + 64;
+}
+
+namespace {
+
+// Map a tile tuple through tce_restricted_4 / _2 (identity when the job is
+// unrestricted or the block is not all-beta).
+void restrict4(const Tiles &T, long &a, long &b, long &c, long &d) {
+  if (T.restricted && T.spn(a) + T.spn(b) + T.spn(c) + T.spn(d) == 8) {
+    a = T.alp(a);
+    b = T.alp(b);
+    c = T.alp(c);
+    d = T.alp(d);
+  }
+}
+bool allowed4(const Tiles &T, long a, long b, long c, long d) {
+  return T.spn(a) + T.spn(b) == T.spn(c) + T.spn(d) && (T.sy(a) ^ T.sy(b) ^ T.sy(c) ^ T.sy(d)) == 0;
+}
+
+struct Packer {
+  const Tiles &T;
+  int err = 0;
+  // Fetch the canonical block with tiles (g3,g4,g1,g2) (already ordered),
+  // returns pointer or nullptr (forbidden block -> zeros).  Sets err when a
+  // needed key is absent.
+  const double *fetch(const double *file, const long *hash, long key, bool needed) {
+    long off = hash_lookup(hash, key);
+    if (off < 0) {
+      if (needed) err = TCE_T_ERR_KEY;
+      return nullptr;
+    }
+    return file + off;
+  }
+};
+
+}  // namespace
+
+// Host side of the upload: every block the path reads is fetched through the
+// hash tables and scattered into the dense device layouts (see the header of
+// this file).  Blocks forbidden by spin/symmetry stay zero; a permitted block
+// missing from a table is an error (the reference would errquit in tce_hash).
+static int pack_dense(const Tiles &T, long Ep, long Mp, const double *t1, const long *t1h,
+                      const double *t2, const long *t2h, const double *v2, const long *v2h,
+                      DenseHost &D) {
+  const long no = T.no, nv = T.nv, noab = T.noab, nvab = T.nvab, N = noab + nvab;
+  Packer pk{T};
+  std::atomic<int> err(0);
+
+  // ---- T1[p][h]: t(p h) blocks, key h1b-1 + noab*(p4b-noab-1) (ccsd_t_singles_gpu.F:233)
+  std::vector<double> &hT1 = D.T1;
+  hT1.assign(nv * no, 0.0);
+  for (long pb = noab + 1; pb <= noab + nvab; ++pb)
+    for (long hb = 1; hb <= noab; ++hb) {
+      long p1 = pb, h1 = hb;
+      if (T.restricted && T.spn(pb) + T.spn(hb) == 4) {
+        p1 = T.alp(pb);
+        h1 = T.alp(hb);
+      }
+      bool need = T.spn(pb) == T.spn(hb) && (T.sy(pb) ^ T.sy(hb)) == 0;
+      const double *blk = pk.fetch(t1, t1h, h1 - 1 + noab * (p1 - noab - 1), need);
+      if (!blk) continue;
+      long rp = T.rng(pb), rh = T.rng(hb), op = T.ofs(pb) - no, oh = T.ofs(hb);
+      for (long p = 0; p < rp; ++p)
+        for (long h = 0; h < rh; ++h) hT1[(op + p) * no + oh + h] = blk[p * rh + h];
+    }
+
+  // ---- T2 in both layouts.  t(p p' h h') for any tile order: canonical block
+  // of the sorted tiles, sign -1 per swapped pair (tce_hashnsort.F:47-64,
+  // 129-144 and the sorts of ccsd_t_doubles_gpu.F:282-304, 942-976).
+  std::vector<double> &hT2e = D.T2e, &hT2m = D.T2m;
+  hT2e.assign((size_t)no * no * nv * Ep, 0.0);
+  hT2m.assign((size_t)no * nv * nv * Mp, 0.0);
+  {
+    std::vector<std::pair<long, long>> ppairs;
+    for (long P1 = noab + 1; P1 <= N; ++P1)
+      for (long P2 = noab + 1; P2 <= N; ++P2) ppairs.push_back({P1, P2});
+    parallel_for((long)ppairs.size(), [&](long q) {
+      long P1 = ppairs[q].first, P2 = ppairs[q].second;
+      for (long H1 = 1; H1 <= noab; ++H1)
+        for (long H2 = 1; H2 <= noab; ++H2) {
+          bool sp = P1 > P2, sh = H1 > H2;
+          long a = sp ? P2 : P1, b = sp ? P1 : P2, c = sh ? H2 : H1, d = sh ? H1 : H2;
+          bool need = allowed4(T, a, b, c, d);
+          restrict4(T, a, b, c, d);
+          long key = d - 1 + noab * (c - 1 + noab * (b - noab - 1 + nvab * (a - noab - 1)));
+          const double *blk = pk.fetch(t2, t2h, key, need);
+          if (!blk) {
+            if (need) err = TCE_T_ERR_KEY;
+            continue;
+          }
+          long r1 = T.rng(P1), r2 = T.rng(P2), r3 = T.rng(H1), r4 = T.rng(H2);
+          long o1 = T.ofs(P1) - no, o2 = T.ofs(P2) - no, o3 = T.ofs(H1), o4 = T.ofs(H2);
+          // canonical block extents
+          long ca = sp ? r2 : r1, cb = sp ? r1 : r2, cc = sh ? r4 : r3, cd = sh ? r3 : r4;
+          (void)ca;
+          double sgn = (sp ? -1.0 : 1.0) * (sh ? -1.0 : 1.0);
+          for (long i = 0; i < r1; ++i)
+            for (long j = 0; j < r2; ++j)
+              for (long k = 0; k < r3; ++k)
+                for (long l = 0; l < r4; ++l) {
+                  long ia = sp ? j : i, ib = sp ? i : j, ic = sh ? l : k, id = sh ? k : l;
+                  double val = sgn * blk[((ia * cb + ib) * cc + ic) * cd + id];
+                  long p = o1 + i, e = o2 + j, h = o3 + k, g = o4 + l;
+                  hT2e[((h * no + g) * nv + p) * Ep + e] = val;  // t(p e h g)
+                  hT2m[((h * nv + p) * nv + e) * Mp + g] = val;  // t(p e h m=g)
+                }
+        }
+    });
+  }
+
+  // ---- <h p||h h> (d1): v(h7 p6 h2 h3), key h3-1+N*(h2-1+N*(p6-1+N*(h7-1)))
+  std::vector<double> &hW = D.Wd1;
+  hW.assign((size_t)no * no * nv * Mp, 0.0);
+  for (long H7 = 1; H7 <= noab; ++H7)
+    for (long P6 = noab + 1; P6 <= N; ++P6)
+      for (long H2 = 1; H2 <= noab; ++H2)
+        for (long H3 = 1; H3 <= noab; ++H3) {
+          bool sw = H2 > H3;
+          long g3 = H7, g4 = P6, g1 = sw ? H3 : H2, g2 = sw ? H2 : H3;
+          bool need = allowed4(T, g3, g4, g1, g2);
+          restrict4(T, g3, g4, g1, g2);
+          const double *blk = pk.fetch(v2, v2h, g2 - 1 + N * (g1 - 1 + N * (g4 - 1 + N * (g3 - 1))), need);
+          if (!blk) {
+            if (need) err = TCE_T_ERR_KEY;
+            continue;
+          }
+          long r7 = T.rng(H7), r6 = T.rng(P6), r2 = T.rng(H2), r3 = T.rng(H3);
+          long c1 = sw ? r3 : r2, c2 = sw ? r2 : r3;
+          (void)c1;
+          double sgn = sw ? -1.0 : 1.0;
+          for (long m = 0; m < r7; ++m)
+            for (long p = 0; p < r6; ++p)
+              for (long h = 0; h < r2; ++h)
+                for (long g = 0; g < r3; ++g) {
+                  long ih = sw ? g : h, ig = sw ? h : g;
+                  double val = sgn * blk[((m * r6 + p) * (sw ? r3 : r2) + ih) * c2 + ig];
+                  long M = T.ofs(H7) + m, Pp = T.ofs(P6) - no + p, Hh = T.ofs(H2) + h, G = T.ofs(H3) + g;
+                  hW[((Hh * no + G) * nv + Pp) * Mp + M] = val;
+                }
+        }
+
+  // ---- <p p||h p> (d2): v(p5 p6 h3 p7), key p7-1+N*(h3-1+N*(p6-1+N*(p5-1)))
+  std::vector<double> &hV = D.Vd2;
+  hV.assign((size_t)no * nv * nv * Ep, 0.0);
+  {
+    std::vector<std::pair<long, long>> ppairs;
+    for (long P5 = noab + 1; P5 <= N; ++P5)
+      for (long P6 = noab + 1; P6 <= N; ++P6) ppairs.push_back({P5, P6});
+    parallel_for((long)ppairs.size(), [&](long q) {
+      long P5 = ppairs[q].first, P6 = ppairs[q].second;
+      bool sw = P5 > P6;
+      for (long H3 = 1; H3 <= noab; ++H3)
+        for (long P7 = noab + 1; P7 <= N; ++P7) {
+          long g3 = sw ? P6 : P5, g4 = sw ? P5 : P6, g1 = H3, g2 = P7;
+          bool need = allowed4(T, g3, g4, g1, g2);
+          restrict4(T, g3, g4, g1, g2);
+          const double *blk = pk.fetch(v2, v2h, g2 - 1 + N * (g1 - 1 + N * (g4 - 1 + N * (g3 - 1))), need);
+          if (!blk) {
+            if (need) err = TCE_T_ERR_KEY;
+            continue;
+          }
+          long r5 = T.rng(P5), r6 = T.rng(P6), r3 = T.rng(H3), r7 = T.rng(P7);
+          long cb = sw ? r5 : r6;
+          double sgn = sw ? -1.0 : 1.0;
+          for (long i = 0; i < r5; ++i)
+            for (long j = 0; j < r6; ++j)
+              for (long h = 0; h < r3; ++h)
+                for (long e = 0; e < r7; ++e) {
+                  long ia = sw ? j : i, ib = sw ? i : j;
+                  double val = sgn * blk[((ia * cb + ib) * r3 + h) * r7 + e];
+                  long p = T.ofs(P5) - no + i, pp = T.ofs(P6) - no + j, w = T.ofs(H3) + h,
+                       E = T.ofs(P7) - no + e;
+                  hV[((w * nv + p) * nv + pp) * Ep + E] = val;
+                }
+        }
+    });
+  }
+
+  // ---- <p p||h h> (s1): v(p5 p6 h2 h3), key h3-1+N*(h2-1+N*(p6-1+N*(p5-1)))
+  std::vector<double> &hS = D.S;
+  hS.assign((size_t)no * no * nv * nv, 0.0);
+  for (long P5 = noab + 1; P5 <= N; ++P5)
+    for (long P6 = noab + 1; P6 <= N; ++P6)
+      for (long H2 = 1; H2 <= noab; ++H2)
+        for (long H3 = 1; H3 <= noab; ++H3) {
+          bool sp = P5 > P6, sh = H2 > H3;
+          long g3 = sp ? P6 : P5, g4 = sp ? P5 : P6, g1 = sh ? H3 : H2, g2 = sh ? H2 : H3;
+          bool need = allowed4(T, g3, g4, g1, g2);
+          restrict4(T, g3, g4, g1, g2);
+          const double *blk = pk.fetch(v2, v2h, g2 - 1 + N * (g1 - 1 + N * (g4 - 1 + N * (g3 - 1))), need);
+          if (!blk) {
+            if (need) err = TCE_T_ERR_KEY;
+            continue;
+          }
+          long r5 = T.rng(P5), r6 = T.rng(P6), r2 = T.rng(H2), r3 = T.rng(H3);
+          long cb = sp ? r5 : r6, cd = sh ? r2 : r3;
+          double sgn = (sp ? -1.0 : 1.0) * (sh ? -1.0 : 1.0);
+          for (long i = 0; i < r5; ++i)
+            for (long j = 0; j < r6; ++j)
+              for (long h = 0; h < r2; ++h)
+                for (long g = 0; g < r3; ++g) {
+                  long ia = sp ? j : i, ib = sp ? i : j, ic = sh ? g : h, id = sh ? h : g;
+                  double val = sgn * blk[((ia * cb + ib) * (sh ? r3 : r2) + ic) * cd + id];
+                  long p = T.ofs(P5) - no + i, pp = T.ofs(P6) - no + j, H = T.ofs(H2) + h,
+                       G = T.ofs(H3) + g;
+                  hS[((H * no + G) * nv + p) * nv + pp] = val;
+                }
+        }
+  if (pk.err || err) {
+    set_error("a block needed by the (T) path is missing from a hash table");
+    return TCE_T_ERR_KEY;
+  }
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1h, const double *t2,
+                                const long *t2h, const double *v2, const long *v2h) {
+  if (!job || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  const Tiles &T = J.T;
+  DenseHost D;
+  int rc = pack_dense(T, J.Ep, J.Mp, t1, t1h, t2, t2h, v2, v2h, D);
+  if (rc != TCE_T_OK) return rc;
+  std::vector<double> &hT1 = D.T1, &hT2e = D.T2e, &hT2m = D.T2m, &hV = D.Vd2, &hW = D.Wd1, &hS = D.S;
+  // ---- device buffers
+  auto up = [&](double *&d, const std::vector<double> &h, const char *what) -> bool {
+    if (d) (void)hipFree(d);
+    d = nullptr;
+    if (!hip_ok(hipMalloc(&d, h.size() * sizeof(double)), what)) return false;
+    return hip_ok(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), what);
+  };
+  if (!J.stream) HIPCHK(hipStreamCreateWithFlags(&J.stream, hipStreamNonBlocking), "stream");
+  if (!J.ev0) HIPCHK(hipEventCreate(&J.ev0), "event");
+  if (!J.ev1) HIPCHK(hipEventCreate(&J.ev1), "event");
+  if (!up(J.dT1, hT1, "T1") || !up(J.dT2e, hT2e, "T2e") || !up(J.dT2m, hT2m, "T2m") ||
+      !up(J.dVd2, hV, "Vd2") || !up(J.dWd1, hW, "Wd1") || !up(J.dS, hS, "S") ||
+      !up(J.devl, T.evl, "evl"))
+    return TCE_T_ERR_HIP;
+  if (J.dtasks) (void)hipFree(J.dtasks);
+  HIPCHK(hipMalloc(&J.dtasks, J.tasks.size() * sizeof(TaskDev)), "tasks");
+  HIPCHK(hipMemcpy(J.dtasks, J.tasks.data(), J.tasks.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
+         "tasks");
+  if (J.dpart) (void)hipFree(J.dpart);
+  HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * 2 * sizeof(double)), "partials");
+  if (J.dtaskE) (void)hipFree(J.dtaskE);
+  HIPCHK(hipMalloc(&J.dtaskE, std::max<size_t>(1, J.tasks.size()) * 2 * sizeof(double)), "taskE");
+  if (!J.dcounter) HIPCHK(hipMalloc(&J.dcounter, sizeof(unsigned long long)), "counter");
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
+  J.grid = std::max(1, cus) * 8;
+  J.uploaded = true;
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt) {
+  if (!job) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (!J.uploaded) {
+    set_error("tce_t_job_run before tce_t_job_upload");
+    return TCE_T_ERR_STATE;
+  }
+  if (precision != TCE_T_FP64) {
+    set_error("unsupported precision mode");
+    return TCE_T_ERR_ARG;
+  }
+  const long ntasks = (long)J.tasks.size();
+  J.taskE.assign(2 * ntasks, 0.0);
+  if (ntasks == 0 || J.nitems == 0) {
+    if (e_bt) *e_bt = 0.0;
+    if (e_pt) *e_pt = 0.0;
+    return TCE_T_OK;
+  }
+  FusedArgs A;
+  A.T1 = J.dT1;
+  A.T2e = J.dT2e;
+  A.T2m = J.dT2m;
+  A.Vd2 = J.dVd2;
+  A.Wd1 = J.dWd1;
+  A.S = J.dS;
+  A.evl = J.devl;
+  A.tasks = J.dtasks;
+  A.ntasks = (int)ntasks;
+  A.nitems = J.nitems;
+  A.no = (int)J.T.no;
+  A.nv = (int)J.T.nv;
+  A.Ep = (int)J.Ep;
+  A.Mp = (int)J.Mp;
+  A.counter = J.dcounter;
+  A.partials = J.dpart;
+  long grid = std::min<long>(J.grid, J.nitems);
+  hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, J.stream, J.dcounter);
+  HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
+  hipLaunchKernelGGL(fused_triples, dim3((unsigned)grid), dim3(64), 0, J.stream, A);
+  HIPCHK(hipGetLastError(), "fused_triples launch");
+  HIPCHK(hipEventRecord(J.ev1, J.stream), "event");
+  hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, J.stream, J.dtasks,
+                     (int)ntasks, J.nitems, J.dpart, J.dtaskE);
+  HIPCHK(hipGetLastError(), "task_reduce launch");
+  HIPCHK(hipMemcpyAsync(J.taskE.data(), J.dtaskE, 2 * ntasks * sizeof(double), hipMemcpyDeviceToHost,
+                        J.stream),
+         "D2H");
+  HIPCHK(hipStreamSynchronize(J.stream), "fused_triples");
+  HIPCHK(hipEventElapsedTime(&J.last_ms, J.ev0, J.ev1), "elapsed");
+  double s4 = 0.0, s5 = 0.0;
+  for (long t = 0; t < ntasks; ++t) {
+    s4 += J.taskE[2 * t];
+    s5 += J.taskE[2 * t + 1];
+  }
+  if (e_bt) *e_bt = s4;
+  if (e_pt) *e_pt = s5;
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
+  if (!job || !out) return TCE_T_ERR_ARG;
+  const Job &J = *reinterpret_cast<const Job *>(job);
+  if ((long)J.tasks.size() != ntasks || (long)J.taskE.size() != 2 * ntasks) return TCE_T_ERR_STATE;
+  std::memcpy(out, J.taskE.data(), 2 * ntasks * sizeof(double));
+  return TCE_T_OK;
+}
+
+extern "C" double tce_t_job_last_kernel_ms(const tce_t_job *job) {
+  return job ? (double)reinterpret_cast<const Job *>(job)->last_ms : -1.0;
+}
+
+extern "C" int tce_t_job_destroy(tce_t_job *job) {
+  if (!job) return TCE_T_OK;
+  Job *J = reinterpret_cast<Job *>(job);
+  if (J->stream) (void)hipStreamSynchronize(J->stream);
+  double *bufs[] = {J->dT1, J->dT2e, J->dT2m, J->dVd2, J->dWd1, J->dS, J->devl, J->dpart, J->dtaskE};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (J->dtasks) (void)hipFree(J->dtasks);
+  if (J->dcounter) (void)hipFree(J->dcounter);
+  if (J->ev0) (void)hipEventDestroy(J->ev0);
+  if (J->ev1) (void)hipEventDestroy(J->ev1);
+  if (J->stream) (void)hipStreamDestroy(J->stream);
+  delete J;
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_pack_dense(const tce_t_tiles *tiles, const double *t1, const long *t1h,
+                                const double *t2, const long *t2h, const double *v2,
+                                const long *v2h, double *T1, double *T2e, double *T2m, double *Wd1,
+                                double *Vd2, double *S) {
+  Tiles T;
+  if (!T.load(tiles) || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
+  DenseHost D;
+  int rc = pack_dense(T, round8(T.nv), round8(T.no), t1, t1h, t2, t2h, v2, v2h, D);
+  if (rc != TCE_T_OK) return rc;
+  auto put = [](double *dst, const std::vector<double> &v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  put(T1, D.T1);
+  put(T2e, D.T2e);
+  put(T2m, D.T2m);
+  put(Wd1, D.Wd1);
+  put(Vd2, D.Vd2);
+  put(S, D.S);
+  return TCE_T_OK;
+}

@@ -1,0 +1,82 @@
+"""Whole-job (T) on the GPU: the host-side mirror of ccsd_t_gpu.F's surface.
+
+    job = TriplesJob(tiles)                       # ccsd_t_gpu(d_t1,k_t1_offset,...)
+    job.upload(blocks)                            # GA gets -> HBM, once
+    e_bt, e_pt = job.run()                        # energy1 = E[T], energy2 = E(T)
+
+Everything runs in libtce_t.so (HIP kernels on gfx950); this module only
+passes pointers.  There is no CPU fallback: without a built library or a
+GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DP, LP, check
+from .tiles import Blocks, TileModel
+
+FP64 = 0
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    global _initialised
+    if not _initialised:
+        check(_lib.lib().tce_t_init(device), "tce_t_init")
+        _initialised = True
+
+
+class TriplesJob:
+    def __init__(self, tiles: TileModel, device: int = 0):
+        init(device)
+        self.tiles = tiles
+        self._h = C.c_void_p()
+        check(_lib.lib().tce_t_job_create(C.byref(tiles.c()), C.byref(self._h)), "tce_t_job_create")
+        self.ntasks = int(_lib.lib().tce_t_job_num_tasks(self._h))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.lib().tce_t_job_device_bytes(self._h))
+
+    def upload(self, b: Blocks) -> None:
+        L = _lib.lib()
+        check(L.tce_t_job_upload(self._h, b.t1.ctypes.data_as(DP), b.t1h.ctypes.data_as(LP),
+                                 b.t2.ctypes.data_as(DP), b.t2h.ctypes.data_as(LP),
+                                 b.v2.ctypes.data_as(DP), b.v2h.ctypes.data_as(LP)),
+              "tce_t_job_upload")
+
+    def run(self, precision: int = FP64) -> tuple[float, float]:
+        e1, e2 = C.c_double(0.0), C.c_double(0.0)
+        check(_lib.lib().tce_t_job_run(self._h, precision, C.byref(e1), C.byref(e2)), "tce_t_job_run")
+        return e1.value, e2.value
+
+    def task_energies(self) -> np.ndarray:
+        out = np.zeros((self.ntasks, 2))
+        check(_lib.lib().tce_t_job_task_energies(self._h, out.ctypes.data_as(DP), self.ntasks),
+              "tce_t_job_task_energies")
+        return out
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return float(_lib.lib().tce_t_job_last_kernel_ms(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().tce_t_job_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

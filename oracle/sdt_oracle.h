@@ -1,0 +1,159 @@
+/*
+ * sdt_oracle.h -- CPU oracle for the TCE CCSD(T) perturbative-triples path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * never by the product library (nwchem_amd/).  It restates, in plain C, the
+ * reference algorithm of NWChem src/tce/ccsd_t for the plain FP64 path:
+ *
+ *   - tile ranges                    tce_tile.F:330-355
+ *   - canonical block offsets/hash   tce_t1_offset.F:20-56, tce_t2_offset.F:1-70,
+ *                                    tce_mo2e_offset.F:1-70, tce_hash.F:271-316
+ *   - block sorts                    sort/tce_sort.F:15-125
+ *   - restricted tile mapping        tce_restricted.F:1-71
+ *   - the 27 sd_t kernels            ccsd_t/ccsd_t_kernels_omp.F:5-1198
+ *   - the task loop + dispatch       ccsd_t/ccsd_t_gpu.F:87-241,
+ *                                    ccsd_t/ccsd_t_singles_gpu.F:101-576,
+ *                                    ccsd_t/ccsd_t_doubles_gpu.F:132-1345
+ *   - the energy                     ccsd_t/ccsd_t_gpu.F:170-200 (inline CPU
+ *                                    branch) and ccsd_t/ccsd_t_dot.F:20-125
+ *
+ * plus an independent dense "textbook" (T) evaluation used to pin the tiled
+ * dispatch (brute force over all orbital sextuples, P(9) antisymmetrisers).
+ *
+ * Pinning (what each function is checked against, see tests/test_oracle.py):
+ *   ora_tile_ranges .......... closed form sum(ranges)=n, |r_i - r_j| <= 1
+ *   ora_*_offset / ora_hash .. brute-force enumeration of canonical blocks,
+ *                              key formula round trip, sortedness of keys
+ *   ora_sort_2/4 ............. permutation round trips and numpy.transpose
+ *   ora_sd_t_{s1,d1,d2} ...... bit-exact against the reference Fortran kernels
+ *                              compiled from the reference tree (oracle/_ref)
+ *   ora_ccsd_t_gpu ........... (a) ora_brute_force on inputs with exact
+ *                              antisymmetry, (b) tile-size invariance,
+ *                              (c) t1 = 0  =>  E(T) == E[T]
+ *   ora_ccsd_t_dot ........... bit-exact against ccsd_t_dot_ from oracle/_ref
+ *   ora_brute_force .......... closed-form special cases (single nonzero
+ *                              amplitude/integral products)
+ *
+ * Integer conventions follow the reference built with -fdefault-integer-8:
+ * every INTEGER is a 64-bit long, tiles are numbered from 1 (holes 1..noab,
+ * particles noab+1..noab+nvab), arrays indexed by tile use [tile-1].
+ */
+#ifndef SDT_ORACLE_H
+#define SDT_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tile model: the subset of the tce.fh common blocks and MA arrays the path
+ * reads (k_spin, k_sym, k_range, k_offset, k_alpha, k_evl_sorted). */
+typedef struct {
+  long noab, nvab;
+  const long *spin, *sym, *range, *offset, *alpha;
+  const double *evl;
+  int restricted;
+} ora_tiles_t;
+
+/* tce_tile.F:330-355 -- split n orbitals into ceil(n/tilesize) tiles whose
+ * sizes differ by at most one.  Returns the number of tiles. */
+long ora_tile_ranges(long n, long tilesize, long *ranges);
+
+/* Canonical block enumeration.  hash (may be NULL to query the length L) gets
+ * [L, key(1..L), offset(1..L)] exactly as the reference builds it; *size gets
+ * the total number of doubles.  Return value: L. */
+long ora_t1_offset(const ora_tiles_t *t, long *hash, long *size);
+long ora_t2_offset(const ora_tiles_t *t, long *hash, long *size);
+/* tce_mo2e_offset.F.  subset=0: every block of the reference table.
+ * subset=1: only the three integral classes the (T) path reads
+ * (h p | h h), (p p | h p), (p p | h h) -- same keys, same order, offsets
+ * packed over the retained blocks. */
+long ora_v2_offset(const ora_tiles_t *t, int subset, long *hash, long *size);
+
+/* tce_hash.F:271-316: offset of key, or -1 when the key is absent. */
+long ora_hash(const long *hash, long key);
+
+void ora_sort_2(const double *unsorted, double *sorted, long a, long b,
+                long i, long j, double factor);
+void ora_sort_4(const double *unsorted, double *sorted, long a, long b, long c,
+                long d, long i, long j, long k, long l, double factor);
+
+/* The 27 kernels, CPU argument order of ccsd_t_kernels_omp.F.  n = 1..9. */
+void ora_sd_t_s1(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
+                 long p4d, double *t3, const double *t1, const double *v2);
+void ora_sd_t_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
+                 long p4d, long h7d, double *t3, const double *t2,
+                 const double *v2);
+void ora_sd_t_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
+                 long p4d, long p7d, double *t3, const double *t2,
+                 const double *v2);
+
+/* Test hook: plant a sign mistake in one built-in kernel variant
+ * (family 0 = s1, 1 = d1, 2 = d2, n = 1..9); family = -1 removes it. */
+void ora_set_mutation(int family, int n);
+
+/* ccsd_t_dot.F:20-125 (energy of one task tile). */
+void ora_ccsd_t_dot(const double *singles, const double *doubles,
+                    int restricted, long h1b, long h2b, long h3b, long p4b,
+                    long p5b, long p6b, const double *o_h1, const double *o_h2,
+                    const double *o_h3, const double *o_p4, const double *o_p5,
+                    const double *o_p6, long r_h1, long r_h2, long r_h3,
+                    long r_p4, long r_p5, long r_p6, double *energy1,
+                    double *energy2);
+
+/* Kernel back-end for the dispatch.  Function pointers take every argument by
+ * reference, exactly like the Fortran-mangled symbols:
+ *   usedevice = 0: CPU order  (h3d,h2d,h1d,p6d,p5d,p4d[,h7d|p7d],t3,a,b)
+ *   usedevice = 1: GPU order  sd_t_s1_N_cuda_(h1d,h2d,h3d,p4d,p5d,p6d,t3,a,b)
+ *                             sd_t_d1_N_cuda_(h1d,h2d,h3d,h7d,p4d,p5d,p6d,..)
+ *                             sd_t_d2_N_cuda_(h1d,h2d,h3d,p4d,p5d,p6d,p7d,..)
+ *                  and the task lifecycle of ccsd_t_gpu.F:126-220.
+ * Passing NULL for the back-end selects the built-in C kernels above. */
+typedef void (*ora_k6_fn)(long *, long *, long *, long *, long *, long *,
+                          double *, double *, double *);
+typedef void (*ora_k7_fn)(long *, long *, long *, long *, long *, long *,
+                          long *, double *, double *, double *);
+typedef struct {
+  int usedevice;
+  ora_k6_fn s1[9];
+  ora_k7_fn d1[9];
+  ora_k7_fn d2[9];
+  void (*initmemmodule)(void);
+  void (*finalizememmodule)(void);
+  void (*dev_mem_s)(long *, long *, long *, long *, long *, long *);
+  void (*dev_mem_d)(long *, long *, long *, long *, long *, long *);
+  void (*dev_release)(void);
+  void (*compute_en)(double *, double *, double *, double *, double *,
+                     double *, double *, double *, long *, long *, long *,
+                     long *, long *, long *, double *, double *);
+} ora_backend_t;
+
+/* Number of tasks (tile sextuples) the task loop of ccsd_t_gpu.F visits. */
+long ora_count_tasks(const ora_tiles_t *t);
+
+/* ccsd_t_gpu.F:2-251 with its singles/doubles drivers, single process.
+ * t1/t2/v2 are the local copies of d_t1, d_t2, d_v2 with their offset hash
+ * tables.  Tasks with index in [task_begin, task_end) are computed (pass 0,-1
+ * for all); task_e (optional) receives {E[T], E(T)} per computed task.
+ * Returns the number of tasks computed, or a negative error code. */
+long ora_ccsd_t_gpu(const ora_tiles_t *t, const double *t1, const long *t1_hash,
+                    const double *t2, const long *t2_hash, const double *v2,
+                    const long *v2_hash, const ora_backend_t *be,
+                    long task_begin, long task_end, double *energy1,
+                    double *energy2, double *task_e);
+
+/* Independent dense (T): all orbital sextuples, textbook antisymmetrisers.
+ *   T1[nv][no], T2[nv][nv][no][no]           t(p h), t(p p h h)
+ *   W[no][nv][no][no]   <m p || h h'>        (d1 integral block layout)
+ *   V[nv][nv][no][nv]   <p p' || h e>        (d2 integral block layout)
+ *   S[nv][nv][no][no]   <p p' || h h'>       (s1 integral block layout)
+ *   eo[no], ev[nv] orbital energies.
+ * e4 = E[T] = (1/36) sum t3d^2/D ; e5 = E(T) = (1/36) sum t3d (t3d+t3s)/D. */
+void ora_brute_force(long no, long nv, const double *T1, const double *T2,
+                     const double *W, const double *V, const double *S,
+                     const double *eo, const double *ev, double *e4,
+                     double *e5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""CPU tests of libtce_t.so's host half: integer bookkeeping (bit-exact against
+the oracle), the seeded generator, the upload pack (GA gets + sorts + signs ->
+dense HBM layouts) and the exported C-ABI surface.  No GPU calls here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from nwchem_amd import _lib
+from nwchem_amd import tiles as tl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DP = C.POINTER(C.c_double)
+LP = C.POINTER(C.c_long)
+
+
+def test_philox_known_answers(lib):
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    cases = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, exp in cases:
+        c = (C.c_uint * 4)(*ctr)
+        k = (C.c_uint * 2)(*key)
+        o = (C.c_uint * 4)()
+        lib.tce_t_philox4x32(c, k, o)
+        assert tuple(o) == exp
+
+
+@pytest.mark.parametrize("n,ts", [(40, 20), (320, 40), (45, 20), (8, 8), (32, 16), (7, 3), (1, 5), (97, 24)])
+def test_tile_ranges_closed_form_and_oracle(n, ts):
+    r = tl.tile_ranges(n, ts)
+    assert r.sum() == n and len(r) == -(-n // ts) and r.max() - r.min() <= 1 and r.max() <= ts
+    ro = np.zeros(len(r), np.int64)
+    oracle.lib().ora_tile_ranges(n, ts, ro.ctypes.data_as(LP))
+    assert np.array_equal(r, ro)
+
+
+SYSTEMS = [(8, 32, 8, 16), (40, 320, 20, 40), (7, 13, 3, 5), (12, 30, 5, 7)]
+
+
+@pytest.mark.parametrize("no,nv,to,tv", SYSTEMS)
+def test_hash_tables_bit_exact_vs_oracle(no, nv, to, tv):
+    tm = tl.TileModel.synthetic(no, nv, to, tv, 1)
+    ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
+    for kind, h in (("t1", tm.t1_hash()), ("t2", tm.t2_hash()), ("v2", tm.v2_hash())):
+        ho, size = ot.hash(kind, 1)
+        assert np.array_equal(h, ho), kind
+        L = int(h[0])
+        keys, offs = h[1:L + 1], h[L + 1:]
+        assert np.all(np.diff(keys) > 0)  # the bisection of tce_hash needs sorted keys
+        assert offs[0] == 0 and np.all(np.diff(offs) > 0)
+        for i in range(0, L, max(1, L // 50)):
+            assert _lib.lib().tce_t_hash(h.ctypes.data_as(LP), int(keys[i])) == offs[i]
+            assert oracle.lib().ora_hash(h.ctypes.data_as(LP), int(keys[i])) == offs[i]
+        assert _lib.lib().tce_t_hash(h.ctypes.data_as(LP), int(keys[-1]) + 1) == -1
+
+
+def test_v2_subset_is_subsequence_of_full_mo2e_table():
+    tm = tl.TileModel.synthetic(7, 13, 3, 5, 1)
+    ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
+    full, _ = ot.hash("v2", 0)
+    sub = tm.v2_hash()
+    Lf, Ls = int(full[0]), int(sub[0])
+    fk = list(full[1:Lf + 1])
+    pos = [fk.index(k) for k in sub[1:Ls + 1]]
+    assert pos == sorted(pos)
+    # (T) reads: (h p|h h), (p p|h p), (p p|h h) -- check by brute decoding
+    N = tm.noab + tm.nvab
+    for k in sub[1:Ls + 1]:
+        g2 = k % N + 1
+        g1 = (k // N) % N + 1
+        g4 = (k // N // N) % N + 1
+        g3 = k // N // N // N + 1
+        p = [g > tm.noab for g in (g3, g4, g1, g2)]
+        assert p in ([False, True, False, False], [True, True, False, True], [True, True, False, False])
+
+
+def test_generator_exact_antisymmetry_and_moments():
+    no, nv = 6, 10
+    d = tl.synth_dense(11, no, nv)
+    T2, W, V, S = d["T2"], d["W"], d["V"], d["S"]
+    assert np.array_equal(T2, -T2.transpose(1, 0, 2, 3))
+    assert np.array_equal(T2, -T2.transpose(0, 1, 3, 2))
+    assert np.array_equal(W, -W.transpose(0, 1, 3, 2))
+    assert np.array_equal(V, -V.transpose(1, 0, 2, 3))
+    assert np.array_equal(S, -S.transpose(1, 0, 2, 3)) and np.array_equal(S, -S.transpose(0, 1, 3, 2))
+    big = tl.synth_dense(12, 20, 40)
+    for name, sig in (("T1", 0.01), ("T2", 0.02), ("W", 0.05), ("V", 0.05), ("S", 0.05)):
+        x = big[name]
+        x = x[x != 0]
+        assert abs(x.std() / sig - 1) < 0.05, name
+        assert abs(x.mean()) < 0.05 * sig, name
+    evl = np.zeros(30)
+    _lib.lib().tce_t_synth_evl(3, 10, 20, evl.ctypes.data_as(DP))
+    eo, ev = evl[:10], evl[10:]
+    assert np.all(np.diff(eo) >= 0) and np.all(np.diff(ev) >= 0)
+    assert eo.min() >= -2.0 and eo.max() <= -0.3 and ev.min() >= 0.2 and ev.max() <= 4.0
+
+
+@pytest.mark.parametrize("no,nv,to,tv", [(8, 32, 8, 16), (7, 13, 3, 5), (5, 9, 2, 4)])
+def test_upload_pack_reproduces_dense_tensors(no, nv, to, tv):
+    """The GA-get half of the upload (hash lookup, tce_sort signs, tile
+    transposes) must rebuild every dense tensor element the path reads."""
+    seed = 314
+    tm = tl.TileModel.synthetic(no, nv, to, tv, seed)
+    b = tl.synth_blocks(tm, seed)
+    d = tl.synth_dense(seed, no, nv)
+    Ep, Mp = -(-nv // 8) * 8, -(-no // 8) * 8
+    T1 = np.zeros((nv, no))
+    T2e = np.zeros((no, no, nv, Ep))
+    T2m = np.zeros((no, nv, nv, Mp))
+    Wd1 = np.zeros((no, no, nv, Mp))
+    Vd2 = np.zeros((no, nv, nv, Ep))
+    S = np.zeros((no, no, nv, nv))
+    rc = _lib.lib().tce_t_pack_dense(C.byref(tm.c()), b.t1.ctypes.data_as(DP), b.t1h.ctypes.data_as(LP),
+                                     b.t2.ctypes.data_as(DP), b.t2h.ctypes.data_as(LP),
+                                     b.v2.ctypes.data_as(DP), b.v2h.ctypes.data_as(LP),
+                                     *[a.ctypes.data_as(DP) for a in (T1, T2e, T2m, Wd1, Vd2, S)])
+    assert rc == 0
+    assert np.array_equal(T1, d["T1"])
+    assert np.array_equal(T2e[..., :nv], d["T2"].transpose(2, 3, 0, 1))  # [h][h'][p][e]
+    assert np.array_equal(T2m[..., :no], d["T2"].transpose(2, 0, 1, 3))  # [h][p][p'][m]
+    assert np.array_equal(Wd1[..., :no], d["W"].transpose(2, 3, 1, 0))   # [h][h'][p][m]
+    assert np.array_equal(Vd2[..., :nv], d["V"].transpose(2, 0, 1, 3))   # [w][p][p'][e]
+    assert np.array_equal(S, d["S"].transpose(2, 3, 0, 1))               # [h][h'][p][p']
+    # padding is zero
+    assert not T2e[..., nv:].any() and not Vd2[..., nv:].any()
+    assert not T2m[..., no:].any() and not Wd1[..., no:].any()
+
+
+def test_upload_pack_reports_missing_block():
+    tm = tl.TileModel.synthetic(5, 9, 2, 4, 1)
+    b = tl.synth_blocks(tm, 1)
+    h = b.t2h.copy()
+    L = int(h[0])
+    h[1 + L // 2] += 10**9  # corrupt one key
+    rc = _lib.lib().tce_t_pack_dense(C.byref(tm.c()), b.t1.ctypes.data_as(DP), b.t1h.ctypes.data_as(LP),
+                                     b.t2.ctypes.data_as(DP), h.ctypes.data_as(LP),
+                                     b.v2.ctypes.data_as(DP), b.v2h.ctypes.data_as(LP), *[None] * 6)
+    assert rc == -4  # TCE_T_ERR_KEY
+
+
+def test_library_exports_every_header_symbol():
+    hdr = open(os.path.join(ROOT, "include", "tce_t.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    hdr = "\n".join(ln for ln in hdr.splitlines() if not ln.lstrip().startswith("#"))
+    declared = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", hdr))
+    declared -= {"if", "sizeof"}
+    exported = set(_lib.exported_symbols())
+    missing = sorted(declared - exported)
+    assert not missing, missing
+    assert len([s for s in declared if s.startswith("sd_t_")]) == 27
+
+
+def test_task_list_matches_oracle():
+    for no, nv, to, tv in SYSTEMS:
+        tm = tl.TileModel.synthetic(no, nv, to, tv, 1)
+        ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
+        assert len(tm.tasks()) == ot.count_tasks()
+    tm = tl.TileModel.synthetic(40, 320, 20, 40, 1)
+    assert len(tm.tasks()) == 480
