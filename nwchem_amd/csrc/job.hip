@@ -63,7 +63,6 @@ struct FusedArgs {
   int ntasks;
   long nitems;
   int no, nv, Ep, Mp;  // orbitals, padded contraction extents
-  unsigned long long *counter;
   double *partials;
 };
 
@@ -129,12 +128,6 @@ __device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restr
   }
 }
 
-__device__ __forceinline__ long rfl64(long v) {
-  int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffL));
-  int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
-  return ((long)hi << 32) | (unsigned)lo;
-}
-
 __device__ __forceinline__ double perm_lane(double v, int src_lane) {
   long long b = __double_as_longlong(v);
   int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b & 0xffffffffLL));
@@ -142,21 +135,16 @@ __device__ __forceinline__ double perm_lane(double v, int src_lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-__global__ void zero_counter(unsigned long long *c) {
-  if (threadIdx.x == 0) *c = 0ull;
-}
-
 __global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
   const int lane = threadIdx.x;
   const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
   const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
   const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
-  for (;;) {
-    long item = 0;
-    if (lane == 0) item = (long)atomicAdd(A.counter, 1ull);
-    item = __shfl(item, 0);
-    item = rfl64(item);
-    if (item >= A.nitems) break;
+  // Persistent loop with a static round-robin over work items: the item
+  // index is wave-uniform by construction (no value broadcast from one lane
+  // decides the loop exit), and every wave leaves after at most
+  // ceil(nitems / gridDim.x) items.
+  for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
     // locate the task (tasks sorted by item0)
     int lo = 0, hi = A.ntasks - 1;
     while (lo < hi) {
@@ -340,7 +328,6 @@ struct Job {
   double *dT1 = nullptr, *dT2e = nullptr, *dT2m = nullptr, *dVd2 = nullptr, *dWd1 = nullptr,
          *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
   TaskDev *dtasks = nullptr;
-  unsigned long long *dcounter = nullptr;
   bool uploaded = false;
   float last_ms = 0.f;
   std::vector<double> taskE;
@@ -684,7 +671,6 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * 2 * sizeof(double)), "partials");
   if (J.dtaskE) (void)hipFree(J.dtaskE);
   HIPCHK(hipMalloc(&J.dtaskE, std::max<size_t>(1, J.tasks.size()) * 2 * sizeof(double)), "taskE");
-  if (!J.dcounter) HIPCHK(hipMalloc(&J.dcounter, sizeof(unsigned long long)), "counter");
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   J.grid = std::max(1, cus) * 8;
@@ -725,10 +711,8 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
   A.nv = (int)J.T.nv;
   A.Ep = (int)J.Ep;
   A.Mp = (int)J.Mp;
-  A.counter = J.dcounter;
   A.partials = J.dpart;
   long grid = std::min<long>(J.grid, J.nitems);
-  hipLaunchKernelGGL(zero_counter, dim3(1), dim3(64), 0, J.stream, J.dcounter);
   HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
   hipLaunchKernelGGL(fused_triples, dim3((unsigned)grid), dim3(64), 0, J.stream, A);
   HIPCHK(hipGetLastError(), "fused_triples launch");
@@ -771,7 +755,6 @@ extern "C" int tce_t_job_destroy(tce_t_job *job) {
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (J->dtasks) (void)hipFree(J->dtasks);
-  if (J->dcounter) (void)hipFree(J->dcounter);
   if (J->ev0) (void)hipEventDestroy(J->ev0);
   if (J->ev1) (void)hipEventDestroy(J->ev1);
   if (J->stream) (void)hipStreamDestroy(J->stream);
