@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 (T) job on one MI355X.
+
+A step is one complete (T) job: every tile sextuple of the task loop of
+ccsd_t_gpu.F contracted (27 sd_t variants) and reduced to E[T], E(T).  Inputs
+(the canonical T1/T2/V2 blocks) are uploaded once before the timed region and
+stay resident in HBM.
+
+Default workload: no=40 occupied (tiles of 20), nv=320 virtual (tiles of 40),
+480 tasks, FP64, seeded synthetic inputs.
+
+Output: one JSON line.  `value` is algorithmic TFLOP/s,
+F_ref = sum_tasks |t3_task| * 18 * (no + nv + 1) per job (the contraction
+flops the reference's tile dispatch performs; energy excluded).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "(T) TFLOP/s & energies/s, FP64 vs split-BF16 at ≤1 µEh; serving p50/p99 latency"
+PEAK_FP64_TFLOPS = 78.6
+
+
+def f_ref(tm) -> float:
+    """Reference tile-dispatch flops of one job (contractions only)."""
+    r = tm.range
+    tot = 0
+    for (p4, p5, p6, h1, h2, h3) in tm.tasks():
+        tot += r[p4 - 1] * r[p5 - 1] * r[p6 - 1] * r[h1 - 1] * r[h2 - 1] * r[h3 - 1]
+    return float(tot) * 18.0 * (tm.no + tm.nv + 1)
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
+    or None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "fused_triples_pmc_" + workload + ".json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(tm, threads: int, calls=(("d2", 16), ("d1", 4), ("s1", 2))):
+    """The reference's own Fortran kernels (oracle/_ref/libsdt_ref_omp.so,
+    ccsd_t_kernels_omp.F built -O3 -fopenmp) timed on the host at the exact
+    per-call shapes of the job's first task: `calls` lists how many calls of
+    each family run (d2 with K = one p7 tile, d1 with K = one h7 tile), the
+    mix of the task's 72 d2 / 18 d1 / 9 s1 calls in miniature.  A full task
+    streams its t3 tile 99 times and takes minutes on the host, so the sample
+    is bounded to a few calls; the rate is F_ref flops of the sampled calls
+    over their wall time."""
+    import ctypes as C
+
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import oracle
+
+    try:
+        R = oracle.ref_lib(omp=True)
+        kind = "reference"
+    except (FileNotFoundError, OSError):
+        return None
+    p4, p5, p6, h1, h2, h3 = tm.tasks()[0]
+    r = tm.range
+    dims = [int(r[h3 - 1]), int(r[h2 - 1]), int(r[h1 - 1]), int(r[p6 - 1]), int(r[p5 - 1]), int(r[p4 - 1])]
+    n3 = int(np.prod(dims))
+    k_p = int(r[tm.noab])  # one particle tile
+    k_h = int(r[0])        # one hole tile
+    rng = np.random.default_rng(0)
+    t3 = np.zeros(n3)
+    t3 += 1e-3  # first touch outside the timed region
+    DP = C.POINTER(C.c_double)
+    flops = 0.0
+    t_tot = 0.0
+    for fam, ncalls in calls:
+        K = {"d2": k_p, "d1": k_h, "s1": 1}[fam]
+        h3d, h2d, h1d, p6d, p5d, p4d = dims
+        if fam == "s1":
+            a = rng.standard_normal(p4d * h1d)
+            b = rng.standard_normal(h3d * h2d * p6d * p5d)
+            args = [C.c_long(v) for v in dims]
+        elif fam == "d1":
+            a = rng.standard_normal(K * p4d * p5d * h1d)
+            b = rng.standard_normal(h3d * h2d * p6d * K)
+            args = [C.c_long(v) for v in dims + [K]]
+        else:
+            a = rng.standard_normal(K * p4d * h1d * h2d)
+            b = rng.standard_normal(K * h3d * p6d * p5d)
+            args = [C.c_long(v) for v in dims + [K]]
+        fn = getattr(R, f"sd_t_{fam}_1_")
+        for _ in range(ncalls):
+            t0 = time.perf_counter()
+            fn(*[C.byref(v) for v in args], t3.ctypes.data_as(DP), a.ctypes.data_as(DP),
+               b.ctypes.data_as(DP))
+            t_tot += time.perf_counter() - t0
+            flops += 2.0 * n3 * K
+    rate = flops / t_tot
+    return {
+        "value": rate / 1e12,
+        "unit": "TFLOP/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "kind": kind,
+        "sample": "reference Fortran sd_t_d2_1 x16 (K=%d), sd_t_d1_1 x4 (K=%d), sd_t_s1_1 x2 at the "
+                  "t3 tile of task 0 (%d elements), %.3e flops in %.1f s; the full job at this "
+                  "rate would take %.0f s" % (k_p, k_h, n3, flops, t_tot, f_ref(tm) / rate),
+        "seconds": t_tot,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no", type=int, default=40)
+    ap.add_argument("--nv", type=int, default=320)
+    ap.add_argument("--tile-o", type=int, default=20)
+    ap.add_argument("--tile-v", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the reference CPU kernels (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the (T) path shards by tasks across GPUs but this "
+              "build runs and reports a single MI355X (use --gpus 1).", file=sys.stderr)
+        return 2
+    if args.steps < 1:
+        print("--steps must be >= 1", file=sys.stderr)
+        return 2
+
+    from nwchem_amd import TriplesJob, tiles as tl
+
+    t0 = time.perf_counter()
+    tm = tl.TileModel.synthetic(args.no, args.nv, args.tile_o, args.tile_v, args.seed)
+    blocks = tl.synth_blocks(tm, args.seed)
+    t_gen = time.perf_counter() - t0
+    F = f_ref(tm)
+    workload = f"no{args.no}t{args.tile_o}_nv{args.nv}t{args.tile_v}"
+    print(f"[bench] {workload}: {len(tm.tasks())} tasks, F_ref={F:.4e} flops, inputs {t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+
+    job = TriplesJob(tm)
+    t0 = time.perf_counter()
+    job.upload(blocks)
+    t_up = time.perf_counter() - t0
+    print(f"[bench] upload (pack + H2D) {t_up:.1f}s, device bytes {job.device_bytes/2**30:.2f} GiB",
+          file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        job.run()
+        print(f"[bench] warmup {w} kernel {job.last_kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    kms = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        e_bt, e_pt = job.run()
+        kms.append(job.last_kernel_ms)
+        print(f"[bench] step {s} kernel {kms[-1]:.1f} ms E[T]={e_bt:.12e} E(T)={e_pt:.12e}",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    job.close()
+
+    ms_step = wall / args.steps * 1e3
+    value = F * args.steps / wall / 1e12
+    k_avg = float(np.mean(kms))
+    achieved = F / (k_avg * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload),
+                "kernel": "fused_triples", "kernel_ms": k_avg, "flops_per_launch": F}
+    cpu = None
+    if args.cpu_baseline:
+        print(f"[bench] cpu baseline on {args.cpu_threads} threads", file=sys.stderr, flush=True)
+        del blocks
+        cpu = cpu_baseline(tm, args.cpu_threads)
+        if cpu:
+            print(f"[bench] cpu baseline {cpu['value']*1e3:.2f} GFLOP/s ({cpu['seconds']:.1f}s)",
+                  file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox inputs, resident in HBM)",
+        "config": {"workload": f"(T) FP64 job, no={args.no} (tiles {args.tile_o}), nv={args.nv} "
+                               f"(tiles {args.tile_v}), {len(tm.tasks())} tasks",
+                   "no": args.no, "nv": args.nv, "tile_o": args.tile_o, "tile_v": args.tile_v,
+                   "tasks": len(tm.tasks()), "precision": "fp64"},
+        "energies_per_s": args.steps / wall,
+        "E_bt": e_bt,
+        "E_pt": e_pt,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -75,40 +75,42 @@ __device__ __forceinline__ double mfma4(double a, double b, double c) {
 // One contraction on the accumulators.  PAIR selects the two hole indices
 // the "pair" operand P carries (0: i,j  1: j,k  2: i,k); the third hole
 // indexes the "single" operand Q.  P_IS_R: P supplies the MFMA rows (A).
-// P/Q point at the lane's particle row; hole offsets are added here; the
-// contraction index runs contiguously, two values per lane per load.
+// Pb/Qb are wave-uniform pointers (operand origin of the item's particle
+// cube); pLane/qLane are the lane's small offsets (doubles) from there.
+// Hole offsets are uniform too, so every load is SGPR base + 32-bit VGPR
+// offset.  The contraction index runs contiguously, two values per lane per
+// load (lane z holds k0+2z and k0+2z+1).
 template <int PAIR, bool P_IS_R>
-__device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restrict__ P, long pS,
-                                         const double *__restrict__ Q, long qS, const int (&hI)[4],
+__device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restrict__ Pb,
+                                         unsigned pLane, long pS, const double *__restrict__ Qb,
+                                         unsigned qLane, long qS, const int (&hI)[4],
                                          const int (&hJ)[4], const int (&hK)[4], int no, int Kp,
                                          int z, double sign) {
-  // hole offsets are wave-uniform; the lane's contraction slot goes into
-  // the base pointers
-  long po[4][4], qo[4];
+  const char *pp[4][4], *qp[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int h1 = PAIR == 1 ? hJ[u] : hI[u];
       int h2 = PAIR == 0 ? hJ[v] : hK[v];
-      po[u][v] = ((long)h1 * no + h2) * pS;
+      pp[u][v] = (const char *)(Pb + ((long)h1 * no + h2) * pS);
     }
 #pragma unroll
   for (int w = 0; w < 4; ++w) {
     int hs = PAIR == 0 ? hK[w] : (PAIR == 1 ? hI[w] : hJ[w]);
-    qo[w] = (long)hs * qS;
+    qp[w] = (const char *)(Qb + (long)hs * qS);
   }
-  P += 2 * z;
-  Q += 2 * z;
+  const unsigned po = (pLane + 2u * z) * 8u, qo = (qLane + 2u * z) * 8u;
   for (int k0 = 0; k0 < Kp; k0 += 8) {
+    const unsigned kb = (unsigned)k0 * 8u;
     double2 pf[4][4], qf[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) pf[u][v] = *(const double2 *)(P + po[u][v] + k0);
+      for (int v = 0; v < 4; ++v) pf[u][v] = *(const double2 *)(pp[u][v] + (po + kb));
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
-      double2 t = *(const double2 *)(Q + qo[w] + k0);
+      double2 t = *(const double2 *)(qp[w] + (qo + kb));
       qf[w].x = sign * t.x;
       qf[w].y = sign * t.y;
     }
@@ -135,7 +137,7 @@ __device__ __forceinline__ double perm_lane(double v, int src_lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-__global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
+__global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
   const int lane = threadIdx.x;
   const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
   const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
@@ -170,10 +172,13 @@ __global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
       hJ[u] = T.oh2 + min(4 * bj + u, T.r2 - 1);
       hK[u] = T.oh3 + min(4 * bk + u, T.r3 - 1);
     }
-    // lane particle indices (global particle space, clamped into the tile)
-    const long ax = T.op4 + min(a0 + x, T.r4 - 1), ay = T.op4 + min(a0 + y, T.r4 - 1);
-    const long bx = T.op5 + min(b0 + x, T.r5 - 1);
-    const long cx = T.op6 + min(c0 + x, T.r6 - 1), cy = T.op6 + min(c0 + y, T.r6 - 1);
+    // Cube origin (global particle indices, wave-uniform) and the lane's
+    // offsets from it, clamped into the tile (lanes past a ragged tile edge
+    // compute duplicates that the energy mask drops).
+    const long A0 = T.op4 + a0, B0 = T.op5 + b0, C0 = T.op6 + c0;
+    const unsigned dxa = min(x, T.r4 - 1 - a0), dya = min(y, T.r4 - 1 - a0);
+    const unsigned dxb = min(x, T.r5 - 1 - b0);
+    const unsigned dxc = min(x, T.r6 - 1 - c0), dyc = min(y, T.r6 - 1 - c0);
 
     acc_t acc[64];
 #pragma unroll
@@ -183,23 +188,40 @@ __global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
     const long sW = nvM;               // stride of (h*no+h') in Wd1
     const long sV = (long)nv * nvE;    // stride of w in Vd2
     const long sT2m = (long)nv * nvM;  // stride of h in T2m
+    const unsigned uEp = Ep, uMp = Mp, unv = nv;
     // ---- phase 1: lone particle a or b
     // d2 N=1..3: rows a from t(a e h h'), cols b / blocks c from <b c||h e>
-    contract<0, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
-    contract<1, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
-    contract<2, true>(acc, A.T2e + ax * Ep, sT2e, A.Vd2 + (bx * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    {
+      const double *P = A.T2e + A0 * Ep, *Q = A.Vd2 + (B0 * nv + C0) * Ep;
+      const unsigned pl = dxa * uEp, ql = (dxb * unv + dyc) * uEp;
+      contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
+      contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
+      contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    }
     // d2 N=4..6: rows a / blocks c from <a c||h e>, cols b from t(b e h h')
-    contract<0, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
-    contract<1, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
-    contract<2, false>(acc, A.T2e + bx * Ep, sT2e, A.Vd2 + (ax * nv + cy) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    {
+      const double *P = A.T2e + B0 * Ep, *Q = A.Vd2 + (A0 * nv + C0) * Ep;
+      const unsigned pl = dxb * uEp, ql = (dxa * unv + dyc) * uEp;
+      contract<0, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
+      contract<1, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
+      contract<2, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
+    }
     // d1 N=4..6: rows a from <m a||h h'>, cols b / blocks c from t(b c h m)
-    contract<1, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-    contract<2, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-    contract<0, true>(acc, A.Wd1 + ax * Mp, sW, A.T2m + (bx * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    {
+      const double *P = A.Wd1 + A0 * Mp, *Q = A.T2m + (B0 * nv + C0) * Mp;
+      const unsigned pl = dxa * uMp, ql = (dxb * unv + dyc) * uMp;
+      contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+      contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+      contract<0, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    }
     // d1 N=7..9: rows a / blocks c from t(a c h m), cols b from <m b||h h'>
-    contract<1, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-    contract<2, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-    contract<0, false>(acc, A.Wd1 + bx * Mp, sW, A.T2m + (ax * nv + cy) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    {
+      const double *P = A.Wd1 + B0 * Mp, *Q = A.T2m + (A0 * nv + C0) * Mp;
+      const unsigned pl = dxb * uMp, ql = (dxa * unv + dyc) * uMp;
+      contract<1, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+      contract<2, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+      contract<0, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+    }
     // ---- switch layouts: element (a,b,c) moves from lane b+4c+16a to b+4a+16c
     {
       const int src = x + 4 * z + 16 * y;
@@ -208,13 +230,21 @@ __global__ void __launch_bounds__(64) fused_triples(FusedArgs A) {
     }
     // ---- phase 2: lone particle c
     // d2 N=7..9: rows c from t(c e h h'), cols b / blocks a from <a b||h e>
-    contract<0, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
-    contract<1, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, -1.0);
-    contract<2, true>(acc, A.T2e + cx * Ep, sT2e, A.Vd2 + (ay * nv + bx) * Ep, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    {
+      const double *P = A.T2e + C0 * Ep, *Q = A.Vd2 + (A0 * nv + B0) * Ep;
+      const unsigned pl = dxc * uEp, ql = (dya * unv + dxb) * uEp;
+      contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
+      contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
+      contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
+    }
     // d1 N=1..3: rows c from <m c||h h'>, cols b / blocks a from t(a b h m)
-    contract<1, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-    contract<2, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-    contract<0, true>(acc, A.Wd1 + cx * Mp, sW, A.T2m + (ay * nv + bx) * Mp, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    {
+      const double *P = A.Wd1 + C0 * Mp, *Q = A.T2m + (A0 * nv + B0) * Mp;
+      const unsigned pl = dxc * uMp, ql = (dya * unv + dxb) * uMp;
+      contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+      contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
+      contract<0, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
+    }
 
     // ---- singles, denominators, energy (layout 2: a=y, b=x, c=z)
     double e4 = 0.0, e5 = 0.0;

@@ -1,0 +1,63 @@
+! tce_t_fdemo.F90 -- a Fortran caller of the whole-job API: builds the tile
+! model the way tce_tile.F does (holes first, then particles; C1, one spin),
+! fills T1/T2/V2 canonical blocks with the seeded generator, uploads them with
+! their hash tables and prints E[T], E(T).
+! Usage: tce_t_fdemo [no nv tile_o tile_v]   (default 8 32 8 16)
+program tce_t_fdemo
+  use iso_c_binding
+  use tce_t_mi355x
+  implicit none
+  integer(c_long) :: no, nv, to, tv, noab, nvab, n, i, l1, l2, lv, s1, s2, sv
+  integer(c_long), allocatable, target :: range(:), offset(:), spin(:), sym(:), alpha(:)
+  integer(c_long), allocatable :: h1(:), h2(:), hv(:)
+  real(c_double), allocatable, target :: evl(:)
+  real(c_double), allocatable :: t1(:), t2(:), v2(:)
+  type(tce_t_tiles) :: tiles
+  type(c_ptr) :: job
+  real(c_double) :: e1, e2
+  integer(c_int) :: rc
+  character(len=32) :: arg
+  integer(c_long_long), parameter :: seed = 1234_c_long_long
+
+  no = 8; nv = 32; to = 8; tv = 16
+  if (command_argument_count() >= 4) then
+    call get_command_argument(1, arg); read(arg, *) no
+    call get_command_argument(2, arg); read(arg, *) nv
+    call get_command_argument(3, arg); read(arg, *) to
+    call get_command_argument(4, arg); read(arg, *) tv
+  end if
+  allocate(range(no + nv))
+  noab = tce_t_tile_ranges(no, to, range)
+  nvab = tce_t_tile_ranges(nv, tv, range(noab + 1:))
+  n = noab + nvab
+  allocate(offset(n), spin(n), sym(n), alpha(n), evl(no + nv))
+  offset(1) = 0
+  do i = 2, n
+    offset(i) = offset(i - 1) + range(i - 1)
+  end do
+  spin = 1; sym = 0
+  do i = 1, n
+    alpha(i) = i
+  end do
+  rc = tce_t_synth_evl(seed, no, nv, evl)
+  tiles = tce_t_tiles(noab, nvab, c_loc(spin), c_loc(sym), c_loc(range), c_loc(offset), &
+                      c_loc(alpha), c_loc(evl), 0)
+  allocate(h1(2 * noab * nvab + 1), h2(2 * (noab * nvab)**2 + 1), hv(2 * n**4 + 1))
+  l1 = tce_t_t1_offset(tiles, h1, s1)
+  l2 = tce_t_t2_offset(tiles, h2, s2)
+  lv = tce_t_v2_offset(tiles, hv, sv)
+  allocate(t1(s1), t2(s2), v2(sv))
+  rc = tce_t_synth_blocks(tiles, seed, t1, h1, t2, h2, v2, hv)
+  if (rc /= TCE_T_OK) stop 2
+  rc = tce_t_init(0_c_int)
+  if (rc /= TCE_T_OK) stop 3
+  rc = tce_t_job_create(tiles, job)
+  if (rc /= TCE_T_OK) stop 4
+  rc = tce_t_job_upload(job, t1, h1, t2, h2, v2, hv)
+  if (rc /= TCE_T_OK) stop 5
+  rc = tce_t_job_run(job, TCE_T_FP64, e1, e2)
+  if (rc /= TCE_T_OK) stop 6
+  write(*, '(A,I0,A,I0,A,ES24.16,A,ES24.16)') 'tasks=', tce_t_job_num_tasks(job), &
+      ' blocks(t2)=', l2, ' E[T]=', e1, ' E(T)=', e2
+  rc = tce_t_job_destroy(job)
+end program tce_t_fdemo

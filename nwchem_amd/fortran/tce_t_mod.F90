@@ -1,0 +1,124 @@
+! tce_t_mod.F90 -- ISO_C_BINDING interface of libtce_t.so for Fortran callers.
+!
+! The per-call kernel surface (sd_t_s1_N_cuda, sd_t_d1_N_cuda, sd_t_d2_N_cuda,
+! compute_en, dev_mem_s/d, dev_release, initmemmodule, finalizememmodule,
+! check_device, device_init) keeps the reference's implicit-interface calls of
+! src/tce/ccsd_t/ccsd_t_gpu.F, ccsd_t_singles_gpu.F and ccsd_t_doubles_gpu.F:
+! those files link against libtce_t.so unchanged.  This module adds explicit
+! interfaces for the whole-job entry points (one device-resident job replacing
+! the tile-sextuple loop of ccsd_t_gpu.F) and the host helpers.
+!
+! All INTEGERs at the boundary are integer(c_long), matching NWChem's
+! -fdefault-integer-8 build.
+module tce_t_mi355x
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: TCE_T_OK = 0
+  integer(c_int), parameter :: TCE_T_FP64 = 0
+
+  ! Mirror of tce_t_tiles: the tce.fh / MA arrays the (T) path reads.
+  type, bind(C) :: tce_t_tiles
+    integer(c_long) :: noab, nvab
+    type(c_ptr) :: spin, sym, range, offset, alpha
+    type(c_ptr) :: evl
+    integer(c_int) :: restricted
+  end type tce_t_tiles
+
+  interface
+    integer(c_int) function tce_t_init(device) bind(C, name="tce_t_init")
+      import :: c_int
+      integer(c_int), value :: device
+    end function
+
+    integer(c_int) function tce_t_job_create(tiles, job) bind(C, name="tce_t_job_create")
+      import :: c_int, c_ptr, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      type(c_ptr), intent(out) :: job
+    end function
+
+    integer(c_long) function tce_t_job_num_tasks(job) bind(C, name="tce_t_job_num_tasks")
+      import :: c_long, c_ptr
+      type(c_ptr), value :: job
+    end function
+
+    integer(c_long) function tce_t_job_device_bytes(job) bind(C, name="tce_t_job_device_bytes")
+      import :: c_long, c_ptr
+      type(c_ptr), value :: job
+    end function
+
+    ! d_t1/k_t1_offset, d_t2/k_t2_offset, d_v2/k_v2_offset of ccsd_t_gpu.F
+    integer(c_int) function tce_t_job_upload(job, t1, k_t1_offset, t2, k_t2_offset, &
+                                              v2, k_v2_offset) bind(C, name="tce_t_job_upload")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      real(c_double), intent(in) :: t1(*), t2(*), v2(*)
+      integer(c_long), intent(in) :: k_t1_offset(*), k_t2_offset(*), k_v2_offset(*)
+    end function
+
+    ! energy1 = E[T], energy2 = E(T)
+    integer(c_int) function tce_t_job_run(job, precision, energy1, energy2) &
+        bind(C, name="tce_t_job_run")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: job
+      integer(c_int), value :: precision
+      real(c_double), intent(out) :: energy1, energy2
+    end function
+
+    integer(c_int) function tce_t_job_task_energies(job, out, ntasks) &
+        bind(C, name="tce_t_job_task_energies")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      real(c_double), intent(out) :: out(*)
+      integer(c_long), value :: ntasks
+    end function
+
+    integer(c_int) function tce_t_job_destroy(job) bind(C, name="tce_t_job_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: job
+    end function
+
+    integer(c_long) function tce_t_tile_ranges(n, tilesize, ranges) bind(C, name="tce_t_tile_ranges")
+      import :: c_long
+      integer(c_long), value :: n, tilesize
+      integer(c_long), intent(out) :: ranges(*)
+    end function
+
+    integer(c_long) function tce_t_t1_offset(tiles, hash, size) bind(C, name="tce_t_t1_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    integer(c_long) function tce_t_t2_offset(tiles, hash, size) bind(C, name="tce_t_t2_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    integer(c_long) function tce_t_v2_offset(tiles, hash, size) bind(C, name="tce_t_v2_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    integer(c_int) function tce_t_synth_evl(seed, no, nv, evl) bind(C, name="tce_t_synth_evl")
+      import :: c_int, c_long, c_long_long, c_double
+      integer(c_long_long), value :: seed
+      integer(c_long), value :: no, nv
+      real(c_double), intent(out) :: evl(*)
+    end function
+
+    integer(c_int) function tce_t_synth_blocks(tiles, seed, t1, t1h, t2, t2h, v2, v2h) &
+        bind(C, name="tce_t_synth_blocks")
+      import :: c_int, c_long, c_long_long, c_double, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long_long), value :: seed
+      real(c_double), intent(out) :: t1(*), t2(*), v2(*)
+      integer(c_long), intent(in) :: t1h(*), t2h(*), v2h(*)
+    end function
+  end interface
+end module tce_t_mi355x

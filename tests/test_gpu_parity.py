@@ -100,3 +100,22 @@ def test_legacy_surface_matches_oracle(case):
     got = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be)
     assert L.tce_t_legacy_status() == 0
     assert_parity(got[:2], got[2], ref)
+
+
+def test_fortran_caller_matches_oracle():
+    """The ISO_C_BINDING module driven from Fortran (nwchem_amd/fortran)."""
+    import os
+    import re
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "nwchem_amd", "fortran", "tce_t_fdemo")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran demo not built")
+    out = subprocess.run([exe, "8", "32", "8", "16"], check=True, capture_output=True, text=True,
+                         timeout=120).stdout
+    m = re.search(r"E\[T\]=\s*(\S+)\s+E\(T\)=\s*(\S+)", out)
+    assert m, out
+    tm, b, ot = system(8, 32, 8, 16, 1234)
+    e1, e2, _ = oracle_energies(ot, b)
+    assert abs(float(m.group(1)) - e1) <= tol(e1) and abs(float(m.group(2)) - e2) <= tol(e2)

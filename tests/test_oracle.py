@@ -198,3 +198,36 @@ def test_sorts_match_numpy_transpose():
     out = np.zeros_like(y)
     L.ora_sort_2(_dp(y), _dp(out), a, b, 2, 1, 1.0)
     assert np.array_equal(out, y.reshape(a, b).T.reshape(-1))
+
+
+def test_brute_force_closed_form_single_triple():
+    """no = nv = 3: one occupied and one virtual triple, so E = t3^2 / D with
+    t3 the (abc,ijk) = (012,012) element.  With only t(0 2 0 1) = t (and its
+    antisymmetric images) and <1 2||2 2> = v (and images), the d2 term gives
+    t3d = -t v; with t1(0 0) = s and <1 2||1 2> = w the singles give
+    t3s = s w; d1 integrals are zero."""
+    no = nv = 3
+    t, v, s, w = 0.3, -0.7, 0.11, 0.9
+    T1 = np.zeros((nv, no)); T1[0, 0] = s
+    T2 = np.zeros((nv, nv, no, no))
+    T2[0, 2, 0, 1], T2[2, 0, 0, 1], T2[0, 2, 1, 0], T2[2, 0, 1, 0] = t, -t, -t, t
+    W = np.zeros((no, nv, no, no))
+    V = np.zeros((nv, nv, no, nv)); V[1, 2, 2, 2], V[2, 1, 2, 2] = v, -v
+    S = np.zeros((nv, nv, no, no))
+    S[1, 2, 1, 2], S[2, 1, 1, 2], S[1, 2, 2, 1], S[2, 1, 2, 1] = w, -w, -w, w
+    eo = np.array([-1.3, -0.9, -0.6]); ev = np.array([0.4, 1.1, 2.5])
+    D = eo.sum() - ev.sum()
+    e4, e5 = oracle.brute_force(no, nv, T1, T2, W, V, S, eo, ev)
+    t3d, t3s = -t * v, s * w
+    assert abs(e4 - t3d * t3d / D) <= 1e-15
+    assert abs(e5 - t3d * (t3d + t3s) / D) <= 1e-15
+    # the d1 term on its own: <m c||j k> with m=0: Y = -t(a b i m) v(m c j k)
+    T2b = np.zeros_like(T2)
+    T2b[0, 1, 0, 0] = 0.0  # diagonal hole pair vanishes by antisymmetry
+    T2b[0, 1, 0, 1], T2b[1, 0, 0, 1], T2b[0, 1, 1, 0], T2b[1, 0, 1, 0] = t, -t, -t, t
+    Wb = np.zeros_like(W)
+    Wb[1, 2, 1, 2], Wb[1, 2, 2, 1] = v, -v  # <m=1 c=2||j=1 k=2>
+    e4b, _ = oracle.brute_force(no, nv, np.zeros_like(T1), T2b, Wb, np.zeros_like(V), np.zeros_like(S), eo, ev)
+    # t3(012,012): P(c/ab)P(i/jk) of -t(a b i m) v(m c j k); only the identity
+    # term survives with i=0, m=1: -t(0 1 0 1) v(1 2 1 2) = -t v
+    assert abs(e4b - (t * v) ** 2 / D) <= 1e-15
