@@ -120,8 +120,8 @@ def cpu_baseline(tm, threads: int, calls=(("d2", 16), ("d1", 4), ("s1", 2))):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no", type=int, default=40)
     ap.add_argument("--nv", type=int, default=320)
     ap.add_argument("--tile-o", type=int, default=20)

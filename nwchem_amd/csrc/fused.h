@@ -1,0 +1,178 @@
+// fused.h -- device-side pieces shared by the fused (T) kernels.
+//
+// A work item is (task, particle box, 4x4x4 hole box) of one task's t3 tile.
+// Every wavefront holds a 4x4x4 particle cube by 4x4x4 hole box (4096 t3
+// elements) in 64 FP64 accumulators:
+//   lane = x + 4y + 16z, register r = 16*ii + 4*jj + kk (hole offsets)
+//   phase 1: (a,b,c) = (z, x, y)    rows a, cols b, blocks c
+//   phase 2: (a,b,c) = (y, x, z)    rows c, cols b, blocks a
+// v_mfma_f64_4x4x4_4b reads A[i][k] of block q from lane i+4q+16k, B[k][j]
+// from lane j+4q+16k and writes D[i][j] to lane j+4q+16i, so a contraction
+// runs on the accumulators iff its row index and column index sit on
+// different operands; the block index may sit on either (the other operand
+// is replicated).  Phase 1 carries the 12 contractions whose lone particle
+// is a or b (d2 N=1..6, d1 N=4..9), phase 2 the 6 whose lone particle is c
+// (d2 N=7..9, d1 N=1..3); one lane permutation switches the layouts.
+//
+// The 27 reference variants in task indices (a,b,c = p4,p5,p6 tiles,
+// i,j,k = h1,h2,h3 tiles; t = t2 with the tce_sort signs folded in):
+//   d2 N=1..3: -t(a e i j)<b c||k e>, -t(a e j k)<b c||i e>, +t(a e i k)<b c||j e>
+//   d2 N=4..6: +t(b e i j)<a c||k e>, +t(b e j k)<a c||i e>, -t(b e i k)<a c||j e>
+//   d2 N=7..9: -t(c e i j)<a b||k e>, -t(c e j k)<a b||i e>, +t(c e i k)<a b||j e>
+//   d1 N=1..3: -t(a b i m)<m c||j k>, +t(a b j m)<m c||i k>, -t(a b k m)<m c||i j>
+//   d1 N=4..6: -t(b c i m)<m a||j k>, +t(b c j m)<m a||i k>, -t(b c k m)<m a||i j>
+//   d1 N=7..9: +t(a c i m)<m b||j k>, -t(a c j m)<m b||i k>, +t(a c k m)<m b||i j>
+//   s1 N=1..9: +t(a i)<b c||j k>, -t(a j)<b c||i k>, +t(a k)<b c||i j>,
+//              -t(b i)<a c||j k>, +t(b j)<a c||i k>, -t(b k)<a c||i j>,
+//              +t(c i)<a b||j k>, -t(c j)<a b||i k>, +t(c k)<a b||i j>
+// (ccsd_t_kernels_omp.F layouts and signs under the row tables of
+// ccsd_t_singles_gpu.F / ccsd_t_doubles_gpu.F).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tce_t {
+
+struct TaskDev {
+  int op4, op5, op6;  // particle-space offsets of the task's particle tiles
+  int oh1, oh2, oh3;  // hole offsets
+  int r4, r5, r6, r1, r2, r3;
+  int nb4, nb5, nb6, nb1, nb2, nb3;  // boxes per index (particle box edge pbox, hole 4)
+  long item0;                        // first work item of the task
+  double factor;
+};
+
+struct FusedArgs {
+  const double *T1, *T2e, *T2m, *Vd2, *Wd1, *S, *evl;
+  const TaskDev *tasks;
+  int ntasks;
+  long nitems;
+  int no, nv, Ep, Mp;  // orbitals, padded contraction extents
+  double *partials;    // {E[T], E(T)} per wave of each item
+};
+
+typedef double acc_t;
+
+__device__ __forceinline__ double mfma4(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double perm_lane(double v, int src_lane) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b & 0xffffffffLL));
+  int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Accumulator index of (pair (u,v), single w) for the three pair choices
+// (0: pair (i,j) single k, 1: pair (j,k) single i, 2: pair (i,k) single j).
+template <int PAIR>
+__device__ __forceinline__ constexpr int acc_index(int u, int v, int w) {
+  return PAIR == 0 ? (u * 4 + v) * 4 + w : (PAIR == 1 ? (w * 4 + u) * 4 + v : (u * 4 + w) * 4 + v);
+}
+
+// 64 MFMAs of one contraction step: P fragments (pairs) x Q fragments
+// (singles), two K values per lane held in .x / .y.
+template <int PAIR, bool P_IS_R>
+__device__ __forceinline__ void mma_step(acc_t (&acc)[64], const double2 (&pf)[4][4],
+                                         const double2 (&qf)[4]) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const int r = acc_index<PAIR>(u, v, w);
+          const double pv = t ? pf[u][v].y : pf[u][v].x;
+          const double qv = t ? qf[w].y : qf[w].x;
+          acc[r] = P_IS_R ? mfma4(pv, qv, acc[r]) : mfma4(qv, pv, acc[r]);
+        }
+}
+
+// Move the accumulators from the phase-1 to the phase-2 layout: element
+// (a,b,c) goes from lane b+4c+16a to lane b+4a+16c.
+__device__ __forceinline__ void switch_layout(acc_t (&acc)[64], int x, int y, int z) {
+  const int src = x + 4 * z + 16 * y;
+#pragma unroll
+  for (int r = 0; r < 64; ++r) acc[r] = perm_lane(acc[r], src);
+}
+
+// Singles, denominators and the energy of one wave's 4096 elements (phase-2
+// layout).  la/lb/lc: this lane's particle indices within the task's tiles;
+// i0/j0/k0: first hole of the box within the tiles; hI/hJ/hK: global hole
+// indices (clamped).  Elements past a ragged tile edge are skipped.
+// denom = d * factor / (e_h1 + e_h2 + e_h3 + denom_0) as ccsd_t_dot.F:107.
+__device__ __forceinline__ void item_energy(const acc_t (&acc)[64], const FusedArgs &A,
+                                            const TaskDev &T, int la, int lb, int lc, int i0,
+                                            int j0, int k0, const int (&hI)[4], const int (&hJ)[4],
+                                            const int (&hK)[4], double &e4, double &e5) {
+  e4 = 0.0;
+  e5 = 0.0;
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  const int no = A.no, nv = A.nv;
+  const long pa = T.op4 + la, pb = T.op5 + lb, pc = T.op6 + lc;
+  const double ea = A.evl[no + pa], eb = A.evl[no + pb], ec = A.evl[no + pc];
+  const double denom_0 = -(ea + eb + ec);
+  const double *T1 = A.T1, *S = A.S;
+  const long sSh = (long)nv * nv;  // stride of (h*no+h') in S
+  const long pbc = pb * nv + pc, pac = pa * nv + pc, pab = pa * nv + pb;
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    if (i0 + ii >= T.r1) continue;
+    const int hi = hI[ii];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      if (j0 + jj >= T.r2) continue;
+      const int hj = hJ[jj];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        if (k0 + kk >= T.r3) continue;
+        const int hk = hK[kk];
+        const double d = acc[ii * 16 + jj * 4 + kk];
+        const long jk = ((long)hj * no + hk) * sSh, ik = ((long)hi * no + hk) * sSh,
+                   ij = ((long)hi * no + hj) * sSh;
+        double s = T1[pa * no + hi] * S[jk + pbc];
+        s -= T1[pa * no + hj] * S[ik + pbc];
+        s += T1[pa * no + hk] * S[ij + pbc];
+        s -= T1[pb * no + hi] * S[jk + pac];
+        s += T1[pb * no + hj] * S[ik + pac];
+        s -= T1[pb * no + hk] * S[ij + pac];
+        s += T1[pc * no + hi] * S[jk + pab];
+        s -= T1[pc * no + hj] * S[ik + pab];
+        s += T1[pc * no + hk] * S[ij + pab];
+        const double den = d * T.factor / (A.evl[hi] + A.evl[hj] + A.evl[hk] + denom_0);
+        e4 += den * d;
+        e5 += den * (d + s);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_sum(double &e4, double &e5) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e4 += __shfl_xor(e4, off);
+    e5 += __shfl_xor(e5, off);
+  }
+}
+
+// Locate the task of a work item (tasks sorted by item0) -- uniform.
+__device__ __forceinline__ int find_task(const TaskDev *tasks, int ntasks, long item) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].item0 <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Host launchers (one file per kernel family).
+int launch_fused_wave(const FusedArgs &A, int grid, hipStream_t s);
+int launch_fused_wg(const FusedArgs &A, int grid, hipStream_t s);
+
+}  // namespace tce_t

@@ -43,269 +43,21 @@
 #include <thread>
 #include <vector>
 
+#include "fused.h"
 #include "internal.h"
 
 namespace tce_t {
 namespace {
 
-struct TaskDev {
-  int op4, op5, op6;  // particle-space offsets of the task's particle tiles
-  int oh1, oh2, oh3;  // hole offsets
-  int r4, r5, r6, r1, r2, r3;
-  int nb4, nb5, nb6, nb1, nb2, nb3;  // 4-wide boxes per index
-  long item0;                        // first work item of the task
-  double factor;
-};
-
-struct FusedArgs {
-  const double *T1, *T2e, *T2m, *Vd2, *Wd1, *S, *evl;
-  const TaskDev *tasks;
-  int ntasks;
-  long nitems;
-  int no, nv, Ep, Mp;  // orbitals, padded contraction extents
-  double *partials;
-};
-
-typedef double acc_t;
-
-__device__ __forceinline__ double mfma4(double a, double b, double c) {
-  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
-}
-
-// One contraction on the accumulators.  PAIR selects the two hole indices
-// the "pair" operand P carries (0: i,j  1: j,k  2: i,k); the third hole
-// indexes the "single" operand Q.  P_IS_R: P supplies the MFMA rows (A).
-// Pb/Qb are wave-uniform pointers (operand origin of the item's particle
-// cube); pLane/qLane are the lane's small offsets (doubles) from there.
-// Hole offsets are uniform too, so every load is SGPR base + 32-bit VGPR
-// offset.  The contraction index runs contiguously, two values per lane per
-// load (lane z holds k0+2z and k0+2z+1).
-template <int PAIR, bool P_IS_R>
-__device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restrict__ Pb,
-                                         unsigned pLane, long pS, const double *__restrict__ Qb,
-                                         unsigned qLane, long qS, const int (&hI)[4],
-                                         const int (&hJ)[4], const int (&hK)[4], int no, int Kp,
-                                         int z, double sign) {
-  const char *pp[4][4], *qp[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      int h1 = PAIR == 1 ? hJ[u] : hI[u];
-      int h2 = PAIR == 0 ? hJ[v] : hK[v];
-      pp[u][v] = (const char *)(Pb + ((long)h1 * no + h2) * pS);
-    }
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    int hs = PAIR == 0 ? hK[w] : (PAIR == 1 ? hI[w] : hJ[w]);
-    qp[w] = (const char *)(Qb + (long)hs * qS);
-  }
-  const unsigned po = (pLane + 2u * z) * 8u, qo = (qLane + 2u * z) * 8u;
-  for (int k0 = 0; k0 < Kp; k0 += 8) {
-    const unsigned kb = (unsigned)k0 * 8u;
-    double2 pf[4][4], qf[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) pf[u][v] = *(const double2 *)(pp[u][v] + (po + kb));
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      double2 t = *(const double2 *)(qp[w] + (qo + kb));
-      qf[w].x = sign * t.x;
-      qf[w].y = sign * t.y;
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-          for (int w = 0; w < 4; ++w) {
-            int r = PAIR == 0 ? (u * 4 + v) * 4 + w : (PAIR == 1 ? (w * 4 + u) * 4 + v : (u * 4 + w) * 4 + v);
-            double pv = t ? pf[u][v].y : pf[u][v].x;
-            double qv = t ? qf[w].y : qf[w].x;
-            acc[r] = P_IS_R ? mfma4(pv, qv, acc[r]) : mfma4(qv, pv, acc[r]);
-          }
-  }
-}
-
-__device__ __forceinline__ double perm_lane(double v, int src_lane) {
-  long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b & 0xffffffffLL));
-  int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-__global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
-  const int lane = threadIdx.x;
-  const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
-  const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
-  const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
-  // Persistent loop with a static round-robin over work items: the item
-  // index is wave-uniform by construction (no value broadcast from one lane
-  // decides the loop exit), and every wave leaves after at most
-  // ceil(nitems / gridDim.x) items.
-  for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
-    // locate the task (tasks sorted by item0)
-    int lo = 0, hi = A.ntasks - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (A.tasks[mid].item0 <= item)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const TaskDev &T = A.tasks[lo];
-    long box = item - T.item0;
-    int bc = box % T.nb6; box /= T.nb6;
-    int bb = box % T.nb5; box /= T.nb5;
-    int ba = box % T.nb4; box /= T.nb4;
-    int bk = box % T.nb3; box /= T.nb3;
-    int bj = box % T.nb2; box /= T.nb2;
-    int bi = (int)box;
-    const int a0 = 4 * ba, b0 = 4 * bb, c0 = 4 * bc;
-    int hI[4], hJ[4], hK[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      hI[u] = T.oh1 + min(4 * bi + u, T.r1 - 1);
-      hJ[u] = T.oh2 + min(4 * bj + u, T.r2 - 1);
-      hK[u] = T.oh3 + min(4 * bk + u, T.r3 - 1);
-    }
-    // Cube origin (global particle indices, wave-uniform) and the lane's
-    // offsets from it, clamped into the tile (lanes past a ragged tile edge
-    // compute duplicates that the energy mask drops).
-    const long A0 = T.op4 + a0, B0 = T.op5 + b0, C0 = T.op6 + c0;
-    const unsigned dxa = min(x, T.r4 - 1 - a0), dya = min(y, T.r4 - 1 - a0);
-    const unsigned dxb = min(x, T.r5 - 1 - b0);
-    const unsigned dxc = min(x, T.r6 - 1 - c0), dyc = min(y, T.r6 - 1 - c0);
-
-    acc_t acc[64];
-#pragma unroll
-    for (int r = 0; r < 64; ++r) acc[r] = 0.0;
-
-    const long sT2e = nvE;             // stride of (h*no+h') in T2e
-    const long sW = nvM;               // stride of (h*no+h') in Wd1
-    const long sV = (long)nv * nvE;    // stride of w in Vd2
-    const long sT2m = (long)nv * nvM;  // stride of h in T2m
-    const unsigned uEp = Ep, uMp = Mp, unv = nv;
-    // ---- phase 1: lone particle a or b
-    // d2 N=1..3: rows a from t(a e h h'), cols b / blocks c from <b c||h e>
-    {
-      const double *P = A.T2e + A0 * Ep, *Q = A.Vd2 + (B0 * nv + C0) * Ep;
-      const unsigned pl = dxa * uEp, ql = (dxb * unv + dyc) * uEp;
-      contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
-      contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
-      contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
-    }
-    // d2 N=4..6: rows a / blocks c from <a c||h e>, cols b from t(b e h h')
-    {
-      const double *P = A.T2e + B0 * Ep, *Q = A.Vd2 + (A0 * nv + C0) * Ep;
-      const unsigned pl = dxb * uEp, ql = (dxa * unv + dyc) * uEp;
-      contract<0, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
-      contract<1, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
-      contract<2, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
-    }
-    // d1 N=4..6: rows a from <m a||h h'>, cols b / blocks c from t(b c h m)
-    {
-      const double *P = A.Wd1 + A0 * Mp, *Q = A.T2m + (B0 * nv + C0) * Mp;
-      const unsigned pl = dxa * uMp, ql = (dxb * unv + dyc) * uMp;
-      contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-      contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-      contract<0, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-    }
-    // d1 N=7..9: rows a / blocks c from t(a c h m), cols b from <m b||h h'>
-    {
-      const double *P = A.Wd1 + B0 * Mp, *Q = A.T2m + (A0 * nv + C0) * Mp;
-      const unsigned pl = dxb * uMp, ql = (dxa * unv + dyc) * uMp;
-      contract<1, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-      contract<2, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-      contract<0, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-    }
-    // ---- switch layouts: element (a,b,c) moves from lane b+4c+16a to b+4a+16c
-    {
-      const int src = x + 4 * z + 16 * y;
-#pragma unroll
-      for (int r = 0; r < 64; ++r) acc[r] = perm_lane(acc[r], src);
-    }
-    // ---- phase 2: lone particle c
-    // d2 N=7..9: rows c from t(c e h h'), cols b / blocks a from <a b||h e>
-    {
-      const double *P = A.T2e + C0 * Ep, *Q = A.Vd2 + (A0 * nv + B0) * Ep;
-      const unsigned pl = dxc * uEp, ql = (dya * unv + dxb) * uEp;
-      contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
-      contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
-      contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
-    }
-    // d1 N=1..3: rows c from <m c||h h'>, cols b / blocks a from t(a b h m)
-    {
-      const double *P = A.Wd1 + C0 * Mp, *Q = A.T2m + (A0 * nv + B0) * Mp;
-      const unsigned pl = dxc * uMp, ql = (dya * unv + dxb) * uMp;
-      contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-      contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
-      contract<0, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
-    }
-
-    // ---- singles, denominators, energy (layout 2: a=y, b=x, c=z)
-    double e4 = 0.0, e5 = 0.0;
-    const int la = a0 + y, lb = b0 + x, lc = c0 + z;
-    if (la < T.r4 && lb < T.r5 && lc < T.r6) {
-      const long pa = T.op4 + la, pb = T.op5 + lb, pc = T.op6 + lc;
-      const double ea = A.evl[no + pa], eb = A.evl[no + pb], ec = A.evl[no + pc];
-      const double denom_0 = -(ea + eb + ec);
-      const double *T1 = A.T1, *S = A.S;
-      const long sSh = (long)nv * nv;  // stride of (h*no+h') in S
-      const long pbc = pb * nv + pc, pac = pa * nv + pc, pab = pa * nv + pb;
-#pragma unroll
-      for (int ii = 0; ii < 4; ++ii) {
-        if (4 * bi + ii >= T.r1) continue;
-        const int hi = hI[ii];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          if (4 * bj + jj >= T.r2) continue;
-          const int hj = hJ[jj];
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) {
-            if (4 * bk + kk >= T.r3) continue;
-            const int hk = hK[kk];
-            const double d = acc[ii * 16 + jj * 4 + kk];
-            const long jk = ((long)hj * no + hk) * sSh, ik = ((long)hi * no + hk) * sSh,
-                       ij = ((long)hi * no + hj) * sSh;
-            double s = T1[pa * no + hi] * S[jk + pbc];
-            s -= T1[pa * no + hj] * S[ik + pbc];
-            s += T1[pa * no + hk] * S[ij + pbc];
-            s -= T1[pb * no + hi] * S[jk + pac];
-            s += T1[pb * no + hj] * S[ik + pac];
-            s -= T1[pb * no + hk] * S[ij + pac];
-            s += T1[pc * no + hi] * S[jk + pab];
-            s -= T1[pc * no + hj] * S[ik + pab];
-            s += T1[pc * no + hk] * S[ij + pab];
-            const double den = d * T.factor / (A.evl[hi] + A.evl[hj] + A.evl[hk] + denom_0);
-            e4 += den * d;
-            e5 += den * (d + s);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      e4 += __shfl_xor(e4, off);
-      e5 += __shfl_xor(e5, off);
-    }
-    if (lane == 0) {
-      A.partials[2 * item] = e4;
-      A.partials[2 * item + 1] = e5;
-    }
-  }
-}
-
-// Per-task fixed-order reduction of the item partials.
+// Per-task fixed-order reduction of the partials (ppi partials per item).
 __global__ void __launch_bounds__(256) task_reduce(const TaskDev *__restrict__ tasks, int ntasks,
-                                                   long nitems, const double *__restrict__ part,
+                                                   long nitems, int ppi,
+                                                   const double *__restrict__ part,
                                                    double *__restrict__ out) {
   const int t = blockIdx.x;
   if (t >= ntasks) return;
-  const long i0 = tasks[t].item0, i1 = (t + 1 < ntasks) ? tasks[t + 1].item0 : nitems;
+  const long i0 = tasks[t].item0 * ppi,
+             i1 = ((t + 1 < ntasks) ? tasks[t + 1].item0 : nitems) * ppi;
   double s4 = 0.0, s5 = 0.0;
   for (long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     s4 += part[2 * i];
@@ -363,6 +115,8 @@ struct Job {
   std::vector<double> taskE;
   long Ep = 0, Mp = 0;
   int grid = 0;
+  int pbox = 4;  // particle box edge of a work item: 4 (fused_wave) or 8 (fused_wg)
+  int ppi = 1;   // energy partials per work item (waves per item)
 };
 
 }  // namespace tce_t
@@ -395,6 +149,24 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
   }
   const Tiles &T = J->T;
   long noab = T.noab, nvab = T.nvab, item = 0;
+  // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
+  // shared through LDS) unless rounding the particle tiles up to multiples
+  // of 8 pads the work by more than 25% over 4-wide boxes.
+  {
+    double w4 = 0, w8 = 0;
+    for (long p = noab + 1; p <= noab + nvab; ++p) {
+      double r = (double)T.rng(p);
+      w4 += std::ceil(r / 4) * 4 / r;
+      w8 += std::ceil(r / 8) * 8 / r;
+    }
+    J->pbox = (w8 <= 1.25 * w4) ? 8 : 4;
+    if (const char *k = getenv("TCE_T_KERNEL")) {
+      if (!strcmp(k, "wave")) J->pbox = 4;
+      if (!strcmp(k, "wg")) J->pbox = 8;
+    }
+    J->ppi = J->pbox == 8 ? 8 : 1;
+  }
+  const int pb = J->pbox;
   // task loop order of ccsd_t_gpu.F:87-92
   for (long p4b = noab + 1; p4b <= noab + nvab; ++p4b)
     for (long p5b = p4b; p5b <= noab + nvab; ++p5b)
@@ -416,9 +188,9 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
               d.r1 = (int)T.rng(h1b);
               d.r2 = (int)T.rng(h2b);
               d.r3 = (int)T.rng(h3b);
-              d.nb4 = (d.r4 + 3) / 4;
-              d.nb5 = (d.r5 + 3) / 4;
-              d.nb6 = (d.r6 + 3) / 4;
+              d.nb4 = (d.r4 + pb - 1) / pb;
+              d.nb5 = (d.r5 + pb - 1) / pb;
+              d.nb6 = (d.r6 + pb - 1) / pb;
               d.nb1 = (d.r1 + 3) / 4;
               d.nb2 = (d.r2 + 3) / 4;
               d.nb3 = (d.r3 + 3) / 4;
@@ -455,7 +227,7 @@ static long dense_bytes(const Job &J) {
 extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
   if (!job) return -1;
   const Job &J = *reinterpret_cast<const Job *>(job);
-  return dense_bytes(J) + J.nitems * 16 + (long)J.tasks.size() * (sizeof(TaskDev) + 16) + 64;
+  return dense_bytes(J) + J.nitems * J.ppi * 16 + (long)J.tasks.size() * (sizeof(TaskDev) + 16) + 64;
 }
 
 namespace {
@@ -698,12 +470,13 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   HIPCHK(hipMemcpy(J.dtasks, J.tasks.data(), J.tasks.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
          "tasks");
   if (J.dpart) (void)hipFree(J.dpart);
-  HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * 2 * sizeof(double)), "partials");
+  HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * J.ppi * 2 * sizeof(double)), "partials");
   if (J.dtaskE) (void)hipFree(J.dtaskE);
   HIPCHK(hipMalloc(&J.dtaskE, std::max<size_t>(1, J.tasks.size()) * 2 * sizeof(double)), "taskE");
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
-  J.grid = std::max(1, cus) * 8;
+  // persistent grid: 8 waves per CU (2 per SIMD)
+  J.grid = std::max(1, cus) * (J.pbox == 8 ? 1 : 8);
   J.uploaded = true;
   return TCE_T_OK;
 }
@@ -744,11 +517,14 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
   A.partials = J.dpart;
   long grid = std::min<long>(J.grid, J.nitems);
   HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
-  hipLaunchKernelGGL(fused_triples, dim3((unsigned)grid), dim3(64), 0, J.stream, A);
-  HIPCHK(hipGetLastError(), "fused_triples launch");
+  if ((J.pbox == 8 ? launch_fused_wg(A, (int)grid, J.stream)
+                    : launch_fused_wave(A, (int)grid, J.stream)) != 0) {
+    set_error("fused kernel launch failed");
+    return TCE_T_ERR_HIP;
+  }
   HIPCHK(hipEventRecord(J.ev1, J.stream), "event");
   hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, J.stream, J.dtasks,
-                     (int)ntasks, J.nitems, J.dpart, J.dtaskE);
+                     (int)ntasks, J.nitems, J.ppi, J.dpart, J.dtaskE);
   HIPCHK(hipGetLastError(), "task_reduce launch");
   HIPCHK(hipMemcpyAsync(J.taskE.data(), J.dtaskE, 2 * ntasks * sizeof(double), hipMemcpyDeviceToHost,
                         J.stream),

@@ -53,12 +53,16 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("kernel", ["wave", "wg"])
 @pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
-def test_job_matches_oracle(case):
+def test_job_matches_oracle(case, kernel, monkeypatch):
+    """Both fused kernels: one wave per 4x4x4 particle cube (fused_wave) and
+    the LDS-cooperative 8-wave workgroup (fused_wg)."""
+    monkeypatch.setenv("TCE_T_KERNEL", kernel)
     no, nv, to, tv, seed, ro, rv = case
     tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
     ref = oracle_energies(ot, b)
-    print(f"case no={no} nv={nv} ranges={list(tm.range)} tasks={len(ref[2])}", flush=True)
+    print(f"case no={no} nv={nv} ranges={list(tm.range)} tasks={len(ref[2])} kernel={kernel}", flush=True)
     with TriplesJob(tm) as job:
         job.upload(b)
         e = job.run()
