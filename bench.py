@@ -39,11 +39,11 @@ def f_ref(tm) -> float:
     return float(tot) * 18.0 * (tm.no + tm.nv + 1)
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(workload: str, kernel: str):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
-    or None when no summary exists for this workload."""
-    path = os.path.join(ROOT, "profiles", "fused_triples_pmc_" + workload + ".json")
+    or None when no summary exists for this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", kernel + "_pmc_" + workload + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -51,7 +51,7 @@ def traffic_from_profiles(workload: str):
     return d.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(tm, threads: int, calls=(("d2", 16), ("d1", 4), ("s1", 2))):
+def cpu_baseline(tm, threads: int, calls=(("d2", 96), ("d1", 24), ("s1", 12))):
     """The reference's own Fortran kernels (oracle/_ref/libsdt_ref_omp.so,
     ccsd_t_kernels_omp.F built -O3 -fopenmp) timed on the host at the exact
     per-call shapes of the job's first task: `calls` lists how many calls of
@@ -166,6 +166,7 @@ def main() -> int:
         print(f"[bench] step {s} kernel {kms[-1]:.1f} ms E[T]={e_bt:.12e} E(T)={e_pt:.12e}",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    kernel = job.kernel
     job.close()
 
     ms_step = wall / args.steps * 1e3
@@ -173,8 +174,8 @@ def main() -> int:
     k_avg = float(np.mean(kms))
     achieved = F / (k_avg * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload),
-                "kernel": "fused_triples", "kernel_ms": k_avg, "flops_per_launch": F}
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload, kernel),
+                "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F}
     cpu = None
     if args.cpu_baseline:
         print(f"[bench] cpu baseline on {args.cpu_threads} threads", file=sys.stderr, flush=True)

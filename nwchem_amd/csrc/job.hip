@@ -553,6 +553,11 @@ extern "C" double tce_t_job_last_kernel_ms(const tce_t_job *job) {
   return job ? (double)reinterpret_cast<const Job *>(job)->last_ms : -1.0;
 }
 
+extern "C" int tce_t_job_kernel(const tce_t_job *job) {
+  if (!job) return TCE_T_ERR_ARG;
+  return reinterpret_cast<const Job *>(job)->pbox == 8 ? TCE_T_KERNEL_WG : TCE_T_KERNEL_WAVE;
+}
+
 extern "C" int tce_t_job_destroy(tce_t_job *job) {
   if (!job) return TCE_T_OK;
   Job *J = reinterpret_cast<Job *>(job);

@@ -64,6 +64,12 @@ class TriplesJob:
     def last_kernel_ms(self) -> float:
         return float(_lib.lib().tce_t_job_last_kernel_ms(self._h))
 
+    @property
+    def kernel(self) -> str:
+        """Name of the contraction kernel the job runs."""
+        k = int(_lib.lib().tce_t_job_kernel(self._h))
+        return {0: "fused_triples", 1: "fused_triples_wg"}[k]
+
     def close(self) -> None:
         if self._h:
             _lib.lib().tce_t_job_destroy(self._h)
