@@ -9,9 +9,13 @@ stay resident in HBM.
 Default workload: no=40 occupied (tiles of 20), nv=320 virtual (tiles of 40),
 480 tasks, FP64, seeded synthetic inputs.
 
-Output: one JSON line.  `value` is algorithmic TFLOP/s,
+Output: one JSON line.  `value` is algorithmic TFLOP/s of the FP64 job,
 F_ref = sum_tasks |t3_task| * 18 * (no + nv + 1) per job (the contraction
-flops the reference's tile dispatch performs; energy excluded).
+flops the reference's tile dispatch performs; energy excluded).  The same
+line reports the reduced-precision modes (`--modes`): the d1/d2 contractions
+in FP32 and in 3-term split-BF16, each timed the same way on the same
+resident inputs, with energies/s, the speedup over FP64 and |dE| in micro-Eh
+against the FP64 result of the run.
 """
 from __future__ import annotations
 
@@ -117,6 +121,113 @@ def cpu_baseline(tm, threads: int, calls=(("d2", 96), ("d1", 24), ("s1", 12))):
     }
 
 
+def ragged(n: int, ts: int) -> list:
+    """Tiles of ts with a ragged last tile (n = k*ts + r)."""
+    r = [ts] * (n // ts)
+    if n % ts:
+        r.append(n % ts)
+    return r
+
+
+def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
+    """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256}, occupied tiles
+    6-20, virtual tiles 16-48, ragged last tiles) in one persistent launch per
+    run (autotuned, replayed as a hipGraph), against the same jobs run one
+    after another."""
+    from nwchem_amd import TriplesBatch, TriplesJob, tiles as tl
+
+    rng = np.random.default_rng(seed)
+    jobs, F = [], 0.0
+    t0 = time.perf_counter()
+    for k in range(njobs):
+        no, nv = int(rng.integers(6, 31)), int(rng.integers(48, 257))
+        to, tv = int(rng.integers(6, 21)), int(rng.integers(16, 49))
+        tm = tl.TileModel.synthetic(no, nv, 0, 0, seed + k, ragged(no, to), ragged(nv, tv))
+        b = tl.synth_blocks(tm, seed + k)
+        j = TriplesJob(tm)
+        j.upload(b)
+        del b
+        jobs.append(j)
+        F += f_ref(tm)
+    t_prep = time.perf_counter() - t0
+    out = {"jobs": njobs, "flops_per_run": F, "prep_s": t_prep}
+    names = {0: "fp64", 1: "fp32", 2: "bf16x3"}
+    with TriplesBatch(jobs) as batch:
+        for p in modes:
+            t1 = time.perf_counter()
+            batch.run(p)  # plan: tuning runs, graph capture, first replay
+            t_plan = time.perf_counter() - t1
+            ms = []
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                batch.run(p)
+                ms.append(batch.last_ms)
+            wall = (time.perf_counter() - t1) / steps
+            seq = 0.0
+            for j in jobs:  # the same jobs one launch each
+                t2 = time.perf_counter()
+                j.run(p)
+                seq += time.perf_counter() - t2
+            out[names[p]] = {"ms_per_run": wall * 1e3, "graph_ms": float(np.mean(ms)),
+                             "energies_per_s": njobs / wall, "fp64_equiv_tflops": F / wall / 1e12,
+                             "sequential_ms": seq * 1e3, "speedup_vs_sequential": seq / wall,
+                             "plan": batch.plan(p), "plan_s": t_plan}
+            print(f"[bench] batch {names[p]}: {wall*1e3:.1f} ms per run of {njobs} jobs "
+                  f"({njobs/wall:.1f} energies/s, {F/wall/1e12:.1f} TF/s), one by one {seq*1e3:.1f} ms; "
+                  f"plan {batch.plan(p)}", file=sys.stderr, flush=True)
+    for j in jobs:
+        j.close()
+    return out
+
+
+def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -> dict:
+    """Serving: a Poisson stream (rate lam) of requests for a pool of seeded
+    systems (no in [6,40], nv in [48,320], log-uniform; tiles of 20/40 by the
+    tce_tile rule), each FP64 or split-BF16 at random; systems are uploaded
+    on first use and stay resident.  Latency = finish - submit."""
+    from nwchem_amd import TriplesServer, tiles as tl
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    t0 = time.perf_counter()
+    for k in range(npool):
+        no = int(round(np.exp(rng.uniform(np.log(6), np.log(40)))))
+        nv = int(round(np.exp(rng.uniform(np.log(48), np.log(320)))))
+        tm = tl.TileModel.synthetic(no, nv, min(no, 20), min(nv, 40), seed + 100 + k)
+        pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), f_ref(tm)))
+    t_prep = time.perf_counter() - t0
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    picks = rng.integers(0, npool, nreq)
+    precs = rng.choice([0, 2], nreq)
+    lat, F = [], 0.0
+    with TriplesServer(slice_ms=slice_ms, max_active=16) as srv:
+        sids = [srv.add_system(tm, b) for tm, b, f in pool]
+        tickets = []
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(sids[picks[i]], int(precs[i])))
+        res = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    for i, r in enumerate(res):
+        lat.append(r["finish"] - r["submit"])
+        F += pool[picks[i]][2]
+    lat = np.array(lat)
+    out = {"requests": nreq, "lambda_per_s": lam, "pool_systems": npool,
+           "precision_mix": {"fp64": int((precs == 0).sum()), "bf16x3": int((precs == 2).sum())},
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(lat.mean()), "wall_s": wall, "energies_per_s": nreq / wall,
+           "fp64_equiv_tflops": F / wall / 1e12, "slices": st["slices"], "prep_s": t_prep,
+           "sizes": [[int(tm.no), int(tm.nv)] for tm, b, f in pool]}
+    print(f"[bench] serve: {nreq} requests at {lam}/s over {npool} systems: p50 {out['p50_s']:.3f} s, "
+          f"p99 {out['p99_s']:.3f} s, {out['energies_per_s']:.2f} energies/s, {st['slices']} slices",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +240,13 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the reference CPU kernels (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--batch-jobs", type=int, default=64, help="config-4 batch size (0: skip)")
+    ap.add_argument("--serve", type=int, default=40, help="serving requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-pool", type=int, default=12)
+    ap.add_argument("--slice-ms", type=float, default=50.0)
+    ap.add_argument("--modes", default="fp32,bf16x3",
+                    help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the (T) path shards by tasks across GPUs but this "
@@ -167,6 +285,34 @@ def main() -> int:
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     kernel = job.kernel
+    modes = {}
+    for name in [m for m in args.modes.split(",") if m]:
+        prec = {"fp32": 1, "bf16x3": 2}[name]
+        t1 = time.perf_counter()
+        job.run(prec)  # converts the contraction tensors on first use (untimed)
+        t_prep = time.perf_counter() - t1
+        mk = []
+        t1 = time.perf_counter()
+        for s in range(args.steps):
+            m_bt, m_pt = job.run(prec)
+            mk.append(job.last_kernel_ms)
+        mwall = time.perf_counter() - t1
+        d_bt, d_pt = (m_bt - e_bt) * 1e6, (m_pt - e_pt) * 1e6
+        print(f"[bench] {name}: kernel {np.mean(mk):.1f} ms (first run incl. conversion "
+              f"{t_prep:.1f} s) E[T]={m_bt:.12e} dE[T]={d_bt:.3f} uEh, E(T)={m_pt:.12e} "
+              f"dE(T)={d_pt:.3f} uEh", file=sys.stderr, flush=True)
+        modes[name] = {
+            "ms_per_step": mwall / args.steps * 1e3,
+            "kernel_ms": float(np.mean(mk)),
+            "energies_per_s": args.steps / mwall,
+            "speedup_vs_fp64": wall / mwall,
+            "fp64_equiv_tflops": F * args.steps / mwall / 1e12,
+            "E_bt": m_bt, "E_pt": m_pt,
+            "dE_bt_uEh": d_bt, "dE_pt_uEh": d_pt,
+            "dE_rel": max(abs(m_bt - e_bt) / abs(e_bt), abs(m_pt - e_pt) / abs(e_pt)) if e_bt and e_pt else None,
+            "kernel": "fused_triples_mixed",
+            "extra_device_bytes": job.precision_bytes(prec),
+        }
     job.close()
 
     ms_step = wall / args.steps * 1e3
@@ -176,6 +322,9 @@ def main() -> int:
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload, kernel),
                 "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F}
+    batch = bench_batch(args.batch_jobs, args.seed, max(1, args.steps)) if args.batch_jobs else None
+    serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms) \
+        if args.serve else None
     cpu = None
     if args.cpu_baseline:
         print(f"[bench] cpu baseline on {args.cpu_threads} threads", file=sys.stderr, flush=True)
@@ -206,6 +355,9 @@ def main() -> int:
         "E_pt": e_pt,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "precision_modes": modes or None,
+        "batch": batch,
+        "serving": serve,
     }
     print(json.dumps(out))
     return 0

@@ -117,9 +117,13 @@ typedef struct {
   int restricted;
 } tce_t_tiles;
 
-/* Arithmetic of the d1/d2 contractions (s1, denominators and the energy sums
- * are FP64 in every mode). */
+/* Arithmetic of the d1/d2 contractions (s1, denominators, their division
+ * and the energy sums are FP64 in every mode). */
 #define TCE_T_FP64 0        /* FP64 MFMA                                 */
+#define TCE_T_FP32 1        /* FP32 operands, FP32 MFMA and accumulation */
+#define TCE_T_BF16X3 2      /* each operand x = hi + lo (two bf16), the  */
+                            /* products hi*hi + hi*lo + lo*hi on BF16    */
+                            /* MFMA with FP32 accumulation               */
 
 typedef struct tce_t_job tce_t_job;
 
@@ -132,6 +136,9 @@ int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **job);
 long tce_t_job_num_tasks(const tce_t_job *job);
 /* Device bytes the job holds once uploaded (admission control). */
 long tce_t_job_device_bytes(const tce_t_job *job);
+/* Additional device bytes the first run in a reduced precision allocates
+ * (converted copies of the four contraction tensors); 0 for TCE_T_FP64. */
+long tce_t_job_precision_bytes(const tce_t_job *job, int precision);
 
 /* Upload T1, T2 and V2 once.  Arguments mirror d_t1/k_t1_offset,
  * d_t2/k_t2_offset, d_v2/k_v2_offset of ccsd_t_gpu.F:2-3: host copies of the
@@ -144,7 +151,9 @@ int tce_t_job_upload(tce_t_job *job, const double *t1, const long *k_t1_offset,
                      const double *v2, const long *k_v2_offset);
 
 /* Run every task; *e_bt = E[T] (energy1), *e_pt = E(T) (energy2) of
- * ccsd_t_gpu.F:243-244.  Synchronous. */
+ * ccsd_t_gpu.F:243-244.  precision: TCE_T_FP64, TCE_T_FP32 or TCE_T_BF16X3
+ * (the first run in a reduced precision converts the contraction tensors on
+ * the device; see tce_t_job_precision_bytes).  Synchronous. */
 int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
@@ -170,6 +179,58 @@ double tce_t_job_last_kernel_ms(const tce_t_job *job);
 #define TCE_T_KERNEL_WG 1
 int tce_t_job_kernel(const tce_t_job *job);
 int tce_t_job_destroy(tce_t_job *job);
+
+/* ---- batches: many uploaded jobs in one persistent launch -------------- */
+/* No counterpart in the reference (its task farm runs one job per call).
+ * The jobs stay owned by the caller and must outlive the batch. */
+typedef struct tce_t_batch tce_t_batch;
+int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch **batch);
+/* Run every job of the batch in `precision`: energies[2j] = E[T],
+ * energies[2j+1] = E(T) of job j (tce_t_job_task_energies of each job is
+ * updated too).  The first run in a precision autotunes the launch (kernel
+ * family, segment order) and captures it in a hipGraph; later runs replay
+ * the graph.  Synchronous. */
+int tce_t_batch_run(tce_t_batch *batch, int precision, double *energies);
+/* Device time (ms) of the last run (graph launch to completion). */
+double tce_t_batch_last_ms(const tce_t_batch *batch);
+/* The plan chosen for `precision`: family 0 = fused_triples (4-wide boxes),
+ * 1 = fused_triples_wg, 2 = fused_triples_mixed; cost_order 1 = costliest
+ * work items first; tuned_ms[4] = candidate times of the tuning runs
+ * (family x order, -1 where not tried, all -1 when the plan came from the
+ * process-wide cache). */
+int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family, int *cost_order,
+                     double *tuned_ms);
+int tce_t_batch_destroy(tce_t_batch *batch);
+
+/* ---- serving: continuous batching of (T) requests on one GPU ---------- */
+/* No counterpart in the reference.  A worker thread owns the device.
+ * Systems are registered once (host blocks owned by the caller, valid until
+ * the server is destroyed); the first request for a system uploads it and it
+ * stays resident until memory is needed (idle systems are evicted LRU).
+ * Requests are admitted first come first served when a slot (max_active) is
+ * free and free HBM minus reserve_bytes covers what they add; every
+ * iteration launches, per precision, one persistent kernel over the next
+ * work items of all active requests (a launch lasts about slice_ms), so
+ * requests arriving while others run share the device at the next slice.
+ * A request's energies do not depend on what it shared the device with. */
+typedef struct tce_t_server tce_t_server;
+int tce_t_server_create(int device, double slice_ms, int max_active, long reserve_bytes,
+                        tce_t_server **server);
+/* Returns the system id (>= 0) or a negative status. */
+int tce_t_server_add_system(tce_t_server *server, const tce_t_tiles *tiles, const double *t1,
+                            const long *k_t1_offset, const double *t2, const long *k_t2_offset,
+                            const double *v2, const long *k_v2_offset);
+/* Queue a request; returns its ticket (>= 0) or a negative status. */
+long tce_t_server_submit(tce_t_server *server, int system, int precision);
+/* Wait for a ticket (timeout_s < 0: forever; TCE_T_ERR_STATE on timeout).
+ * times[4] = submit, admit, first slice, finish (seconds since creation).
+ * Returns the request's status. */
+int tce_t_server_wait(tce_t_server *server, long ticket, double timeout_s, double *e_bt,
+                      double *e_pt, double *times);
+/* Slices launched so far and systems resident now. */
+int tce_t_server_stats(tce_t_server *server, long *slices, int *resident);
+/* Finish every queued and active request, then free everything. */
+int tce_t_server_destroy(tce_t_server *server);
 
 /* ================================================================ 3 ===== */
 /* tce_tile.F:330-355 -- tiles of n orbitals, sizes differing by <= 1. */

@@ -70,6 +70,21 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_task_energies.argtypes = [C.c_void_p, DP, C.c_long]
     L.tce_t_job_last_kernel_ms.argtypes = [C.c_void_p]
     L.tce_t_job_last_kernel_ms.restype = C.c_double
+    L.tce_t_job_precision_bytes.argtypes = [C.c_void_p, C.c_int]
+    L.tce_t_job_precision_bytes.restype = C.c_long
+    L.tce_t_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+    L.tce_t_batch_run.argtypes = [C.c_void_p, C.c_int, DP]
+    L.tce_t_batch_last_ms.argtypes = [C.c_void_p]
+    L.tce_t_batch_last_ms.restype = C.c_double
+    L.tce_t_batch_plan.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), DP]
+    L.tce_t_batch_destroy.argtypes = [C.c_void_p]
+    L.tce_t_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_long, C.POINTER(C.c_void_p)]
+    L.tce_t_server_add_system.argtypes = [C.c_void_p, C.POINTER(TilesC), DP, LP, DP, LP, DP, LP]
+    L.tce_t_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.tce_t_server_submit.restype = C.c_long
+    L.tce_t_server_wait.argtypes = [C.c_void_p, C.c_long, C.c_double, DP, DP, DP]
+    L.tce_t_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_int)]
+    L.tce_t_server_destroy.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.restype = C.c_int
     L.tce_t_job_destroy.argtypes = [C.c_void_p]

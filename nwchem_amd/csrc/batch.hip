@@ -1,0 +1,345 @@
+// batch.hip -- many independent (T) jobs in one persistent launch.
+//
+// A batch holds uploaded jobs (owned by the caller).  Running it in a
+// precision launches one fused kernel over a segment table -- one segment
+// per job, all jobs' work items in one global range -- followed by each
+// job's fixed-order task reduction and the copy of its task energies to
+// pinned host memory.  The first run in a precision plans the launch:
+//
+//   autotuner  candidates are the kernel family for FP64 (fused_triples_wg
+//              with 8-wide particle boxes, fused_triples with 4-wide) and
+//              the segment order (as given; costliest items first, so the
+//              static round-robin schedule ends on the cheapest items).
+//              Each candidate runs once, timed with HIP events; the fastest
+//              is kept and remembered per batch shape (the sorted list of
+//              job dimensions and tilings) for later batches in the process.
+//   graph      the chosen launch sequence is captured once into a hipGraph
+//              (kernel, reductions, device-to-host copies; no memset nodes)
+//              and replayed by every later run.
+//
+// Each job's energies are bit-identical to its own tce_t_job_run in the
+// same precision and kernel family: work items do not interact and every
+// job's partials are reduced in its own fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "job.h"
+
+namespace tce_t {
+namespace {
+
+enum Family { FAM_WAVE = 0, FAM_WG = 1, FAM_MIXED = 2 };
+
+struct Choice {
+  int family = FAM_WG;
+  bool cost_order = true;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, Choice> g_tuned;  // batch shape -> choice
+
+struct Plan {
+  bool ready = false;
+  Choice choice;
+  void *tab = nullptr;       // device: job arguments + segments
+  long total = 0;            // work items of the launch
+  int nseg = 0, grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double tuned_ms[4] = {-1, -1, -1, -1};  // candidate times (family x order)
+};
+
+struct Batch {
+  std::vector<Job *> jobs;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double *hE = nullptr;            // pinned: task energies of all jobs
+  std::vector<long> eoff;          // per job: first task slot in hE
+  long ntasks = 0;
+  Plan plan[3];
+  float last_ms = 0.f;
+  int cus = 1;
+};
+
+std::string shape_key(const Batch &B, int precision) {
+  std::vector<std::string> parts;
+  for (const Job *J : B.jobs) {
+    std::string s = std::to_string(J->T.no) + "/" + std::to_string(J->T.nv) + ":";
+    for (long t = 1; t <= J->T.noab + J->T.nvab; ++t) s += std::to_string(J->T.rng(t)) + ",";
+    parts.push_back(s);
+  }
+  std::sort(parts.begin(), parts.end());
+  std::string key = std::to_string(precision) + "|";
+  for (auto &p : parts) key += p + ";";
+  return key;
+}
+
+// Segment order of a choice: job indices, costliest items first or as given.
+std::vector<int> job_order(const Batch &B, int precision, bool cost_order) {
+  std::vector<int> ord(B.jobs.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  if (cost_order)
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return item_cost(*B.jobs[a], precision) > item_cost(*B.jobs[b], precision);
+    });
+  return ord;
+}
+
+// Build the device table (job arguments, then segments) of a choice.
+int build_table(Batch &B, int precision, const Choice &c, Plan &P) {
+  const bool mixed = precision != TCE_T_FP64;
+  const bool wide = mixed || c.family == FAM_WG;
+  const std::vector<int> ord = job_order(B, precision, c.cost_order);
+  const size_t asz = ((mixed ? sizeof(MixedArgs) : sizeof(FusedArgs)) + 15) / 16 * 16;
+  const size_t ssz = mixed ? sizeof(MixedSegment) : sizeof(Segment);
+  const size_t n = B.jobs.size();
+  std::vector<char> h(n * asz + n * ssz, 0);
+  if (P.tab) (void)hipFree(P.tab);
+  P.tab = nullptr;
+  HIPCHK(hipMalloc(&P.tab, h.size()), "batch table");
+  char *d = (char *)P.tab;
+  long g = 0;
+  int nseg = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const Job &J = *B.jobs[ord[k]];
+    const long items = wide ? J.nitems8 : J.nitems;
+    if (items == 0) continue;
+    char *ha = h.data() + (size_t)nseg * asz;
+    if (mixed) {
+      MixedArgs M = job_margs(J, precision);
+      std::memcpy(ha, &M, sizeof M);
+      MixedSegment s{(const MixedArgs *)(d + (size_t)nseg * asz), 0, items, g};
+      std::memcpy(h.data() + n * asz + (size_t)nseg * ssz, &s, sizeof s);
+    } else {
+      FusedArgs A = job_args(J, wide);
+      std::memcpy(ha, &A, sizeof A);
+      Segment s{(const FusedArgs *)(d + (size_t)nseg * asz), 0, items, g};
+      std::memcpy(h.data() + n * asz + (size_t)nseg * ssz, &s, sizeof s);
+    }
+    g += items;
+    ++nseg;
+  }
+  HIPCHK(hipMemcpy(P.tab, h.data(), h.size(), hipMemcpyHostToDevice), "batch table");
+  P.total = g;
+  P.nseg = nseg;
+  const int per_cu = (!mixed && c.family == FAM_WAVE) ? 8 : 1;
+  P.grid = (int)std::max<long>(1, std::min<long>((long)B.cus * per_cu, g));
+  P.choice = c;
+  return TCE_T_OK;
+}
+
+// Enqueue the launch sequence of a plan on the batch stream.
+int enqueue(Batch &B, int precision, const Plan &P) {
+  const bool mixed = precision != TCE_T_FP64;
+  const bool wide = mixed || P.choice.family == FAM_WG;
+  const size_t n = B.jobs.size();
+  const size_t asz = ((mixed ? sizeof(MixedArgs) : sizeof(FusedArgs)) + 15) / 16 * 16;
+  const char *segs = (const char *)P.tab + n * asz;
+  if (P.total > 0) {
+    int rc;
+    if (mixed)
+      rc = launch_fused_mixed((const MixedSegment *)segs, P.nseg, P.total, precision, P.grid,
+                              B.stream);
+    else if (P.choice.family == FAM_WG)
+      rc = launch_fused_wg((const Segment *)segs, P.nseg, P.total, P.grid, B.stream);
+    else
+      rc = launch_fused_wave((const Segment *)segs, P.nseg, P.total, P.grid, B.stream);
+    if (rc != 0) {
+      set_error("batch kernel launch failed");
+      return TCE_T_ERR_HIP;
+    }
+  }
+  for (size_t j = 0; j < n; ++j) {
+    const Job &J = *B.jobs[j];
+    const long nt = (long)J.tasks.size();
+    if (nt == 0) continue;
+    const FusedArgs A = job_args(J, wide);
+    if (launch_task_reduce(A.tasks, (int)nt, A.nitems, wide ? 8 : J.ppi, A.partials, J.dtaskE,
+                           B.stream) != 0) {
+      set_error("batch task_reduce launch failed");
+      return TCE_T_ERR_HIP;
+    }
+    HIPCHK(hipMemcpyAsync(B.hE + 2 * B.eoff[j], J.dtaskE, 2 * nt * sizeof(double),
+                          hipMemcpyDeviceToHost, B.stream),
+           "batch D2H");
+  }
+  return TCE_T_OK;
+}
+
+// Time one eager run of a plan (kernel and reductions).
+int time_plan(Batch &B, int precision, const Plan &P, float &ms) {
+  HIPCHK(hipEventRecord(B.ev0, B.stream), "event");
+  const int rc = enqueue(B, precision, P);
+  if (rc != TCE_T_OK) return rc;
+  HIPCHK(hipEventRecord(B.ev1, B.stream), "event");
+  HIPCHK(hipStreamSynchronize(B.stream), "batch run");
+  HIPCHK(hipEventElapsedTime(&ms, B.ev0, B.ev1), "elapsed");
+  return TCE_T_OK;
+}
+
+int plan_batch(Batch &B, int precision) {
+  Plan &P = B.plan[precision];
+  const bool mixed = precision != TCE_T_FP64;
+  for (Job *J : B.jobs) {
+    int rc = mixed ? prepare_mixed(*J, precision) : ensure_tasks8(*J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  const std::string key = shape_key(B, precision);
+  Choice best;
+  bool known = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) best = it->second, known = true;
+  }
+  if (!known) {
+    // 4-wide boxes need every job's 4-wide task table (jobs whose own kernel
+    // is fused_triples)
+    bool all4 = true;
+    for (const Job *J : B.jobs) all4 = all4 && J->pbox == 4;
+    std::vector<int> fams = mixed ? std::vector<int>{FAM_MIXED}
+                                  : (all4 ? std::vector<int>{FAM_WG, FAM_WAVE} : std::vector<int>{FAM_WG});
+    std::vector<Choice> cands;
+    for (int fam : fams)
+      for (bool co : {true, false}) cands.push_back(Choice{fam, co});
+    float best_ms = 0.f;
+    for (size_t i = 0; i < cands.size(); ++i) {
+      int rc = build_table(B, precision, cands[i], P);
+      if (rc != TCE_T_OK) return rc;
+      float ms = 0.f;
+      rc = time_plan(B, precision, P, ms);
+      if (rc != TCE_T_OK) return rc;
+      P.tuned_ms[i] = ms;
+      if (i == 0 || ms < best_ms) best_ms = ms, best = cands[i];
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = best;
+  }
+  int rc = build_table(B, precision, best, P);
+  if (rc != TCE_T_OK) return rc;
+  HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal), "capture");
+  rc = enqueue(B, precision, P);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(B.stream, &g);
+  if (rc != TCE_T_OK) return rc;
+  HIPCHK(e, "end capture");
+  P.graph = g;
+  HIPCHK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0), "instantiate");
+  P.ready = true;
+  return TCE_T_OK;
+}
+
+}  // namespace
+}  // namespace tce_t
+
+using namespace tce_t;
+
+extern "C" int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch **out) {
+  if (!out || njobs < 1 || !jobs) return TCE_T_ERR_ARG;
+  auto B = new Batch;
+  int dev = 0;
+  for (int j = 0; j < njobs; ++j) {
+    Job *J = reinterpret_cast<Job *>(jobs[j]);
+    if (!J || !J->uploaded) {
+      delete B;
+      set_error("tce_t_batch_create: every job must be uploaded");
+      return TCE_T_ERR_STATE;
+    }
+    B->jobs.push_back(J);
+    B->eoff.push_back(B->ntasks);
+    B->ntasks += (long)J->tasks.size();
+    dev = J->device;
+  }
+  auto fail = [&](hipError_t e, const char *what) {
+    hip_ok(e, what);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return TCE_T_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(e, "batch stream");
+  if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return fail(e, "event");
+  if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return fail(e, "event");
+  if ((e = hipHostMalloc(&B->hE, std::max<long>(1, 2 * B->ntasks) * sizeof(double))) != hipSuccess)
+    return fail(e, "batch energies");
+  if ((e = hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return fail(e, "attr");
+  *out = reinterpret_cast<tce_t_batch *>(B);
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_batch_run(tce_t_batch *batch, int precision, double *energies) {
+  if (!batch) return TCE_T_ERR_ARG;
+  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+    set_error("unsupported precision mode");
+    return TCE_T_ERR_ARG;
+  }
+  Batch &B = *reinterpret_cast<Batch *>(batch);
+  Plan &P = B.plan[precision];
+  if (!P.ready) {
+    const int rc = plan_batch(B, precision);
+    if (rc != TCE_T_OK) return rc;
+  }
+  HIPCHK(hipEventRecord(B.ev0, B.stream), "event");
+  HIPCHK(hipGraphLaunch(P.exec, B.stream), "graph launch");
+  HIPCHK(hipEventRecord(B.ev1, B.stream), "event");
+  HIPCHK(hipStreamSynchronize(B.stream), "batch run");
+  HIPCHK(hipEventElapsedTime(&B.last_ms, B.ev0, B.ev1), "elapsed");
+  for (size_t j = 0; j < B.jobs.size(); ++j) {
+    Job &J = *B.jobs[j];
+    const long nt = (long)J.tasks.size();
+    J.taskE.assign(B.hE + 2 * B.eoff[j], B.hE + 2 * (B.eoff[j] + nt));
+    double s4 = 0.0, s5 = 0.0;
+    for (long t = 0; t < nt; ++t) {
+      s4 += J.taskE[2 * t];
+      s5 += J.taskE[2 * t + 1];
+    }
+    if (energies) {
+      energies[2 * j] = s4;
+      energies[2 * j + 1] = s5;
+    }
+  }
+  return TCE_T_OK;
+}
+
+extern "C" double tce_t_batch_last_ms(const tce_t_batch *batch) {
+  return batch ? (double)reinterpret_cast<const Batch *>(batch)->last_ms : -1.0;
+}
+
+extern "C" int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family,
+                                int *cost_order, double *tuned_ms) {
+  if (!batch || precision < 0 || precision > 2) return TCE_T_ERR_ARG;
+  const Plan &P = reinterpret_cast<const Batch *>(batch)->plan[precision];
+  if (!P.ready) return TCE_T_ERR_STATE;
+  if (family) *family = P.choice.family;
+  if (cost_order) *cost_order = P.choice.cost_order ? 1 : 0;
+  if (tuned_ms)
+    for (int i = 0; i < 4; ++i) tuned_ms[i] = P.tuned_ms[i];
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_batch_destroy(tce_t_batch *batch) {
+  if (!batch) return TCE_T_OK;
+  Batch *B = reinterpret_cast<Batch *>(batch);
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  for (Plan &P : B->plan) {
+    if (P.exec) (void)hipGraphExecDestroy(P.exec);
+    if (P.graph) (void)hipGraphDestroy(P.graph);
+    if (P.tab) (void)hipFree(P.tab);
+  }
+  if (B->hE) (void)hipHostFree(B->hE);
+  if (B->ev0) (void)hipEventDestroy(B->ev0);
+  if (B->ev1) (void)hipEventDestroy(B->ev1);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return TCE_T_OK;
+}

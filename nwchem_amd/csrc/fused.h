@@ -31,6 +31,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/tce_t.h"
+
 namespace tce_t {
 
 struct TaskDev {
@@ -50,6 +52,105 @@ struct FusedArgs {
   int no, nv, Ep, Mp;  // orbitals, padded contraction extents
   double *partials;    // {E[T], E(T)} per wave of each item
 };
+
+// Reduced-precision operands of the d1/d2 contractions (fused_mixed.hip):
+// the dense tensors of FusedArgs converted to FP32 (part 0) or split into
+// bf16 hi (part 0) and lo (part 1), contraction index padded with zeros to
+// Ek (d2) / Mk (d1) elements.
+struct MixedArgs {
+  FusedArgs f;
+  const void *x2[2], *y2[2];  // d2: T2e, Vd2
+  const void *x1[2], *y1[2];  // d1: Wd1, T2m
+  int Ek, Mk;
+};
+
+// A launch covers a table of segments: work items [item0, item1) of the job
+// whose arguments sit at *args (device memory); g0 is the segment's first
+// index in the launch's global item range (segments sorted by g0).  One job
+// is one segment; a batch of jobs, or a slice of several jobs' remaining
+// items, is one launch over several.
+template <class Args>
+struct SegmentT {
+  const Args *args;
+  long item0, item1, g0;
+};
+typedef SegmentT<FusedArgs> Segment;
+typedef SegmentT<MixedArgs> MixedSegment;
+
+// Pointers read from memory are generic to the compiler, which then issues
+// flat (per-lane) loads and treats what they return as divergent.  Every
+// pointer of the argument tables is global memory: the kernels read the
+// tables into these mirrors (same layout, pointers typed global), which keeps
+// the loads scalar/global and the values uniform.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+struct FusedArgsG {
+  gptr<const double> T1, T2e, T2m, Vd2, Wd1, S, evl;
+  gptr<const TaskDev> tasks;
+  int ntasks;
+  long nitems;
+  int no, nv, Ep, Mp;
+  gptr<double> partials;
+};
+struct MixedArgsG {
+  FusedArgsG f;
+  gptr<const void> x2[2], y2[2];
+  gptr<const void> x1[2], y1[2];
+  int Ek, Mk;
+};
+static_assert(sizeof(FusedArgsG) == sizeof(FusedArgs), "argument mirror");
+static_assert(sizeof(MixedArgsG) == sizeof(MixedArgs), "argument mirror");
+
+__device__ __forceinline__ FusedArgsG load_args(const FusedArgs *p) {
+  const gptr<const FusedArgs> q = (gptr<const FusedArgs>)p;
+  FusedArgsG A;
+  A.T1 = (gptr<const double>)q->T1;
+  A.T2e = (gptr<const double>)q->T2e;
+  A.T2m = (gptr<const double>)q->T2m;
+  A.Vd2 = (gptr<const double>)q->Vd2;
+  A.Wd1 = (gptr<const double>)q->Wd1;
+  A.S = (gptr<const double>)q->S;
+  A.evl = (gptr<const double>)q->evl;
+  A.tasks = (gptr<const TaskDev>)q->tasks;
+  A.ntasks = q->ntasks;
+  A.nitems = q->nitems;
+  A.no = q->no;
+  A.nv = q->nv;
+  A.Ep = q->Ep;
+  A.Mp = q->Mp;
+  A.partials = (gptr<double>)q->partials;
+  return A;
+}
+
+__device__ __forceinline__ MixedArgsG load_args(const MixedArgs *p) {
+  const gptr<const MixedArgs> q = (gptr<const MixedArgs>)p;
+  MixedArgsG M;
+  M.f = load_args(&p->f);
+  for (int i = 0; i < 2; ++i) {
+    M.x2[i] = (gptr<const void>)q->x2[i];
+    M.y2[i] = (gptr<const void>)q->y2[i];
+    M.x1[i] = (gptr<const void>)q->x1[i];
+    M.y1[i] = (gptr<const void>)q->y1[i];
+  }
+  M.Ek = q->Ek;
+  M.Mk = q->Mk;
+  return M;
+}
+
+// Segment of global index g (uniform): the last with g0 <= g.
+template <class Args>
+__device__ __forceinline__ int find_segment(const SegmentT<Args> *s, int n, long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (s[mid].g0 <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 typedef double acc_t;
 
@@ -104,7 +205,7 @@ __device__ __forceinline__ void switch_layout(acc_t (&acc)[64], int x, int y, in
 // i0/j0/k0: first hole of the box within the tiles; hI/hJ/hK: global hole
 // indices (clamped).  Elements past a ragged tile edge are skipped.
 // denom = d * factor / (e_h1 + e_h2 + e_h3 + denom_0) as ccsd_t_dot.F:107.
-__device__ __forceinline__ void item_energy(const acc_t (&acc)[64], const FusedArgs &A,
+__device__ __forceinline__ void item_energy(const acc_t (&acc)[64], const FusedArgsG &A,
                                             const TaskDev &T, int la, int lb, int lc, int i0,
                                             int j0, int k0, const int (&hI)[4], const int (&hJ)[4],
                                             const int (&hK)[4], double &e4, double &e5) {
@@ -115,7 +216,7 @@ __device__ __forceinline__ void item_energy(const acc_t (&acc)[64], const FusedA
   const long pa = T.op4 + la, pb = T.op5 + lb, pc = T.op6 + lc;
   const double ea = A.evl[no + pa], eb = A.evl[no + pb], ec = A.evl[no + pc];
   const double denom_0 = -(ea + eb + ec);
-  const double *T1 = A.T1, *S = A.S;
+  const gptr<const double> T1 = A.T1, S = A.S;
   const long sSh = (long)nv * nv;  // stride of (h*no+h') in S
   const long pbc = pb * nv + pc, pac = pa * nv + pc, pab = pa * nv + pb;
 #pragma unroll
@@ -159,7 +260,7 @@ __device__ __forceinline__ void wave_sum(double &e4, double &e5) {
 }
 
 // Locate the task of a work item (tasks sorted by item0) -- uniform.
-__device__ __forceinline__ int find_task(const TaskDev *tasks, int ntasks, long item) {
+__device__ __forceinline__ int find_task(gptr<const TaskDev> tasks, int ntasks, long item) {
   int lo = 0, hi = ntasks - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -171,8 +272,11 @@ __device__ __forceinline__ int find_task(const TaskDev *tasks, int ntasks, long 
   return lo;
 }
 
-// Host launchers (one file per kernel family).
-int launch_fused_wave(const FusedArgs &A, int grid, hipStream_t s);
-int launch_fused_wg(const FusedArgs &A, int grid, hipStream_t s);
+// Host launchers (one file per kernel family): `total` items over `nseg`
+// segments (device table), `grid` persistent workgroups.
+int launch_fused_wave(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
+int launch_fused_wg(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
+int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
+                       hipStream_t s);
 
 }  // namespace tce_t

@@ -1,0 +1,554 @@
+// fused_mixed.hip -- the fused (T) kernel with the d1/d2 contractions in
+// reduced precision on 16x16 MFMA tiles, the denominators, their division,
+// the singles term and every energy sum in FP64.
+//
+//   TCE_T_FP32    v_mfma_f32_16x16x4_f32, FP32 operands and accumulation
+//   TCE_T_BF16X3  v_mfma_f32_16x16x32_bf16, each operand x split into
+//                 hi = bf16(x), lo = bf16(x - hi); per product hi*hi + hi*lo
+//                 + lo*hi (3 terms), FP32 accumulation
+//
+// Work item and workgroup as fused_wg.hip: one task x an 8x8x8 particle box x
+// a 4x4x4 hole box, 512 threads (8 waves, 2 per SIMD); wave w owns the 4x4x4
+// particle cube (w>>2&1, w>>1&1, w&1) by the hole box, 4096 t3 elements in
+// 64 FP32 accumulators per lane.
+//
+// MFMA lane layout (16x16, probed on the device): A[i][k] at lane i+16(k/E),
+// B[k][j] at lane j+16(k/E) (E = k values per lane: 1 for f32, 8 for bf16),
+// D[i][j] at lane j+16(i/4), register i%4.  Every d1/d2 contraction is
+// X(L, r, s; e) * Y(p1, p2, h; e): lone particle L with the hole pair (r, s)
+// on one operand, the particle pair (p1, p2) with the lone hole h on the
+// other.  With L fixed, row i = 4L' + r and col j = 4p1' + p2', so
+//   lane = 4 p1' + p2' + 16 L', accumulator quad (s, h), component r.
+// The lane keeps element (i, j, k) of its hole box at acc[16i + 4j + k]; a
+// pass only chooses which 4 of them form each MFMA quad (r = i for lone hole
+// j or k, r = j for lone hole i), so the three contractions of one L share
+// the lane layout and changing L (a -> b -> c) is one lane permutation.
+// Pass order: L=a {h=k, h=j, h=i} x {d2, d1}, L=b, L=c (18 passes).
+//
+// Operands stream through LDS in chunks of KC contraction values (16 f32 or
+// 32 bf16 = four 16-byte quads per row) with three buffers: direct
+// global->LDS loads (global_load_lds_dwordx4) fetch chunk j+2 while the
+// waves run chunk j from LDS; the workgroup waits for its own loads of chunk
+// j+1 and synchronises once per chunk.  A pass with a minus sign negates the
+// accumulators before and after it.
+//
+// LDS layout of one buffer part (hi or lo; quads, kq = quad 0..3 of a row):
+//   X  [4 s][8 L slots][4 r][4 kq ^ ((slot & 3) ^ (r & 2))]         512 quads
+//   Y  [4 h][8 p1 slots][8 p2 slots][4 kq ^ ((p1 & 3) ^ (p2 & 2))] 1024 quads
+// Fragment reads (16 lanes of one kq: 4 row/col slots x 4 inner indices)
+// touch 16 distinct 4-bank groups; each load instruction writes 64
+// consecutive quads and 4 consecutive lanes read one 64-byte row segment.
+#include "fused.h"
+
+namespace tce_t {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <int PREC>
+struct Prec;
+template <>
+struct Prec<TCE_T_FP32> {
+  static constexpr int KC = 16;    // contraction values per chunk
+  static constexpr int NPART = 1;  // operand parts
+  static constexpr int ES = 4;     // bytes per element
+};
+template <>
+struct Prec<TCE_T_BF16X3> {
+  static constexpr int KC = 32;
+  static constexpr int NPART = 2;  // hi, lo
+  static constexpr int ES = 2;
+};
+
+constexpr int X_QUADS = 512, Y_QUADS = 1024, PART_QUADS = X_QUADS + Y_QUADS;
+constexpr int NBUF = 3;
+
+// Epilogue staging (doubles), overlaying the chunk buffers after the last
+// chunk:
+//   S   [hole pair t: (j,k) (i,k) (i,j)][16 pairs u*4+v]
+//       [particle pair s: (b,c) (a,c) (a,b)][8 x 8 slots]           9216
+//   T1  [particle box a b c][hole box i j k][8 slots][4 holes]        288
+constexpr int EPI_S = 3 * 16 * 3 * 64;
+constexpr int EPI_T1 = 9 * 32;
+constexpr int EPI_QUADS = (EPI_S + EPI_T1) / 2;
+
+template <int PREC>
+constexpr int buf_quads() {
+  return Prec<PREC>::NPART * PART_QUADS;
+}
+template <int PREC>
+constexpr int lds_quads() {
+  return NBUF * buf_quads<PREC>() > EPI_QUADS ? NBUF * buf_quads<PREC>() : EPI_QUADS;
+}
+
+// The 18 passes: kind (0 = d2, 1 = d1), lone particle L (a b c), lone hole
+// h (i j k), sign.  d2: X = t(L e r s) (T2e), Y = <p1 p2||h e> (Vd2); d1:
+// X = <m L||r s> (Wd1), Y = t(p1 p2 h m) (T2m); signs and pairs as the
+// variant table in fused.h (ccsd_t_kernels_omp.F layouts).
+struct PassInfo {
+  int kind, L, h, minus;
+};
+constexpr PassInfo PASS[18] = {
+    {0, 0, 2, 1},  // d2 N=1: -t(a e i j)<b c||k e>
+    {1, 0, 2, 1},  // d1 N=6: -t(b c k m)<m a||i j>
+    {0, 0, 1, 0},  // d2 N=3: +t(a e i k)<b c||j e>
+    {1, 0, 1, 0},  // d1 N=5: +t(b c j m)<m a||i k>
+    {0, 0, 0, 1},  // d2 N=2: -t(a e j k)<b c||i e>
+    {1, 0, 0, 1},  // d1 N=4: -t(b c i m)<m a||j k>
+    {0, 1, 0, 0},  // d2 N=5: +t(b e j k)<a c||i e>
+    {1, 1, 0, 0},  // d1 N=7: +t(a c i m)<m b||j k>
+    {0, 1, 2, 0},  // d2 N=4: +t(b e i j)<a c||k e>
+    {1, 1, 2, 0},  // d1 N=9: +t(a c k m)<m b||i j>
+    {0, 1, 1, 1},  // d2 N=6: -t(b e i k)<a c||j e>
+    {1, 1, 1, 1},  // d1 N=8: -t(a c j m)<m b||i k>
+    {0, 2, 2, 1},  // d2 N=7: -t(c e i j)<a b||k e>
+    {1, 2, 2, 1},  // d1 N=3: -t(a b k m)<m c||i j>
+    {0, 2, 1, 0},  // d2 N=9: +t(c e i k)<a b||j e>
+    {1, 2, 1, 0},  // d1 N=2: +t(a b j m)<m c||i k>
+    {0, 2, 0, 1},  // d2 N=8: -t(c e j k)<a b||i e>
+    {1, 2, 0, 1},  // d1 N=1: -t(a b i m)<m c||j k>
+};
+
+// Hole box of an item: global hole o + min(b0 + u, rm), u = 0..3.
+struct HoleBox {
+  int o, b0, rm;
+};
+__device__ __forceinline__ int hole(const HoleBox &h, int u) { return h.o + min(h.b0 + u, h.rm); }
+// Particle range of the box: global index g0 + min(slot, sm), slot = 0..7.
+struct PRange {
+  long g0;
+  int sm;
+};
+struct ItemGeo {
+  HoleBox I, J, K;
+  PRange a, b, c;
+};
+template <int T>
+__device__ __forceinline__ int hole_t(const ItemGeo &G, int u) {
+  if constexpr (T == 0) return hole(G.I, u);
+  else if constexpr (T == 1) return hole(G.J, u);
+  else return hole(G.K, u);
+}
+template <int T>
+__device__ __forceinline__ long particle_t(const ItemGeo &G, int slot) {
+  if constexpr (T == 0) return G.a.g0 + min(slot, G.a.sm);
+  else if constexpr (T == 1) return G.b.g0 + min(slot, G.b.sm);
+  else return G.c.g0 + min(slot, G.c.sm);
+}
+// Per-lane selections (ternaries on scalars).
+__device__ __forceinline__ int hole_of(const ItemGeo &G, int t, int u) {
+  return t == 0 ? hole(G.I, u) : (t == 1 ? hole(G.J, u) : hole(G.K, u));
+}
+__device__ __forceinline__ long particle_of(const ItemGeo &G, int t, int slot) {
+  return t == 0 ? G.a.g0 + min(slot, G.a.sm)
+                : (t == 1 ? G.b.g0 + min(slot, G.b.sm) : G.c.g0 + min(slot, G.c.sm));
+}
+// This wave's cube coordinate along particle x (0 = a, 1 = b, 2 = c).
+__device__ __forceinline__ int cube(int wave, int x) { return (wave >> (2 - x)) & 1; }
+
+__device__ __forceinline__ int x_swz(int slot, int r) { return (slot & 3) ^ (r & 2); }
+__device__ __forceinline__ int y_swz(int p1, int p2) { return (p1 & 3) ^ (p2 & 2); }
+
+// Direct global->LDS load of 16 bytes per lane: lane l writes LDS bytes
+// [m0 + 16 l, m0 + 16 l + 16).  Issued by inline assembly so that the
+// compiler does not tie later LDS reads of other buffers to it; the kernel
+// waits for these loads itself (vmcnt) before the barrier that publishes a
+// buffer.
+__device__ __forceinline__ void lds_load16(const void *g, unsigned lds_byte) {
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_byte), "v"(g)
+               : "memory");
+}
+// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+__device__ __forceinline__ void block_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// One thread's load sources of a pass, per operand part: the X quad of LDS
+// block `wave`, the Y quads of blocks 2w and 2w+1 (byte pointers at k = 0).
+template <int PREC>
+struct Src {
+  const char *x[Prec<PREC>::NPART], *y0[Prec<PREC>::NPART], *y1[Prec<PREC>::NPART];
+};
+
+template <int PREC, int P>
+__device__ __forceinline__ Src<PREC> pass_src(const MixedArgsG &M, const ItemGeo &G, int wave,
+                                              int lane) {
+  using Pr = Prec<PREC>;
+  constexpr PassInfo pi = PASS[P];
+  constexpr int L = pi.L, h = pi.h;
+  constexpr int r = h == 0 ? 1 : 0, s = h == 2 ? 1 : 2, p1 = L == 0 ? 1 : 0, p2 = L == 2 ? 1 : 2;
+  constexpr int QE = 16 / Pr::ES;  // elements per quad
+  const int no = M.f.no, nv = M.f.nv;
+  const long Kp = pi.kind ? M.Mk : M.Ek;
+  Src<PREC> S;
+  // X block `wave`: s-index wave>>1, slots 4*(wave&1) + lane>>4
+  const int xs = wave >> 1, slot = 4 * (wave & 1) + (lane >> 4), xr = (lane >> 2) & 3;
+  const long xe = (((long)hole_t<r>(G, xr) * no + hole_t<s>(G, xs)) * nv + particle_t<L>(G, slot)) *
+                      Kp + QE * ((lane & 3) ^ x_swz(slot, xr));
+  // Y blocks 2w+t: h-index bq>>2, p1 slots 2*(bq&3) + lane>>5
+  long ye[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int bq = 2 * wave + t;
+    const int yh = bq >> 2, p1s = 2 * (bq & 3) + (lane >> 5), p2s = (lane >> 2) & 7;
+    const long row = particle_t<p1>(G, p1s) * nv + particle_t<p2>(G, p2s);
+    ye[t] = ((long)hole_t<h>(G, yh) * nv * nv + row) * Kp + QE * ((lane & 3) ^ y_swz(p1s, p2s));
+  }
+#pragma unroll
+  for (int part = 0; part < Pr::NPART; ++part) {
+    const char *X = (const char *)(pi.kind ? M.x1[part] : M.x2[part]);
+    const char *Y = (const char *)(pi.kind ? M.y1[part] : M.y2[part]);
+    S.x[part] = X + xe * Pr::ES;
+    S.y0[part] = Y + ye[0] * Pr::ES;
+    S.y1[part] = Y + ye[1] * Pr::ES;
+  }
+  return S;
+}
+
+// Issue the loads of one chunk (byte offset kb within the rows) into buffer b.
+template <int PREC>
+__device__ __forceinline__ void load_chunk(const Src<PREC> &S, long kb, unsigned lds_base, int b,
+                                           int wave) {
+#pragma unroll
+  for (int part = 0; part < Prec<PREC>::NPART; ++part) {
+    const unsigned base = lds_base + (unsigned)(b * buf_quads<PREC>() + part * PART_QUADS) * 16u;
+    lds_load16(S.x[part] + kb, __builtin_amdgcn_readfirstlane(base + wave * 1024u));
+    lds_load16(S.y0[part] + kb,
+               __builtin_amdgcn_readfirstlane(base + X_QUADS * 16u + 2 * wave * 1024u));
+    lds_load16(S.y1[part] + kb,
+               __builtin_amdgcn_readfirstlane(base + X_QUADS * 16u + (2 * wave + 1) * 1024u));
+  }
+}
+
+template <int PREC>
+__device__ __forceinline__ Src<PREC> select_src(bool first, const Src<PREC> &a, bool second,
+                                                const Src<PREC> &b, const Src<PREC> &c) {
+  Src<PREC> S;
+#pragma unroll
+  for (int part = 0; part < Prec<PREC>::NPART; ++part) {
+    S.x[part] = first ? a.x[part] : (second ? b.x[part] : c.x[part]);
+    S.y0[part] = first ? a.y0[part] : (second ? b.y0[part] : c.y0[part]);
+    S.y1[part] = first ? a.y1[part] : (second ? b.y1[part] : c.y1[part]);
+  }
+  return S;
+}
+
+// Accumulator index of quad (s, h), component r for row hole RH (0 = i,
+// 1 = j) and lone hole H: element (i, j, k) at acc[16 i + 4 j + k].
+template <int H>
+__device__ __forceinline__ constexpr int acc_at(int s, int h, int r) {
+  return H == 0 ? 16 * h + 4 * r + s      // r = j, s = k, h = i
+                : (H == 2 ? 16 * r + 4 * s + h   // r = i, s = j, h = k
+                          : 16 * r + 4 * h + s);  // r = i, s = k, h = j
+}
+
+// The MFMAs of one chunk.  xa/yb: this lane's X / Y quad offsets for s = 0 /
+// h = 0 (s adds 128 quads, h adds 256).
+template <int PREC, int H>
+__device__ __forceinline__ void chunk_mfma(float (&a)[64], const f4 *cb, int xa, int yb) {
+  if constexpr (PREC == TCE_T_FP32) {
+    f4 xf[4], yf[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xf[s] = cb[xa + 128 * s];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) yf[h] = cb[X_QUADS + yb + 256 * h];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                  a[acc_at<H>(s, h, 3)]};
+          c = __builtin_amdgcn_mfma_f32_16x16x4f32(xf[s][t], yf[h][t], c, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+        }
+  } else {
+    const bf8 *cq = reinterpret_cast<const bf8 *>(cb);
+    bf8 xh[4], xl[4], yh[4], yl[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      xh[s] = cq[xa + 128 * s];
+      xl[s] = cq[PART_QUADS + xa + 128 * s];
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      yh[h] = cq[X_QUADS + yb + 256 * h];
+      yl[h] = cq[PART_QUADS + X_QUADS + yb + 256 * h];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                a[acc_at<H>(s, h, 3)]};
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[s], yh[h], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[s], yl[h], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[s], yh[h], c, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+      }
+  }
+}
+
+__device__ __forceinline__ void permute_lanes(float (&a)[64], int src) {
+#pragma unroll
+  for (int r = 0; r < 64; ++r)
+    a[r] = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(a[r])));
+}
+
+__device__ __forceinline__ void negate(float (&a)[64]) {
+#pragma unroll
+  for (int r = 0; r < 64; ++r) a[r] = -a[r];
+}
+
+template <int PREC>
+__device__ __forceinline__ constexpr int chunks_of(const MixedArgsG &M, int p) {
+  return p >= 18 ? 0 : (PASS[p].kind ? M.Mk : M.Ek) / Prec<PREC>::KC;
+}
+
+// Pass P: layout change, fragment offsets, then its chunks.  Each chunk
+// first issues the loads of chunk j+2 (of this pass or of the next two),
+// then runs its MFMAs from buffer j % 3, waits for this thread's loads of
+// chunk j+1 and synchronises.
+template <int PREC, int P>
+__device__ __forceinline__ void pass_step(float (&acc)[64], const f4 *lds, unsigned lds_base,
+                                          const MixedArgsG &M, const ItemGeo &G, int wave,
+                                          int lane, int &j, int &buf, int total) {
+  using Pr = Prec<PREC>;
+  constexpr PassInfo pi = PASS[P];
+  constexpr int Lp = pi.L, H = pi.h;
+  constexpr int p1 = Lp == 0 ? 1 : 0, p2 = Lp == 2 ? 1 : 2;
+  constexpr int NLOAD = 3 * Pr::NPART;  // load instructions per chunk
+  const int kq = lane >> 4, hi = (lane & 15) >> 2, lo = lane & 3;
+  if constexpr (P == 6) {  // L: a -> b; lane (4b'+c') + 16a' -> (4a'+c') + 16b'
+    permute_lanes(acc, 4 * kq + lo + 16 * hi);
+  } else if constexpr (P == 12) {  // L: b -> c; lane (4a'+c') + 16b' -> (4a'+b') + 16c'
+    permute_lanes(acc, 4 * hi + kq + 16 * lo);
+  }
+  if constexpr (pi.minus) negate(acc);
+  const int nch0 = chunks_of<PREC>(M, P), nch1 = chunks_of<PREC>(M, P + 1);
+  // X fragment: row (lane & 15) = 4L' + r (L' = hi, r = lo), quad kq
+  const int slot = 4 * cube(wave, Lp) + hi;
+  const int xa = (slot * 4 + lo) * 4 + (kq ^ x_swz(slot, lo));
+  // Y fragment: col (lane & 15) = 4p1' + p2' (p1' = hi, p2' = lo)
+  const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
+  const int yb = (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s));
+  const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
+  const Src<PREC> s1 = pass_src<PREC, (P + 1 < 18 ? P + 1 : 17)>(M, G, wave, lane);
+  const Src<PREC> s2 = pass_src<PREC, (P + 2 < 18 ? P + 2 : 17)>(M, G, wave, lane);
+  for (int c = 0; c < nch0; ++c, ++j) {
+    const bool more = j + 2 < total;
+    if (more) {
+      const int c2 = c + 2;
+      const bool in0 = c2 < nch0, in1 = c2 - nch0 < nch1;
+      const int cc = in0 ? c2 : (in1 ? c2 - nch0 : c2 - nch0 - nch1);
+      load_chunk<PREC>(select_src<PREC>(in0, s0, in1, s1, s2), (long)cc * Pr::KC * Pr::ES,
+                       lds_base, buf == 0 ? 2 : buf - 1, wave);
+    }
+    chunk_mfma<PREC, H>(acc, lds + buf * buf_quads<PREC>(), xa, yb);
+    // own loads of chunk j+1 have landed (only chunk j+2's may be pending)
+    if (more)
+      wait_vm<NLOAD>();
+    else
+      wait_vm<0>();
+    block_barrier();
+    buf = buf == 2 ? 0 : buf + 1;
+  }
+  if constexpr (pi.minus) negate(acc);
+}
+
+template <int PREC, int P>
+__device__ __forceinline__ void all_passes(float (&acc)[64], const f4 *lds, unsigned lds_base,
+                                           const MixedArgsG &M, const ItemGeo &G, int wave,
+                                           int lane, int &j, int &buf, int total) {
+  pass_step<PREC, P>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
+  if constexpr (P + 1 < 18) all_passes<PREC, P + 1>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
+}
+
+// Stage the item's singles operands into LDS (FP64, as fused_wg.hip).
+__device__ __forceinline__ void stage_epilogue(double *e, const FusedArgsG &A, int tid,
+                                               const ItemGeo &G) {
+  const int no = A.no, nv = A.nv;
+#pragma unroll
+  for (int rb = 0; rb < 18; rb += 6) {
+    double v[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      const int idx = tid + 512 * (rb + r);
+      const int q = idx >> 6, p1 = (idx >> 3) & 7, p2 = idx & 7;
+      const int sp = q % 3, tp = q / 3, t = tp >> 4, u = (tp >> 2) & 3, w = tp & 3;
+      const int h1 = hole_of(G, t == 0 ? 1 : 0, u), h2 = hole_of(G, t == 2 ? 1 : 2, w);
+      const long g1 = particle_of(G, sp == 0 ? 1 : 0, p1), g2 = particle_of(G, sp == 2 ? 1 : 2, p2);
+      v[r] = A.S[(((long)h1 * no + h2) * nv + g1) * nv + g2];
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) e[tid + 512 * (rb + r)] = v[r];
+  }
+  if (tid < EPI_T1) {
+    const int u = tid & 3, slot = (tid >> 2) & 7, pt = (tid >> 5) / 3, ht = (tid >> 5) % 3;
+    e[EPI_S + tid] = A.T1[particle_of(G, pt, slot) * no + hole_of(G, ht, u)];
+  }
+}
+
+// FP64 energy of one wave's 4096 elements: d[16 i + 4 j + k] (converted
+// from the FP32 accumulators); den = d * factor / (e_h1 + e_h2 + e_h3 -
+// e_p4 - e_p5 - e_p6), E[T] += den * d, E(T) += den * (d + s) with the 9
+// singles products (ccsd_t_dot.F:107 and the s1 table, as fused_wg.hip).
+__device__ __forceinline__ void energy(const float (&acc)[64], const double *e,
+                                       const FusedArgsG &A, const TaskDev &T, int sa, int sb,
+                                       int sc, int la, int lb, int lc, int i0, int j0, int k0,
+                                       const ItemGeo &G, double &e4, double &e5) {
+  e4 = 0.0;
+  e5 = 0.0;
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  const int no = A.no;
+  const double denom_0 =
+      -(A.evl[no + T.op4 + la] + A.evl[no + T.op5 + lb] + A.evl[no + T.op6 + lc]);
+  double eh[3][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    eh[0][u] = A.evl[hole(G.I, u)];
+    eh[1][u] = A.evl[hole(G.J, u)];
+    eh[2][u] = A.evl[hole(G.K, u)];
+  }
+  double den[64];
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int r = 16 * ii + 4 * jj + kk;
+        const bool ok = i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3;
+        const double d = (double)acc[r];
+        const double dn = d * T.factor / (eh[0][ii] + eh[1][jj] + eh[2][kk] + denom_0);
+        e4 += ok ? dn * d : 0.0;
+        den[r] = ok ? dn : 0.0;
+      }
+  const int obc = 0 * 64 + sb * 8 + sc, oac = 1 * 64 + sa * 8 + sc, oab = 2 * 64 + sa * 8 + sb;
+  const double *t1 = e + EPI_S;
+  double x = 0.0;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    double ta[4], tb[4], tc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ta[u] = t1[((0 * 3 + t) * 8 + sa) * 4 + u];
+      tb[u] = t1[((1 * 3 + t) * 8 + sb) * 4 + u];
+      tc[u] = t1[((2 * 3 + t) * 8 + sc) * 4 + u];
+    }
+    const double sg = t == 1 ? -1.0 : 1.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const double *sp = e + (t * 16 + u * 4 + w) * 192;
+        const double sbc = sg * sp[obc], sac = sg * sp[oac], sab = sg * sp[oab];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int r = t == 0 ? m * 16 + u * 4 + w : (t == 1 ? u * 16 + m * 4 + w : u * 16 + w * 4 + m);
+          x += den[r] * (ta[m] * sbc - tb[m] * sac + tc[m] * sab);
+        }
+      }
+  }
+  e5 = e4 + x;
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *segs, int nseg,
+                                                           long nglobal) {
+  __shared__ f4 lds[lds_quads<PREC>()];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const unsigned lds_base = (unsigned)(unsigned long)(__attribute__((address_space(3))) f4 *)lds;
+  const int kq = lane >> 4, hi = (lane & 15) >> 2, lo = lane & 3;
+  using Pr = Prec<PREC>;
+  // Persistent loop, static round-robin over the launch's items (uniform
+  // across the workgroup, so every barrier is reached by all 8 waves).
+  for (long g = blockIdx.x; g < nglobal; g += gridDim.x) {
+    const MixedSegment &sg = segs[find_segment(segs, nseg, g)];
+    const MixedArgsG M = load_args(sg.args);
+    const FusedArgsG &A = M.f;
+    const long item = sg.item0 + (g - sg.g0);
+    const int total = 9 * (M.Ek / Pr::KC) + 9 * (M.Mk / Pr::KC);  // chunks per item
+    const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));
+    long box = item - T.item0;
+    const int bc = box % T.nb6; box /= T.nb6;
+    const int bb = box % T.nb5; box /= T.nb5;
+    const int ba = box % T.nb4; box /= T.nb4;
+    const int bk = box % T.nb3; box /= T.nb3;
+    const int bj = box % T.nb2; box /= T.nb2;
+    const int bi = (int)box;
+    const int a0 = 8 * ba, b0 = 8 * bb, c0 = 8 * bc;
+    ItemGeo G;
+    G.I = HoleBox{T.oh1, 4 * bi, T.r1 - 1};
+    G.J = HoleBox{T.oh2, 4 * bj, T.r2 - 1};
+    G.K = HoleBox{T.oh3, 4 * bk, T.r3 - 1};
+    G.a = PRange{T.op4 + a0, min(7, T.r4 - 1 - a0)};
+    G.b = PRange{T.op5 + b0, min(7, T.r5 - 1 - b0)};
+    G.c = PRange{T.op6 + c0, min(7, T.r6 - 1 - c0)};
+
+    float acc[64];  // element (i, j, k) of the lane at acc[16 i + 4 j + k]
+#pragma unroll
+    for (int r = 0; r < 64; ++r) acc[r] = 0.0f;
+
+    {  // chunks 0 and 1 (pass 0, or passes 0 and 1 when pass 0 has one chunk)
+      const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
+      const Src<PREC> s1 = pass_src<PREC, 1>(M, G, wave, lane);
+      const int n0 = chunks_of<PREC>(M, 0);
+      load_chunk<PREC>(s0, 0, lds_base, 0, wave);
+      if (n0 > 1)
+        load_chunk<PREC>(s0, (long)Pr::KC * Pr::ES, lds_base, 1, wave);
+      else
+        load_chunk<PREC>(s1, 0, lds_base, 1, wave);
+      wait_vm<3 * Pr::NPART>();
+      block_barrier();
+    }
+    int j = 0, buf = 0;
+    all_passes<PREC, 0>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
+
+    // Epilogue: the chunk buffers are free (the last chunk ended with a
+    // barrier and no load is outstanding).
+    double *epi = reinterpret_cast<double *>(lds);
+    stage_epilogue(epi, A, tid, G);
+    __syncthreads();
+    // final layout: L = c, lane = 4a' + b' + 16c'
+    const int sa = 4 * cube(wave, 0) + hi, sb = 4 * cube(wave, 1) + lo, sc = 4 * cube(wave, 2) + kq;
+    double e4, e5;
+    energy(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G, e4, e5);
+    wave_sum(e4, e5);
+    if (lane == 0) {
+      A.partials[2 * (item * 8 + wave)] = e4;
+      A.partials[2 * (item * 8 + wave) + 1] = e5;
+    }
+    __syncthreads();  // the next item's first loads overwrite the staged data
+  }
+}
+
+}  // namespace
+
+// The contraction extents of every job in the table must be multiples of
+// the precision's chunk (16 FP32 / 32 bf16 values); the job layer pads them.
+int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
+                       hipStream_t s) {
+  if (precision == TCE_T_FP32) {
+    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_FP32>, dim3((unsigned)grid), dim3(512), 0, s, segs,
+                       nseg, total);
+  } else if (precision == TCE_T_BF16X3) {
+    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_BF16X3>, dim3((unsigned)grid), dim3(512), 0, s,
+                       segs, nseg, total);
+  } else {
+    return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace tce_t

@@ -52,16 +52,19 @@ __device__ __forceinline__ void contract(acc_t (&acc)[64], const double *__restr
   }
 }
 
-__global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
+__global__ void __launch_bounds__(64, 2) fused_triples(const Segment *segs, int nseg, long total) {
   const int lane = threadIdx.x;
   const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
-  const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
-  const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
-  // Persistent loop, static round-robin over work items: the item index is
-  // wave-uniform by construction and every wave leaves after at most
-  // ceil(nitems / gridDim.x) items.
-  for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
-    const TaskDev &T = A.tasks[find_task(A.tasks, A.ntasks, item)];
+  // Persistent loop, static round-robin over the launch's items: the item
+  // index is wave-uniform by construction and every wave leaves after at
+  // most ceil(total / gridDim.x) items.
+  for (long g = blockIdx.x; g < total; g += gridDim.x) {
+    const Segment &sg = segs[find_segment(segs, nseg, g)];
+    const FusedArgsG A = load_args(sg.args);
+    const long item = sg.item0 + (g - sg.g0);
+    const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
+    const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
+    const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));
     long box = item - T.item0;
     int bc = box % T.nb6; box /= T.nb6;
     int bb = box % T.nb5; box /= T.nb5;
@@ -96,28 +99,28 @@ __global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
     const unsigned uEp = Ep, uMp = Mp, unv = nv;
     // ---- phase 1: lone particle a or b
     {  // d2 N=1..3: rows a from t(a e h h'), cols b / blocks c from <b c||h e>
-      const double *P = A.T2e + A0 * Ep, *Q = A.Vd2 + (B0 * nv + C0) * Ep;
+      const double *P = (const double *)(A.T2e + A0 * Ep), *Q = (const double *)(A.Vd2 + (B0 * nv + C0) * Ep);
       const unsigned pl = dxa * uEp, ql = (dxb * unv + dyc) * uEp;
       contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
       contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
       contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
     }
     {  // d2 N=4..6: rows a / blocks c from <a c||h e>, cols b from t(b e h h')
-      const double *P = A.T2e + B0 * Ep, *Q = A.Vd2 + (A0 * nv + C0) * Ep;
+      const double *P = (const double *)(A.T2e + B0 * Ep), *Q = (const double *)(A.Vd2 + (A0 * nv + C0) * Ep);
       const unsigned pl = dxb * uEp, ql = (dxa * unv + dyc) * uEp;
       contract<0, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
       contract<1, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
       contract<2, false>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
     }
     {  // d1 N=4..6: rows a from <m a||h h'>, cols b / blocks c from t(b c h m)
-      const double *P = A.Wd1 + A0 * Mp, *Q = A.T2m + (B0 * nv + C0) * Mp;
+      const double *P = (const double *)(A.Wd1 + A0 * Mp), *Q = (const double *)(A.T2m + (B0 * nv + C0) * Mp);
       const unsigned pl = dxa * uMp, ql = (dxb * unv + dyc) * uMp;
       contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
       contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
       contract<0, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
     }
     {  // d1 N=7..9: rows a / blocks c from t(a c h m), cols b from <m b||h h'>
-      const double *P = A.Wd1 + B0 * Mp, *Q = A.T2m + (A0 * nv + C0) * Mp;
+      const double *P = (const double *)(A.Wd1 + B0 * Mp), *Q = (const double *)(A.T2m + (A0 * nv + C0) * Mp);
       const unsigned pl = dxb * uMp, ql = (dxa * unv + dyc) * uMp;
       contract<1, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
       contract<2, false>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
@@ -126,14 +129,14 @@ __global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
     switch_layout(acc, x, y, z);
     // ---- phase 2: lone particle c
     {  // d2 N=7..9: rows c from t(c e h h'), cols b / blocks a from <a b||h e>
-      const double *P = A.T2e + C0 * Ep, *Q = A.Vd2 + (A0 * nv + B0) * Ep;
+      const double *P = (const double *)(A.T2e + C0 * Ep), *Q = (const double *)(A.Vd2 + (A0 * nv + B0) * Ep);
       const unsigned pl = dxc * uEp, ql = (dya * unv + dxb) * uEp;
       contract<0, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
       contract<1, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, -1.0);
       contract<2, true>(acc, P, pl, sT2e, Q, ql, sV, hI, hJ, hK, no, Ep, z, +1.0);
     }
     {  // d1 N=1..3: rows c from <m c||h h'>, cols b / blocks a from t(a b h m)
-      const double *P = A.Wd1 + C0 * Mp, *Q = A.T2m + (A0 * nv + B0) * Mp;
+      const double *P = (const double *)(A.Wd1 + C0 * Mp), *Q = (const double *)(A.T2m + (A0 * nv + B0) * Mp);
       const unsigned pl = dxc * uMp, ql = (dya * unv + dxb) * uMp;
       contract<1, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, -1.0);
       contract<2, true>(acc, P, pl, sW, Q, ql, sT2m, hI, hJ, hK, no, Mp, z, +1.0);
@@ -151,8 +154,8 @@ __global__ void __launch_bounds__(64, 2) fused_triples(FusedArgs A) {
 
 }  // namespace
 
-int launch_fused_wave(const FusedArgs &A, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(fused_triples, dim3((unsigned)grid), dim3(64), 0, s, A);
+int launch_fused_wave(const Segment *segs, int nseg, long total, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(fused_triples, dim3((unsigned)grid), dim3(64), 0, s, segs, nseg, total);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

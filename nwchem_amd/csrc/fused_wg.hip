@@ -64,10 +64,10 @@ struct PRange {
 // qx_first: the lane-x particle of Q is its first index (phase 1) or its
 // second (phase 2).
 struct PassDesc {
-  const double *P;
+  gptr<const double> P;
   long pS;
   PRange pr;
-  const double *Q;
+  gptr<const double> Q;
   long qS;
   PRange fr, gr;
   int pair;
@@ -78,7 +78,7 @@ struct PassDesc {
 // One thread's share of staging a chunk of a pass: three 16-byte global
 // reads (P row quad, two Q row quads) and their LDS quad slots.
 struct Stage {
-  const double *pg, *qg0, *qg1;
+  gptr<const double> pg, qg0, qg1;
   int pl, ql0, ql1;
 };
 
@@ -99,7 +99,7 @@ __device__ __forceinline__ Stage make_stage(const PassDesc &d, int tid, const Ho
     const int w = row >> 6, pp = row & 63, fs = pp >> 3, gs = pp & 7;
     const int hs = d.pair == 0 ? hole(hK, w) : (d.pair == 1 ? hole(hI, w) : hole(hJ, w));
     const long f = d.fr.g0 + min(fs, d.fr.sm), g = d.gr.g0 + min(gs, d.gr.sm);
-    const double *src = d.Q + (long)hs * d.qS + (f * nv + g) * d.Kp + 2 * q;
+    const gptr<const double> src = d.Q + (long)hs * d.qS + (f * nv + g) * d.Kp + 2 * q;
     const int sx = d.qx_first ? fs : gs, sy = d.qx_first ? gs : fs;
     const int dst = P_QUADS + (w * 4 + q) * Q_BLK + sy * 8 + q_swz(sx, sy, q);
     if (r == 0) {
@@ -113,11 +113,18 @@ __device__ __forceinline__ Stage make_stage(const PassDesc &d, int tid, const Ho
   return st;
 }
 
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 load_quad(gptr<const double> p) {
+  const dv2 v = *(gptr<const dv2>)p;
+  return double2(v.x, v.y);
+}
+
 __device__ __forceinline__ void stage_load(const Stage &st, int k, double2 &a, double2 &b,
                                            double2 &c) {
-  a = *(const double2 *)(st.pg + k);
-  b = *(const double2 *)(st.qg0 + k);
-  c = *(const double2 *)(st.qg1 + k);
+  a = load_quad(st.pg + k);
+  b = load_quad(st.qg0 + k);
+  c = load_quad(st.qg1 + k);
 }
 
 __device__ __forceinline__ void stage_store(double2 *buf, int pl, int ql0, int ql1, double2 a,
@@ -192,7 +199,7 @@ __device__ __forceinline__ void pass(acc_t (&acc)[64], double2 *lds, int &buf, c
 // hole box by 8x8x8 particle box needs and the T1(p, h) slices.  18 + 1
 // independent loads per thread, issued together.  Slots past a ragged tile
 // edge read clamped (in-range) indices; their elements are masked later.
-__device__ __forceinline__ void stage_epilogue(double *e, const FusedArgs &A, int tid,
+__device__ __forceinline__ void stage_epilogue(double *e, const FusedArgsG &A, int tid,
                                                const HoleBox (&H)[3], const PRange (&R)[3], int no,
                                                int nv) {
   double v[18];
@@ -227,7 +234,7 @@ __device__ __forceinline__ void stage_epilogue(double *e, const FusedArgs &A, in
 // (ccsd_t_dot.F:107 and the s1 table).  The d registers are overwritten
 // with den, and E(T) - E[T] = sum den * s is accumulated hole pair by hole
 // pair so that each staged S value is read once per lane.
-__device__ __forceinline__ void wg_energy(acc_t (&acc)[64], const double *e, const FusedArgs &A,
+__device__ __forceinline__ void wg_energy(acc_t (&acc)[64], const double *e, const FusedArgsG &A,
                                           const TaskDev &T, int sa, int sb, int sc, int la, int lb,
                                           int lc, int i0, int j0, int k0, const HoleBox (&H)[3],
                                           double &e4, double &e5) {
@@ -286,20 +293,23 @@ __device__ __forceinline__ void wg_energy(acc_t (&acc)[64], const double *e, con
   e5 = e4 + x;
 }
 
-__global__ void __launch_bounds__(512) fused_triples_wg(FusedArgs A) {
+__global__ void __launch_bounds__(512) fused_triples_wg(const Segment *segs, int nseg, long total) {
   __shared__ double2 lds[LDS_QUADS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int x = lane & 3, y = (lane >> 2) & 3, z = lane >> 4;
   const int ca = (wave >> 2) & 1, cb = (wave >> 1) & 1, cc = wave & 1;
-  const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
-  const long sT2e = (long)nv * Ep;      // stride of (h*no+h') in T2e
-  const long sW = (long)nv * Mp;        // stride of (h*no+h') in Wd1
-  const long sV = (long)nv * nv * Ep;   // stride of w in Vd2
-  const long sT2m = (long)nv * nv * Mp; // stride of h in T2m
-  // Persistent loop, static round-robin over work items (uniform across the
-  // workgroup, so every barrier below is reached by all 8 waves).
-  for (long item = blockIdx.x; item < A.nitems; item += gridDim.x) {
-    const TaskDev &T = A.tasks[find_task(A.tasks, A.ntasks, item)];
+  // Persistent loop, static round-robin over the launch's items (uniform
+  // across the workgroup, so every barrier below is reached by all 8 waves).
+  for (long g = blockIdx.x; g < total; g += gridDim.x) {
+    const Segment &sg = segs[find_segment(segs, nseg, g)];
+    const FusedArgsG A = load_args(sg.args);
+    const long item = sg.item0 + (g - sg.g0);
+    const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
+    const long sT2e = (long)nv * Ep;      // stride of (h*no+h') in T2e
+    const long sW = (long)nv * Mp;        // stride of (h*no+h') in Wd1
+    const long sV = (long)nv * nv * Ep;   // stride of w in Vd2
+    const long sT2m = (long)nv * nv * Mp; // stride of h in T2m
+    const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));
     long box = item - T.item0;
     const int bc = box % T.nb6; box /= T.nb6;
     const int bb = box % T.nb5; box /= T.nb5;
@@ -395,8 +405,8 @@ __global__ void __launch_bounds__(512) fused_triples_wg(FusedArgs A) {
 
 }  // namespace
 
-int launch_fused_wg(const FusedArgs &A, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(fused_triples_wg, dim3((unsigned)grid), dim3(512), 0, s, A);
+int launch_fused_wg(const Segment *segs, int nseg, long total, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(fused_triples_wg, dim3((unsigned)grid), dim3(512), 0, s, segs, nseg, total);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1,0 +1,72 @@
+// job.h -- the device-resident (T) job shared by the whole-job (job.hip)
+// and batch (batch.hip) layers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fused.h"
+#include "internal.h"
+
+namespace tce_t {
+
+struct Job {
+  Tiles T;
+  std::vector<TaskDev> tasks;
+  std::vector<long> task_tiles;  // 6 per task: p4b,p5b,p6b,h1b,h2b,h3b
+  long nitems = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double *dT1 = nullptr, *dT2e = nullptr, *dT2m = nullptr, *dVd2 = nullptr, *dWd1 = nullptr,
+         *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
+  TaskDev *dtasks = nullptr;
+  bool uploaded = false;
+  float last_ms = 0.f;
+  std::vector<double> taskE;
+  long Ep = 0, Mp = 0;
+  int grid = 0;
+  int pbox = 4;  // particle box edge of a work item: 4 (fused_wave) or 8 (fused_wg)
+  int ppi = 1;   // energy partials per work item (waves per item)
+  // reduced-precision path (fused_mixed.hip): 8-wide particle boxes always
+  std::vector<TaskDev> tasks8;
+  long nitems8 = 0;
+  TaskDev *dtasks8 = nullptr;  // == dtasks when pbox == 8
+  double *dpart8 = nullptr;    // == dpart when pbox == 8
+  // launch tables (job arguments + segments): pinned host staging, device copy
+  void *tab_host = nullptr, *tab_dev = nullptr;
+  struct Mixed {
+    void *x2[2] = {nullptr, nullptr}, *y2[2] = {nullptr, nullptr};  // T2e, Vd2
+    void *x1[2] = {nullptr, nullptr}, *y1[2] = {nullptr, nullptr};  // Wd1, T2m
+    int Ek = 0, Mk = 0;
+    bool ready = false;
+  } mx[3];  // indexed by precision (1 = FP32, 2 = BF16X3)
+};
+
+// Records a HIP failure (tce_t_last_error) and returns false.
+bool hip_ok(hipError_t e, const char *what);
+#define HIPCHK(x, what)                             \
+  do {                                              \
+    if (!::tce_t::hip_ok((x), what)) return TCE_T_ERR_HIP; \
+  } while (0)
+
+// Fixed-order per-task sums of a job's partials (ppi per item) into out
+// (2 doubles per task), on stream s.
+int launch_task_reduce(const TaskDev *tasks, int ntasks, long nitems, int ppi, const double *part,
+                       double *out, hipStream_t s);
+// Device task table and partials of 8-wide particle boxes (the cooperative
+// and reduced-precision kernels); a no-op when the job already uses them.
+int ensure_tasks8(Job &J);
+// Reduced-precision copies of the contraction tensors (converted once).
+int prepare_mixed(Job &J, int precision);
+// Kernel arguments of a job: FP64 with the job's own task table (wide =
+// false) or the 8-wide one (wide = true); reduced precision.
+FusedArgs job_args(const Job &J, bool wide);
+MixedArgs job_margs(const Job &J, int precision);
+// Free every device buffer of the job (it can be uploaded again).
+void job_release(Job &J);
+// Per-item cost weight (contraction chunks of an item) for scheduling.
+double item_cost(const Job &J, int precision);
+
+}  // namespace tce_t

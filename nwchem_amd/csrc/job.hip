@@ -43,8 +43,7 @@
 #include <thread>
 #include <vector>
 
-#include "fused.h"
-#include "internal.h"
+#include "job.h"
 
 namespace tce_t {
 namespace {
@@ -80,6 +79,33 @@ __global__ void __launch_bounds__(256) task_reduce(const TaskDev *__restrict__ t
   }
 }
 
+// Rows of K valid FP64 values (row stride kin) -> rows of kout reduced-
+// precision values, zero-padded: FP32 (out0), or bf16 hi = bf16(x) (out0)
+// and lo = bf16(x - hi) (out1), both rounded to nearest even.
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__global__ void convert_rows(const double *__restrict__ in, long rows, int K, int kin, int kout,
+                             int precision, void *out0, void *out1) {
+  const long n = rows * kout;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / kout;
+    const int k = (int)(i - r * kout);
+    const double x = k < K ? in[r * kin + k] : 0.0;
+    if (precision == TCE_T_FP32) {
+      ((float *)out0)[i] = (float)x;
+    } else {
+      const unsigned short h = bf16_rne((float)x);
+      const double hv = (double)__uint_as_float((unsigned)h << 16);
+      ((unsigned short *)out0)[i] = h;
+      ((unsigned short *)out1)[i] = bf16_rne((float)(x - hv));
+    }
+  }
+}
+
 template <class F> void parallel_for(long n, F &&f) {
   unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   std::vector<std::thread> th;
@@ -99,75 +125,40 @@ struct DenseHost {
   std::vector<double> T1, T2e, T2m, Wd1, Vd2, S;
 };
 
-struct Job {
-  Tiles T;
-  std::vector<TaskDev> tasks;
-  std::vector<long> task_tiles;  // 6 per task: p4b,p5b,p6b,h1b,h2b,h3b
-  long nitems = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double *dT1 = nullptr, *dT2e = nullptr, *dT2m = nullptr, *dVd2 = nullptr, *dWd1 = nullptr,
-         *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
-  TaskDev *dtasks = nullptr;
-  bool uploaded = false;
-  float last_ms = 0.f;
-  std::vector<double> taskE;
-  long Ep = 0, Mp = 0;
-  int grid = 0;
-  int pbox = 4;  // particle box edge of a work item: 4 (fused_wave) or 8 (fused_wg)
-  int ppi = 1;   // energy partials per work item (waves per item)
-};
 
 }  // namespace tce_t
 
 using namespace tce_t;
 
-static bool hip_ok(hipError_t e, const char *what) {
+namespace tce_t {
+bool hip_ok(hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
   char buf[256];
   snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
   set_error(buf);
   return false;
 }
-#define HIPCHK(x, what)                  \
-  do {                                   \
-    if (!hip_ok((x), what)) return TCE_T_ERR_HIP; \
-  } while (0)
+
+int launch_task_reduce(const TaskDev *tasks, int ntasks, long nitems, int ppi, const double *part,
+                       double *out, hipStream_t s) {
+  hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, s, tasks, ntasks, nitems,
+                     ppi, part, out);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+}  // namespace tce_t
 
 extern "C" int tce_t_init(int device) {
   HIPCHK(hipSetDevice(device), "hipSetDevice");
   return TCE_T_OK;
 }
 
-extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
-  if (!out) return TCE_T_ERR_ARG;
-  auto J = std::make_unique<Job>();
-  if (!J->T.load(tiles)) {
-    set_error("invalid tile description");
-    return TCE_T_ERR_ARG;
-  }
-  const Tiles &T = J->T;
-  long noab = T.noab, nvab = T.nvab, item = 0;
-  // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
-  // shared through LDS) unless rounding the particle tiles up to multiples
-  // of 8 pads the work by more than 25% over 4-wide boxes.
-  {
-    double w4 = 0, w8 = 0;
-    for (long p = noab + 1; p <= noab + nvab; ++p) {
-      double r = (double)T.rng(p);
-      w4 += std::ceil(r / 4) * 4 / r;
-      w8 += std::ceil(r / 8) * 8 / r;
-    }
-    J->pbox = (w8 <= 1.25 * w4) ? 8 : 4;
-    if (const char *k = getenv("TCE_T_KERNEL")) {
-      if (!strcmp(k, "wave")) J->pbox = 4;
-      if (!strcmp(k, "wg")) J->pbox = 8;
-    }
-    J->ppi = J->pbox == 8 ? 8 : 1;
-  }
-  const int pb = J->pbox;
-  // task loop order of ccsd_t_gpu.F:87-92
+// Task table in the order of the reference task loop (ccsd_t_gpu.F:87-92)
+// with work items of pb-wide particle boxes and 4-wide hole boxes.
+static long build_tasks(const Tiles &T, int pb, std::vector<TaskDev> &tasks,
+                        std::vector<long> *task_tiles) {
+  const long noab = T.noab, nvab = T.nvab;
+  long item = 0;
+  tasks.clear();
   for (long p4b = noab + 1; p4b <= noab + nvab; ++p4b)
     for (long p5b = p4b; p5b <= noab + nvab; ++p5b)
       for (long p6b = p5b; p6b <= noab + nvab; ++p6b)
@@ -197,11 +188,46 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
               d.item0 = item;
               d.factor = task_factor(T.restricted, p4b, p5b, p6b, h1b, h2b, h3b);
               item += (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
-              J->tasks.push_back(d);
-              long tt[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
-              J->task_tiles.insert(J->task_tiles.end(), tt, tt + 6);
+              tasks.push_back(d);
+              if (task_tiles) {
+                long tt[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
+                task_tiles->insert(task_tiles->end(), tt, tt + 6);
+              }
             }
-  J->nitems = item;
+  return item;
+}
+
+extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
+  if (!out) return TCE_T_ERR_ARG;
+  auto J = std::make_unique<Job>();
+  if (!J->T.load(tiles)) {
+    set_error("invalid tile description");
+    return TCE_T_ERR_ARG;
+  }
+  const Tiles &T = J->T;
+  const long noab = T.noab, nvab = T.nvab;
+  // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
+  // shared through LDS) unless rounding the particle tiles up to multiples
+  // of 8 pads the work by more than 25% over 4-wide boxes.
+  {
+    double w4 = 0, w8 = 0;
+    for (long p = noab + 1; p <= noab + nvab; ++p) {
+      double r = (double)T.rng(p);
+      w4 += std::ceil(r / 4) * 4 / r;
+      w8 += std::ceil(r / 8) * 8 / r;
+    }
+    J->pbox = (w8 <= 1.25 * w4) ? 8 : 4;
+    if (const char *k = getenv("TCE_T_KERNEL")) {
+      if (!strcmp(k, "wave")) J->pbox = 4;
+      if (!strcmp(k, "wg")) J->pbox = 8;
+    }
+    J->ppi = J->pbox == 8 ? 8 : 1;
+  }
+  J->nitems = build_tasks(T, J->pbox, J->tasks, &J->task_tiles);
+  if (J->pbox == 8)
+    J->tasks8 = J->tasks, J->nitems8 = J->nitems;
+  else
+    J->nitems8 = build_tasks(T, 8, J->tasks8, nullptr);
   J->Ep = round8(T.nv);
   J->Mp = round8(T.no);
   *out = reinterpret_cast<tce_t_job *>(J.release());
@@ -478,7 +504,159 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   // persistent grid: 8 waves per CU (2 per SIMD)
   J.grid = std::max(1, cus) * (J.pbox == 8 ? 1 : 8);
   J.uploaded = true;
+  if (J.pbox == 8) {
+    J.dtasks8 = J.dtasks;
+    J.dpart8 = J.dpart;
+  }
   return TCE_T_OK;
+}
+
+// Room for one job's launch table: arguments and one segment, FP64 or
+// reduced precision (the larger of the two layouts).
+constexpr size_t TAB_BYTES = 1024;
+static_assert(sizeof(MixedArgs) + sizeof(MixedSegment) <= TAB_BYTES, "launch table");
+static_assert(sizeof(FusedArgs) + sizeof(Segment) <= TAB_BYTES, "launch table");
+
+static int ensure_table(Job &J) {
+  if (!J.tab_host) HIPCHK(hipHostMalloc(&J.tab_host, TAB_BYTES), "launch table");
+  if (!J.tab_dev) HIPCHK(hipMalloc(&J.tab_dev, TAB_BYTES), "launch table");
+  return TCE_T_OK;
+}
+
+// Copy `args` and one segment covering items [0, nitems) into the job's
+// launch table (stream-ordered); returns the device segment pointer.
+template <class Args>
+static const SegmentT<Args> *stage_table(Job &J, const Args &args, long nitems) {
+  char *h = (char *)J.tab_host;
+  char *d = (char *)J.tab_dev;
+  std::memcpy(h, &args, sizeof(Args));
+  SegmentT<Args> sg;
+  sg.args = (const Args *)d;
+  sg.item0 = 0;
+  sg.item1 = nitems;
+  sg.g0 = 0;
+  const size_t off = (sizeof(Args) + 15) / 16 * 16;
+  std::memcpy(h + off, &sg, sizeof sg);
+  if (hipMemcpyAsync(d, h, off + sizeof sg, hipMemcpyHostToDevice, J.stream) != hipSuccess)
+    return nullptr;
+  return (const SegmentT<Args> *)(d + off);
+}
+
+// Contraction extent of a reduced-precision mode: K padded to the kernel's
+// chunk (16 FP32 or 32 bf16 values).
+static int mixed_extent(long K, int precision) {
+  const int kc = precision == TCE_T_FP32 ? 16 : 32;
+  return (int)((K + kc - 1) / kc * kc);
+}
+
+static long mixed_bytes(const Job &J, int precision) {
+  const long no = J.T.no, nv = J.T.nv;
+  const long Ek = mixed_extent(nv, precision), Mk = mixed_extent(no, precision);
+  const long elems = no * no * nv * Ek + no * nv * nv * Ek + no * no * nv * Mk + no * nv * nv * Mk;
+  return elems * 4;  // 4 bytes (FP32) or 2 x 2 bytes (bf16 hi, lo) per element
+}
+
+// Convert the four contraction tensors once per precision (device to
+// device, on the job's stream) and set up the pbox-8 task table.
+int tce_t::ensure_tasks8(Job &J) {
+  if (J.dtasks8) return TCE_T_OK;
+  if (J.pbox == 8) {
+    J.dtasks8 = J.dtasks;
+    J.dpart8 = J.dpart;
+    return TCE_T_OK;
+  }
+  HIPCHK(hipMalloc(&J.dtasks8, std::max<size_t>(1, J.tasks8.size()) * sizeof(TaskDev)), "tasks8");
+  HIPCHK(hipMemcpy(J.dtasks8, J.tasks8.data(), J.tasks8.size() * sizeof(TaskDev),
+                   hipMemcpyHostToDevice),
+         "tasks8");
+  HIPCHK(hipMalloc(&J.dpart8, std::max(1L, J.nitems8) * 8 * 2 * sizeof(double)), "partials8");
+  return TCE_T_OK;
+}
+
+// Convert the four contraction tensors once per precision (device to
+// device, on the job's stream) and set up the 8-wide task table.
+int tce_t::prepare_mixed(Job &J, int precision) {
+  Job::Mixed &m = J.mx[precision];
+  if (m.ready) return TCE_T_OK;
+  const long no = J.T.no, nv = J.T.nv;
+  m.Ek = mixed_extent(nv, precision);
+  m.Mk = mixed_extent(no, precision);
+  const int nparts = precision == TCE_T_FP32 ? 1 : 2;
+  const size_t es = precision == TCE_T_FP32 ? 4 : 2;
+  struct Conv {
+    const double *src;
+    long rows;
+    int K, kin, kout;
+    void **dst;
+  } ts[4] = {{J.dT2e, no * no * nv, (int)nv, (int)J.Ep, m.Ek, m.x2},
+             {J.dVd2, no * nv * nv, (int)nv, (int)J.Ep, m.Ek, m.y2},
+             {J.dWd1, no * no * nv, (int)no, (int)J.Mp, m.Mk, m.x1},
+             {J.dT2m, no * nv * nv, (int)no, (int)J.Mp, m.Mk, m.y1}};
+  for (auto &t : ts) {
+    for (int p = 0; p < nparts; ++p)
+      HIPCHK(hipMalloc(&t.dst[p], std::max<size_t>(16, (size_t)t.rows * t.kout * es)), "mixed tensors");
+    hipLaunchKernelGGL(convert_rows, dim3(4096), dim3(256), 0, J.stream, t.src, t.rows, t.K, t.kin,
+                       t.kout, precision, t.dst[0], nparts > 1 ? t.dst[1] : nullptr);
+    HIPCHK(hipGetLastError(), "convert launch");
+  }
+  {
+    const int rc = ensure_tasks8(J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(J.stream), "convert");
+  m.ready = true;
+  return TCE_T_OK;
+}
+
+FusedArgs tce_t::job_args(const Job &J, bool wide) {
+  FusedArgs A;
+  A.T1 = J.dT1;
+  A.T2e = J.dT2e;
+  A.T2m = J.dT2m;
+  A.Vd2 = J.dVd2;
+  A.Wd1 = J.dWd1;
+  A.S = J.dS;
+  A.evl = J.devl;
+  A.tasks = wide ? J.dtasks8 : J.dtasks;
+  A.ntasks = (int)J.tasks.size();
+  A.nitems = wide ? J.nitems8 : J.nitems;
+  A.no = (int)J.T.no;
+  A.nv = (int)J.T.nv;
+  A.Ep = (int)J.Ep;
+  A.Mp = (int)J.Mp;
+  A.partials = wide ? J.dpart8 : J.dpart;
+  return A;
+}
+
+MixedArgs tce_t::job_margs(const Job &J, int precision) {
+  const Job::Mixed &m = J.mx[precision];
+  MixedArgs M;
+  M.f = job_args(J, true);
+  for (int p = 0; p < 2; ++p) {
+    M.x2[p] = m.x2[p];
+    M.y2[p] = m.y2[p];
+    M.x1[p] = m.x1[p];
+    M.y1[p] = m.y1[p];
+  }
+  M.Ek = m.Ek;
+  M.Mk = m.Mk;
+  return M;
+}
+
+double tce_t::item_cost(const Job &J, int precision) {
+  if (precision == TCE_T_FP64) return (double)(J.Ep + J.Mp);
+  const Job::Mixed &m = J.mx[precision];
+  return (double)(m.Ek + m.Mk) * (precision == TCE_T_FP32 ? 0.5 : 0.25);
+}
+
+extern "C" long tce_t_job_precision_bytes(const tce_t_job *job, int precision) {
+  if (!job) return -1;
+  const Job &J = *reinterpret_cast<const Job *>(job);
+  if (precision == TCE_T_FP64) return 0;
+  if (precision != TCE_T_FP32 && precision != TCE_T_BF16X3) return -1;
+  long extra = mixed_bytes(J, precision);
+  if (J.pbox != 8) extra += (long)J.tasks8.size() * sizeof(TaskDev) + J.nitems8 * 8 * 16;
+  return extra;
 }
 
 extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt) {
@@ -488,9 +666,14 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     set_error("tce_t_job_run before tce_t_job_upload");
     return TCE_T_ERR_STATE;
   }
-  if (precision != TCE_T_FP64) {
+  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
+  }
+  const bool mixed = precision != TCE_T_FP64;
+  if (mixed) {
+    const int rc = prepare_mixed(J, precision);
+    if (rc != TCE_T_OK) return rc;
   }
   const long ntasks = (long)J.tasks.size();
   J.taskE.assign(2 * ntasks, 0.0);
@@ -499,33 +682,38 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     if (e_pt) *e_pt = 0.0;
     return TCE_T_OK;
   }
-  FusedArgs A;
-  A.T1 = J.dT1;
-  A.T2e = J.dT2e;
-  A.T2m = J.dT2m;
-  A.Vd2 = J.dVd2;
-  A.Wd1 = J.dWd1;
-  A.S = J.dS;
-  A.evl = J.devl;
-  A.tasks = J.dtasks;
-  A.ntasks = (int)ntasks;
-  A.nitems = J.nitems;
-  A.no = (int)J.T.no;
-  A.nv = (int)J.T.nv;
-  A.Ep = (int)J.Ep;
-  A.Mp = (int)J.Mp;
-  A.partials = J.dpart;
-  long grid = std::min<long>(J.grid, J.nitems);
-  HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
-  if ((J.pbox == 8 ? launch_fused_wg(A, (int)grid, J.stream)
-                    : launch_fused_wave(A, (int)grid, J.stream)) != 0) {
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
+  {
+    const int rc = ensure_table(J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  int lrc;
+  const FusedArgs A = job_args(J, mixed || J.pbox == 8);
+  if (mixed) {
+    const MixedSegment *sg = stage_table(J, job_margs(J, precision), J.nitems8);
+    if (!sg) return TCE_T_ERR_HIP;
+    HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
+    lrc = launch_fused_mixed(sg, 1, J.nitems8, precision,
+                             (int)std::min<long>(std::max(1, cus), J.nitems8), J.stream);
+  } else {
+    const long grid = std::min<long>(J.grid, J.nitems);
+    const Segment *sg = stage_table(J, A, J.nitems);
+    if (!sg) return TCE_T_ERR_HIP;
+    HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
+    lrc = J.pbox == 8 ? launch_fused_wg(sg, 1, J.nitems, (int)grid, J.stream)
+                      : launch_fused_wave(sg, 1, J.nitems, (int)grid, J.stream);
+  }
+  if (lrc != 0) {
     set_error("fused kernel launch failed");
     return TCE_T_ERR_HIP;
   }
   HIPCHK(hipEventRecord(J.ev1, J.stream), "event");
-  hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, J.stream, J.dtasks,
-                     (int)ntasks, J.nitems, J.ppi, J.dpart, J.dtaskE);
-  HIPCHK(hipGetLastError(), "task_reduce launch");
+  if (launch_task_reduce(A.tasks, (int)ntasks, A.nitems, mixed ? 8 : J.ppi, A.partials, J.dtaskE,
+                         J.stream) != 0) {
+    set_error("task_reduce launch failed");
+    return TCE_T_ERR_HIP;
+  }
   HIPCHK(hipMemcpyAsync(J.taskE.data(), J.dtaskE, 2 * ntasks * sizeof(double), hipMemcpyDeviceToHost,
                         J.stream),
          "D2H");
@@ -558,14 +746,35 @@ extern "C" int tce_t_job_kernel(const tce_t_job *job) {
   return reinterpret_cast<const Job *>(job)->pbox == 8 ? TCE_T_KERNEL_WG : TCE_T_KERNEL_WAVE;
 }
 
+void tce_t::job_release(Job &J) {
+  if (J.stream) (void)hipStreamSynchronize(J.stream);
+  if (J.dtasks8 && J.dtasks8 != J.dtasks) (void)hipFree(J.dtasks8);
+  if (J.dpart8 && J.dpart8 != J.dpart) (void)hipFree(J.dpart8);
+  J.dtasks8 = nullptr;
+  J.dpart8 = nullptr;
+  double **bufs[] = {&J.dT1, &J.dT2e, &J.dT2m, &J.dVd2, &J.dWd1, &J.dS, &J.devl, &J.dpart, &J.dtaskE};
+  for (double **b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+  if (J.dtasks) (void)hipFree(J.dtasks);
+  J.dtasks = nullptr;
+  for (auto &m : J.mx) {
+    for (void **b : {&m.x2[0], &m.x2[1], &m.y2[0], &m.y2[1], &m.x1[0], &m.x1[1], &m.y1[0], &m.y1[1]}) {
+      if (*b) (void)hipFree(*b);
+      *b = nullptr;
+    }
+    m.ready = false;
+  }
+  J.uploaded = false;
+}
+
 extern "C" int tce_t_job_destroy(tce_t_job *job) {
   if (!job) return TCE_T_OK;
   Job *J = reinterpret_cast<Job *>(job);
-  if (J->stream) (void)hipStreamSynchronize(J->stream);
-  double *bufs[] = {J->dT1, J->dT2e, J->dT2m, J->dVd2, J->dWd1, J->dS, J->devl, J->dpart, J->dtaskE};
-  for (double *b : bufs)
-    if (b) (void)hipFree(b);
-  if (J->dtasks) (void)hipFree(J->dtasks);
+  job_release(*J);
+  if (J->tab_dev) (void)hipFree(J->tab_dev);
+  if (J->tab_host) (void)hipHostFree(J->tab_host);
   if (J->ev0) (void)hipEventDestroy(J->ev0);
   if (J->ev1) (void)hipEventDestroy(J->ev1);
   if (J->stream) (void)hipStreamDestroy(J->stream);

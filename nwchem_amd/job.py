@@ -18,7 +18,9 @@ from . import _lib
 from ._lib import DP, LP, check
 from .tiles import Blocks, TileModel
 
-FP64 = 0
+FP64 = 0    # d1/d2 contractions on FP64 MFMA
+FP32 = 1    # FP32 operands, FP32 MFMA and accumulation
+BF16X3 = 2  # operands split into bf16 hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulation
 
 _initialised = False
 
@@ -60,6 +62,10 @@ class TriplesJob:
               "tce_t_job_task_energies")
         return out
 
+    def precision_bytes(self, precision: int) -> int:
+        """Extra device bytes the first run in `precision` allocates."""
+        return int(_lib.lib().tce_t_job_precision_bytes(self._h, precision))
+
     @property
     def last_kernel_ms(self) -> float:
         return float(_lib.lib().tce_t_job_last_kernel_ms(self._h))
@@ -70,9 +76,64 @@ class TriplesJob:
         k = int(_lib.lib().tce_t_job_kernel(self._h))
         return {0: "fused_triples", 1: "fused_triples_wg"}[k]
 
+    @staticmethod
+    def kernel_for(precision: int, fp64_kernel: str) -> str:
+        """Kernel that runs the contractions in `precision`."""
+        return fp64_kernel if precision == FP64 else "fused_triples_mixed"
+
     def close(self) -> None:
         if self._h:
             _lib.lib().tce_t_job_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed"}
+
+
+class TriplesBatch:
+    """Several uploaded TriplesJobs run by one persistent launch per call
+    (autotuned on the first call in a precision, then replayed as a
+    hipGraph).  The jobs must stay open while the batch exists."""
+
+    def __init__(self, jobs):
+        self.jobs = list(jobs)
+        arr = (C.c_void_p * len(self.jobs))(*[j._h for j in self.jobs])
+        self._h = C.c_void_p()
+        check(_lib.lib().tce_t_batch_create(arr, len(self.jobs), C.byref(self._h)), "tce_t_batch_create")
+
+    def run(self, precision: int = FP64) -> np.ndarray:
+        """(njobs, 2) array of E[T], E(T)."""
+        out = np.zeros((len(self.jobs), 2))
+        check(_lib.lib().tce_t_batch_run(self._h, precision, out.ctypes.data_as(DP)), "tce_t_batch_run")
+        return out
+
+    @property
+    def last_ms(self) -> float:
+        return float(_lib.lib().tce_t_batch_last_ms(self._h))
+
+    def plan(self, precision: int) -> dict:
+        fam, co = C.c_int(), C.c_int()
+        tms = np.zeros(4)
+        check(_lib.lib().tce_t_batch_plan(self._h, precision, C.byref(fam), C.byref(co),
+                                          tms.ctypes.data_as(DP)), "tce_t_batch_plan")
+        return {"kernel": FAMILIES[fam.value], "cost_order": bool(co.value),
+                "tuned_ms": [float(x) for x in tms if x >= 0]}
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().tce_t_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

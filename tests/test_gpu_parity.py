@@ -123,3 +123,179 @@ def test_fortran_caller_matches_oracle():
     tm, b, ot = system(8, 32, 8, 16, 1234)
     e1, e2, _ = oracle_energies(ot, b)
     assert abs(float(m.group(1)) - e1) <= tol(e1) and abs(float(m.group(2)) - e2) <= tol(e2)
+
+
+# Reduced-precision modes: the d1/d2 products in FP32, or split-BF16 with
+# three terms (hi*hi + hi*lo + lo*hi), FP32 accumulation; s1, denominators,
+# division and every energy sum in FP64.  The bound is relative to |E|: a
+# layout, sign or indexing mistake moves E by O(1) relative, while FP32
+# rounding (2^-24) and the bf16 hi+lo representation (2^-17 per operand,
+# unbiased) leave |dE|/|E| many orders below these bounds on these sizes.
+MIXED_REL = {1: 2e-6, 2: 1e-4}
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
+def test_mixed_precision_matches_oracle(case, precision):
+    no, nv, to, tv, seed, ro, rv = case
+    tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
+    e1, e2, te = oracle_energies(ot, b)
+    rel = MIXED_REL[precision]
+    print(f"case no={no} nv={nv} ranges={list(tm.range)} precision={precision}", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        e = job.run(precision)
+        gte = job.task_energies()
+    print(f"  E[T] {e[0]:.12e} vs {e1:.12e} (dE {e[0] - e1:.3e}), "
+          f"E(T) {e[1]:.12e} vs {e2:.12e} (dE {e[1] - e2:.3e})", flush=True)
+    assert abs(e[0] - e1) <= rel * abs(e1) + 1e-12
+    assert abs(e[1] - e2) <= rel * abs(e2) + 1e-12
+    scale = np.abs(te).max()
+    assert np.all(np.abs(gte - te) <= rel * scale + 1e-12)
+
+
+def test_mixed_precision_deterministic_and_fp64_unchanged():
+    """Reduced-precision runs repeat bit for bit, and running them leaves
+    the FP64 result of the same job untouched."""
+    tm, b, ot = system(8, 32, 8, 16, 77)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        f1 = job.run(0)
+        a = [job.run(p) for p in (1, 2, 1, 2)]
+        f2 = job.run(0)
+    assert a[0] == a[2] and a[1] == a[3]
+    assert f1 == f2
+    assert a[0] != f1 and a[1] != f1
+
+
+# ---- batches: several jobs in one persistent launch (hipGraph replays)
+BATCH = [(8, 32, 8, 16, 21, None, None), (7, 13, 3, 5, 22, None, None),
+         (10, 22, 4, 6, 23, None, None), (9, 41, 0, 0, 24, [6, 3], [16, 16, 9]),
+         (6, 17, 6, 17, 25, None, None)]
+
+
+def _batch_systems():
+    out = []
+    for no, nv, to, tv, seed, ro, rv in BATCH:
+        tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
+        out.append((tm, b, ot))
+    return out
+
+
+def test_batch_matches_oracle_and_single_runs():
+    """FP64 batch energies match the oracle per job; reduced-precision batch
+    energies equal each job's own run bit for bit (same kernel, each job's
+    partials reduced in its own fixed order)."""
+    from nwchem_amd import TriplesBatch
+    systems = _batch_systems()
+    jobs = []
+    try:
+        for tm, b, ot in systems:
+            j = TriplesJob(tm)
+            j.upload(b)
+            jobs.append(j)
+        singles = {p: [j.run(p) for j in jobs] for p in (1, 2)}
+        with TriplesBatch(jobs) as batch:
+            e64 = batch.run(0)
+            print("fp64 plan", batch.plan(0), flush=True)
+            for k, (tm, b, ot) in enumerate(systems):
+                e1, e2, te = oracle_energies(ot, b)
+                assert abs(e64[k, 0] - e1) <= tol(e1), (k, e64[k], e1)
+                assert abs(e64[k, 1] - e2) <= tol(e2), (k, e64[k], e2)
+                assert_parity(tuple(e64[k]), jobs[k].task_energies(), (e1, e2, te))
+            for p in (1, 2):
+                ep = batch.run(p)
+                print(p, "plan", batch.plan(p), flush=True)
+                for k in range(len(jobs)):
+                    assert tuple(ep[k]) == singles[p][k], (p, k, ep[k], singles[p][k])
+    finally:
+        for j in jobs:
+            j.close()
+
+
+def test_batch_graph_replays_are_identical():
+    """The captured graph replayed several times, with eager single-job runs
+    and synchronisations in between, gives the eager result every time."""
+    from nwchem_amd import TriplesBatch
+    systems = _batch_systems()[:3]
+    jobs = []
+    try:
+        for tm, b, ot in systems:
+            j = TriplesJob(tm)
+            j.upload(b)
+            jobs.append(j)
+        with TriplesBatch(jobs) as batch:
+            for p in (0, 2):
+                first = batch.run(p)  # plans (tuning runs are eager) and replays once
+                for rep in range(3):
+                    eager = [jobs[k].run(p) for k in range(len(jobs))]
+                    again = batch.run(p)
+                    assert np.array_equal(again, first), (p, rep)
+                    if p == 2:
+                        assert [tuple(x) for x in again] == eager
+    finally:
+        for j in jobs:
+            j.close()
+
+
+# ---- serving: continuous batching of requests
+def test_server_results_same_alone_and_in_the_mix():
+    """Each request's energies are identical (bit for bit) whether it ran
+    alone on the server or overlapped with others of different sizes and
+    precisions; FP64 requests also match the oracle."""
+    from nwchem_amd import TriplesServer
+    systems = _batch_systems()
+    reqs = [(k, p) for k in range(len(systems)) for p in (0, 2)] + [(0, 1), (3, 1)]
+    alone = {}
+    with TriplesServer(slice_ms=2.0, max_active=8, reserve_bytes=0) as srv:
+        sids = [srv.add_system(tm, b) for tm, b, ot in systems]
+        for k, p in reqs:  # one at a time
+            r = srv.wait(srv.submit(sids[k], p))
+            alone[(k, p)] = (r["E_bt"], r["E_pt"])
+    with TriplesServer(slice_ms=2.0, max_active=8, reserve_bytes=0) as srv:
+        sids = [srv.add_system(tm, b) for tm, b, ot in systems]
+        tickets = [(k, p, srv.submit(sids[k], p)) for k, p in reqs]  # all at once
+        mix = {(k, p): srv.wait(t) for k, p, t in tickets}
+        st = srv.stats()
+    print("slices", st, flush=True)
+    overlapped = 0
+    for (k, p), r in mix.items():
+        assert (r["E_bt"], r["E_pt"]) == alone[(k, p)], (k, p)
+        assert r["submit"] <= r["admit"] <= r["start"] <= r["finish"]
+    spans = sorted((r["start"], r["finish"]) for r in mix.values())
+    for a, b in zip(spans, spans[1:]):
+        overlapped += b[0] < a[1]
+    assert overlapped > 0  # requests shared the device
+    for k, (tm, b, ot) in enumerate(systems):
+        e1, e2, te = oracle_energies(ot, b)
+        assert abs(alone[(k, 0)][0] - e1) <= tol(e1)
+        assert abs(alone[(k, 0)][1] - e2) <= tol(e2)
+
+
+def test_server_evicts_idle_systems():
+    """With free memory for about two and a half systems and two slots,
+    requests for three different systems all complete: the third evicts an
+    idle one.  Each result matches the system's own job run."""
+    from nwchem_amd import TriplesServer
+    import torch  # device memory query only
+    systems = [system(8, 64, 8, 16, 31 + k) for k in range(3)]
+    ref, one = [], 0
+    for tm, b, ot in systems:
+        with TriplesJob(tm) as j:
+            j.upload(b)
+            ref.append(j.run())
+            one = max(one, j.device_bytes)
+    free_b, total_b = torch.cuda.mem_get_info(0)
+    reserve = max(0, free_b - (5 * one) // 2)
+    print(f"system bytes {one}, free {free_b}, reserve {reserve}", flush=True)
+    with TriplesServer(slice_ms=2.0, max_active=2, reserve_bytes=reserve) as srv:
+        sids = [srv.add_system(tm, b) for tm, b, ot in systems]
+        rs = [srv.wait(srv.submit(s, 0)) for s in sids[:1]]  # warm: one resident
+        ts = [srv.submit(s, 0) for s in sids]
+        rs = [srv.wait(t) for t in ts]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    assert st["resident"] <= 3
+    for (e1, e2), r in zip(ref, rs):
+        assert abs(r["E_bt"] - e1) <= tol(e1)
+        assert abs(r["E_pt"] - e2) <= tol(e2)
