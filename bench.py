@@ -149,6 +149,8 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
         del b
         jobs.append(j)
         F += f_ref(tm)
+        print(f"[bench] batch job {k}: no={no} nv={nv} tiles {to}/{tv} ({len(tm.tasks())} tasks) "
+              f"ready at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
     out = {"jobs": njobs, "flops_per_run": F, "prep_s": t_prep}
     names = {0: "fp64", 1: "fp32", 2: "bf16x3"}
@@ -195,6 +197,8 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -
         nv = int(round(np.exp(rng.uniform(np.log(48), np.log(320)))))
         tm = tl.TileModel.synthetic(no, nv, min(no, 20), min(nv, 40), seed + 100 + k)
         pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), f_ref(tm)))
+        print(f"[bench] serve system {k}: no={no} nv={nv} ready at {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
     arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
     picks = rng.integers(0, npool, nreq)
@@ -209,7 +213,17 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -
             if dt > 0:
                 time.sleep(dt)
             tickets.append(srv.submit(sids[picks[i]], int(precs[i])))
-        res = [srv.wait(t) for t in tickets]
+        print(f"[bench] serve: {nreq} requests submitted in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        res = []
+        for i, t in enumerate(tickets):
+            while True:
+                try:
+                    res.append(srv.wait(t, 30.0))
+                    break
+                except TimeoutError:
+                    print(f"[bench] serve: waiting for request {i} at {time.perf_counter() - t0:.1f} s",
+                          file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         st = srv.stats()
     for i, r in enumerate(res):
@@ -240,8 +254,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the reference CPU kernels (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--batch-jobs", type=int, default=64, help="config-4 batch size (0: skip)")
-    ap.add_argument("--serve", type=int, default=40, help="serving requests (0: skip)")
+    ap.add_argument("--batch-jobs", type=int, default=16,
+                    help="batch of independent jobs (config 4 has 64; 0: skip)")
+    ap.add_argument("--serve", type=int, default=24, help="serving requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--slice-ms", type=float, default=50.0)

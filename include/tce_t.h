@@ -129,6 +129,8 @@ typedef struct tce_t_job tce_t_job;
 
 /* Bind the library to a device (call once per process). */
 int tce_t_init(int device);
+/* Free and total device memory in bytes (hipMemGetInfo) of the current device. */
+int tce_t_device_memory(long *free_bytes, long *total_bytes);
 
 /* Plan a job: enumerate the tile sextuples of ccsd_t_gpu.F:87-112 and size
  * the device tensors.  The tile arrays are copied. */

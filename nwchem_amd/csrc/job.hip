@@ -13,26 +13,23 @@
 //              S  [h][h'][p][p'] = <p p'||h h'>      (s1 integral)
 //              T1 [p][h]         = t(p h)
 //            (contraction extents padded with zeros to a multiple of 8).
-//   run      one persistent kernel; each work item is a 4x4x4 particle cube
-//            by 4x4x4 hole box of one task's t3 tile.  A wavefront keeps the
-//            4096 t3 elements of its item in 64 accumulator registers, runs
-//            all 18 d1/d2 contractions of the item on v_mfma_f64_4x4x4_4b,
-//            then adds the 9 s1 products and the denominator-weighted energy
-//            in FP64 and writes one {E[T],E(T)} partial.  t3 never touches
-//            memory.  A second kernel sums the partials of each task in a
-//            fixed order (bit-reproducible task energies).
+//   run      one persistent fused kernel over the job's work items (task x
+//            particle box x 4x4x4 hole box): fused_triples_wg (8-wide
+//            particle boxes, LDS-staged operands), fused_triples (4-wide
+//            boxes) for small or ragged tilings, or fused_triples_mixed in a
+//            reduced precision.  Each item's t3 block lives in registers for
+//            all 18 d1/d2 contractions, then gets its 9 s1 products and the
+//            denominator-weighted energy in FP64; one {E[T], E(T)} partial
+//            per wave.  A second kernel sums each task's partials in a fixed
+//            order (bit-reproducible task energies); the host sums tasks in
+//            task-loop order.
+//   reduced  the first run in FP32 / split-BF16 converts the four
+//            contraction tensors on the device (zero-padded to the kernel's
+//            chunk) and builds the 8-wide task table when the job's own
+//            kernel uses 4-wide boxes.
 //
-// Register layout of an item (one double per lane per register r):
-//   lane = x + 4y + 16z, register r = 16*ii + 4*jj + kk (hole offsets)
-//   phase 1: (a,b,c) = (z, x, y)    rows a, cols b, blocks c
-//   phase 2: (a,b,c) = (y, x, z)    rows c, cols b, blocks a
-// v_mfma_f64_4x4x4_4b reads A[i][k] of block q from lane i+4q+16k, B[k][j]
-// from lane j+4q+16k and writes D[i][j] to lane j+4q+16i, so a contraction
-// can run on the accumulators iff its row index and column index sit on
-// different operands; the block index may sit on either (the other operand is
-// replicated).  Phase 1 carries the 12 contractions whose lone particle is a
-// or b (d2 N=1..6, d1 N=4..9), phase 2 the 6 whose lone particle is c (d2
-// N=7..9, d1 N=1..3); one lane permutation switches the layouts.
+// Kernel launches take a table of (job arguments, item range) segments in
+// device memory (one segment here; many in batch.hip and server.cpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -146,6 +143,14 @@ int launch_task_reduce(const TaskDev *tasks, int ntasks, long nitems, int ppi, c
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }  // namespace tce_t
+
+extern "C" int tce_t_device_memory(long *free_bytes, long *total_bytes) {
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  if (free_bytes) *free_bytes = (long)fr;
+  if (total_bytes) *total_bytes = (long)tot;
+  return TCE_T_OK;
+}
 
 extern "C" int tce_t_init(int device) {
   HIPCHK(hipSetDevice(device), "hipSetDevice");

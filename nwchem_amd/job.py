@@ -32,6 +32,13 @@ def init(device: int = 0) -> None:
         _initialised = True
 
 
+def device_memory() -> tuple[int, int]:
+    """(free, total) bytes of the current device."""
+    fr, tot = C.c_long(), C.c_long()
+    check(_lib.lib().tce_t_device_memory(C.byref(fr), C.byref(tot)), "tce_t_device_memory")
+    return fr.value, tot.value
+
+
 class TriplesJob:
     def __init__(self, tiles: TileModel, device: int = 0):
         init(device)

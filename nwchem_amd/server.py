@@ -19,6 +19,9 @@ from .job import init
 from .tiles import Blocks, TileModel
 
 
+STATUS_TIMEOUT = -5  # TCE_T_ERR_STATE: the wait timed out
+
+
 class TriplesServer:
     def __init__(self, device: int = 0, slice_ms: float = 50.0, max_active: int = 16,
                  reserve_bytes: int = 4 << 30):
@@ -48,6 +51,8 @@ class TriplesServer:
         tm = np.zeros(4)
         rc = _lib.lib().tce_t_server_wait(self._h, ticket, timeout, C.byref(e1), C.byref(e2),
                                           tm.ctypes.data_as(DP))
+        if rc == STATUS_TIMEOUT and tm[3] == 0.0 and timeout >= 0:
+            raise TimeoutError(f"request {ticket} not finished after {timeout} s")
         check(rc, "tce_t_server_wait")
         return {"E_bt": e1.value, "E_pt": e2.value, "submit": tm[0], "admit": tm[1],
                 "start": tm[2], "finish": tm[3]}

@@ -277,7 +277,7 @@ def test_server_evicts_idle_systems():
     requests for three different systems all complete: the third evicts an
     idle one.  Each result matches the system's own job run."""
     from nwchem_amd import TriplesServer
-    import torch  # device memory query only
+    from nwchem_amd.job import device_memory
     systems = [system(8, 64, 8, 16, 31 + k) for k in range(3)]
     ref, one = [], 0
     for tm, b, ot in systems:
@@ -285,7 +285,7 @@ def test_server_evicts_idle_systems():
             j.upload(b)
             ref.append(j.run())
             one = max(one, j.device_bytes)
-    free_b, total_b = torch.cuda.mem_get_info(0)
+    free_b, total_b = device_memory()
     reserve = max(0, free_b - (5 * one) // 2)
     print(f"system bytes {one}, free {free_b}, reserve {reserve}", flush=True)
     with TriplesServer(slice_ms=2.0, max_active=2, reserve_bytes=reserve) as srv:
