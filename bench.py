@@ -156,20 +156,29 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
     names = {0: "fp64", 1: "fp32", 2: "bf16x3"}
     with TriplesBatch(jobs) as batch:
         for p in modes:
+            print(f"[bench] batch {names[p]}: planning (one timed run per candidate, graph capture)",
+                  file=sys.stderr, flush=True)
             t1 = time.perf_counter()
             batch.run(p)  # plan: tuning runs, graph capture, first replay
             t_plan = time.perf_counter() - t1
+            print(f"[bench] batch {names[p]}: planned in {t_plan:.1f} s: {batch.plan(p)}",
+                  file=sys.stderr, flush=True)
             ms = []
             t1 = time.perf_counter()
-            for _ in range(steps):
+            for st in range(steps):
                 batch.run(p)
                 ms.append(batch.last_ms)
+                print(f"[bench] batch {names[p]}: replay {st} {batch.last_ms:.1f} ms",
+                      file=sys.stderr, flush=True)
             wall = (time.perf_counter() - t1) / steps
             seq = 0.0
-            for j in jobs:  # the same jobs one launch each
+            for k, j in enumerate(jobs):  # the same jobs one launch each
                 t2 = time.perf_counter()
                 j.run(p)
                 seq += time.perf_counter() - t2
+                if k % 16 == 15:
+                    print(f"[bench] batch {names[p]}: one by one, {k + 1} jobs in {seq:.1f} s",
+                          file=sys.stderr, flush=True)
             out[names[p]] = {"ms_per_run": wall * 1e3, "graph_ms": float(np.mean(ms)),
                              "energies_per_s": njobs / wall, "fp64_equiv_tflops": F / wall / 1e12,
                              "sequential_ms": seq * 1e3, "speedup_vs_sequential": seq / wall,

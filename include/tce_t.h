@@ -157,6 +157,16 @@ int tce_t_job_upload(tce_t_job *job, const double *t1, const long *k_t1_offset,
  * (the first run in a reduced precision converts the contraction tensors on
  * the device; see tce_t_job_precision_bytes).  Synchronous. */
 int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt);
+/* Restartable run (ccsd_t_restart.F:58-262): the tasks in blocks of one
+ * outer virtual tile p4b = noab + b, for b = *begin .. nvab (1-based).
+ * Each block's E(T) goes into table[b-1] (nvab entries, kept by the caller
+ * like tce:restart_triples_table) and *begin advances past it (like
+ * tce:ccsd_t_restart_begin), at most max_tiles blocks per call (<= 0: all),
+ * so a caller that saves (*begin, table) after each call can resume after a
+ * stop.  Once *begin > nvab, *t_energy = sum(table).  Table entries of
+ * blocks without allowed tasks are 0. */
+int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin, double *table,
+                          long max_tiles, double *t_energy);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);
@@ -179,6 +189,7 @@ double tce_t_job_last_kernel_ms(const tce_t_job *job);
  * tce_t_job_create from the tile shapes.  Negative on a NULL job. */
 #define TCE_T_KERNEL_WAVE 0
 #define TCE_T_KERNEL_WG 1
+#define TCE_T_KERNEL_WG16 2 /* 16x16 FP64 MFMA (fused_triples_mixed); TCE_T_KERNEL=wg16 */
 int tce_t_job_kernel(const tce_t_job *job);
 int tce_t_job_destroy(tce_t_job *job);
 
@@ -196,10 +207,11 @@ int tce_t_batch_run(tce_t_batch *batch, int precision, double *energies);
 /* Device time (ms) of the last run (graph launch to completion). */
 double tce_t_batch_last_ms(const tce_t_batch *batch);
 /* The plan chosen for `precision`: family 0 = fused_triples (4-wide boxes),
- * 1 = fused_triples_wg, 2 = fused_triples_mixed; cost_order 1 = costliest
- * work items first; tuned_ms[4] = candidate times of the tuning runs
- * (family x order, -1 where not tried, all -1 when the plan came from the
- * process-wide cache). */
+ * 1 = fused_triples_wg, 2 = fused_triples_mixed (reduced precision), 3 =
+ * fused_triples_mixed on FP64 (16x16x4); cost_order 1 = costliest work
+ * items first; tuned_ms[6] = candidate times of the tuning runs (family x
+ * order in trial order, -1 where not tried, all -1 when the plan came from
+ * the process-wide cache). */
 int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family, int *cost_order,
                      double *tuned_ms);
 int tce_t_batch_destroy(tce_t_batch *batch);

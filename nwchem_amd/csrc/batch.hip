@@ -7,7 +7,8 @@
 // pinned host memory.  The first run in a precision plans the launch:
 //
 //   autotuner  candidates are the kernel family for FP64 (fused_triples_wg
-//              with 8-wide particle boxes, fused_triples with 4-wide) and
+//              with 8-wide particle boxes on 4x4x4 MFMA, fused_triples_mixed
+//              on 16x16x4, fused_triples with 4-wide boxes) and
 //              the segment order (as given; costliest items first, so the
 //              static round-robin schedule ends on the cheapest items).
 //              Each candidate runs once, timed with HIP events; the fastest
@@ -36,7 +37,8 @@
 namespace tce_t {
 namespace {
 
-enum Family { FAM_WAVE = 0, FAM_WG = 1, FAM_MIXED = 2 };
+enum Family { FAM_WAVE = 0, FAM_WG = 1, FAM_MIXED = 2, FAM_WG16 = 3 };
+constexpr int NCAND = 6;  // at most 3 families x 2 orders
 
 struct Choice {
   int family = FAM_WG;
@@ -54,7 +56,7 @@ struct Plan {
   int nseg = 0, grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  double tuned_ms[4] = {-1, -1, -1, -1};  // candidate times (family x order)
+  double tuned_ms[NCAND] = {-1, -1, -1, -1, -1, -1};  // candidate times (family x order)
 };
 
 struct Batch {
@@ -95,7 +97,8 @@ std::vector<int> job_order(const Batch &B, int precision, bool cost_order) {
 
 // Build the device table (job arguments, then segments) of a choice.
 int build_table(Batch &B, int precision, const Choice &c, Plan &P) {
-  const bool mixed = precision != TCE_T_FP64;
+  // the 16x16 kernel (reduced precision, or FP64 with family WG16) takes MixedArgs
+  const bool mixed = precision != TCE_T_FP64 || c.family == FAM_WG16;
   const bool wide = mixed || c.family == FAM_WG;
   const std::vector<int> ord = job_order(B, precision, c.cost_order);
   const size_t asz = ((mixed ? sizeof(MixedArgs) : sizeof(FusedArgs)) + 15) / 16 * 16;
@@ -138,7 +141,7 @@ int build_table(Batch &B, int precision, const Choice &c, Plan &P) {
 
 // Enqueue the launch sequence of a plan on the batch stream.
 int enqueue(Batch &B, int precision, const Plan &P) {
-  const bool mixed = precision != TCE_T_FP64;
+  const bool mixed = precision != TCE_T_FP64 || P.choice.family == FAM_WG16;
   const bool wide = mixed || P.choice.family == FAM_WG;
   const size_t n = B.jobs.size();
   const size_t asz = ((mixed ? sizeof(MixedArgs) : sizeof(FusedArgs)) + 15) / 16 * 16;
@@ -206,7 +209,8 @@ int plan_batch(Batch &B, int precision) {
     bool all4 = true;
     for (const Job *J : B.jobs) all4 = all4 && J->pbox == 4;
     std::vector<int> fams = mixed ? std::vector<int>{FAM_MIXED}
-                                  : (all4 ? std::vector<int>{FAM_WG, FAM_WAVE} : std::vector<int>{FAM_WG});
+                                  : (all4 ? std::vector<int>{FAM_WG, FAM_WG16, FAM_WAVE}
+                                          : std::vector<int>{FAM_WG, FAM_WG16});
     std::vector<Choice> cands;
     for (int fam : fams)
       for (bool co : {true, false}) cands.push_back(Choice{fam, co});
@@ -323,7 +327,7 @@ extern "C" int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *fa
   if (family) *family = P.choice.family;
   if (cost_order) *cost_order = P.choice.cost_order ? 1 : 0;
   if (tuned_ms)
-    for (int i = 0; i < 4; ++i) tuned_ms[i] = P.tuned_ms[i];
+    for (int i = 0; i < NCAND; ++i) tuned_ms[i] = P.tuned_ms[i];
   return TCE_T_OK;
 }
 

@@ -138,6 +138,16 @@ __device__ __forceinline__ MixedArgsG load_args(const MixedArgs *p) {
   return M;
 }
 
+// First item of workgroup b in a persistent launch of G workgroups.  The
+// hardware deals workgroups to the 8 XCDs round-robin (b -> XCD b % 8), and
+// each XCD has its own L2; mapping b to (b % 8) * (G / 8) + b / 8 hands each
+// XCD a contiguous run of items, so the items that share operand slices
+// (consecutive boxes of one task) meet in the same L2.  The loop then steps
+// by G as before; any G works (a G that is not a multiple of 8 keeps b).
+__device__ __forceinline__ long first_item(unsigned b, unsigned G) {
+  return (G % 8u) ? (long)b : (long)(b % 8u) * (G / 8u) + b / 8u;
+}
+
 // Segment of global index g (uniform): the last with g0 <= g.
 template <class Args>
 __device__ __forceinline__ int find_segment(const SegmentT<Args> *s, int n, long g) {

@@ -49,17 +49,28 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 template <int PREC>
 struct Prec;
 template <>
-struct Prec<TCE_T_FP32> {
-  static constexpr int KC = 16;    // contraction values per chunk
+struct Prec<TCE_T_FP64> {
+  static constexpr int KC = 8;     // contraction values per chunk
   static constexpr int NPART = 1;  // operand parts
-  static constexpr int ES = 4;     // bytes per element
+  static constexpr int ES = 8;     // bytes per element
+  typedef double acc;
+};
+template <>
+struct Prec<TCE_T_FP32> {
+  static constexpr int KC = 16;
+  static constexpr int NPART = 1;
+  static constexpr int ES = 4;
+  typedef float acc;
 };
 template <>
 struct Prec<TCE_T_BF16X3> {
   static constexpr int KC = 32;
   static constexpr int NPART = 2;  // hi, lo
   static constexpr int ES = 2;
+  typedef float acc;
 };
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double dv2 __attribute__((ext_vector_type(2)));
 
 constexpr int X_QUADS = 512, Y_QUADS = 1024, PART_QUADS = X_QUADS + Y_QUADS;
 constexpr int NBUF = 3;
@@ -227,19 +238,6 @@ __device__ __forceinline__ void load_chunk(const Src<PREC> &S, long kb, unsigned
   }
 }
 
-template <int PREC>
-__device__ __forceinline__ Src<PREC> select_src(bool first, const Src<PREC> &a, bool second,
-                                                const Src<PREC> &b, const Src<PREC> &c) {
-  Src<PREC> S;
-#pragma unroll
-  for (int part = 0; part < Prec<PREC>::NPART; ++part) {
-    S.x[part] = first ? a.x[part] : (second ? b.x[part] : c.x[part]);
-    S.y0[part] = first ? a.y0[part] : (second ? b.y0[part] : c.y0[part]);
-    S.y1[part] = first ? a.y1[part] : (second ? b.y1[part] : c.y1[part]);
-  }
-  return S;
-}
-
 // Accumulator index of quad (s, h), component r for row hole RH (0 = i,
 // 1 = j) and lone hole H: element (i, j, k) at acc[16 i + 4 j + k].
 template <int H>
@@ -252,8 +250,28 @@ __device__ __forceinline__ constexpr int acc_at(int s, int h, int r) {
 // The MFMAs of one chunk.  xa/yb: this lane's X / Y quad offsets for s = 0 /
 // h = 0 (s adds 128 quads, h adds 256).
 template <int PREC, int H>
-__device__ __forceinline__ void chunk_mfma(float (&a)[64], const f4 *cb, int xa, int yb) {
-  if constexpr (PREC == TCE_T_FP32) {
+__device__ __forceinline__ void chunk_mfma(typename Prec<PREC>::acc (&a)[64], const f4 *cb, int xa,
+                                           int yb) {
+  if constexpr (PREC == TCE_T_FP64) {
+    const dv2 *cd = reinterpret_cast<const dv2 *>(cb);
+    dv2 xf[4], yf[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xf[s] = cd[xa + 128 * s];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) yf[h] = cd[X_QUADS + yb + 256 * h];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          d4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                  a[acc_at<H>(s, h, 3)]};
+          c = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s][t], yf[h][t], c, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+        }
+  } else if constexpr (PREC == TCE_T_FP32) {
     f4 xf[4], yf[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) xf[s] = cb[xa + 128 * s];
@@ -304,8 +322,13 @@ __device__ __forceinline__ void permute_lanes(float (&a)[64], int src) {
   for (int r = 0; r < 64; ++r)
     a[r] = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(a[r])));
 }
+__device__ __forceinline__ void permute_lanes(double (&a)[64], int src) {
+#pragma unroll
+  for (int r = 0; r < 64; ++r) a[r] = perm_lane(a[r], src);
+}
 
-__device__ __forceinline__ void negate(float (&a)[64]) {
+template <class T>
+__device__ __forceinline__ void negate(T (&a)[64]) {
 #pragma unroll
   for (int r = 0; r < 64; ++r) a[r] = -a[r];
 }
@@ -320,7 +343,8 @@ __device__ __forceinline__ constexpr int chunks_of(const MixedArgsG &M, int p) {
 // then runs its MFMAs from buffer j % 3, waits for this thread's loads of
 // chunk j+1 and synchronises.
 template <int PREC, int P>
-__device__ __forceinline__ void pass_step(float (&acc)[64], const f4 *lds, unsigned lds_base,
+__device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
+                                          unsigned lds_base,
                                           const MixedArgsG &M, const ItemGeo &G, int wave,
                                           int lane, int &j, int &buf, int total) {
   using Pr = Prec<PREC>;
@@ -336,23 +360,31 @@ __device__ __forceinline__ void pass_step(float (&acc)[64], const f4 *lds, unsig
   }
   if constexpr (pi.minus) negate(acc);
   const int nch0 = chunks_of<PREC>(M, P), nch1 = chunks_of<PREC>(M, P + 1);
-  // X fragment: row (lane & 15) = 4L' + r (L' = hi, r = lo), quad kq
-  const int slot = 4 * cube(wave, Lp) + hi;
-  const int xa = (slot * 4 + lo) * 4 + (kq ^ x_swz(slot, lo));
+  // X fragment: row (lane & 15) = 4L' + r (L' = hi, r = lo), quad kq.  The
+  // f64 MFMA writes row i to lane group i % 4, register i / 4 (the f32 and
+  // bf16 ones to group i / 4, register i % 4), so for FP64 the rows are
+  // ordered 4r + L' instead: either way the result lands at lane 16L' + col,
+  // component r.
+  const int xL = PREC == TCE_T_FP64 ? lo : hi, xr = PREC == TCE_T_FP64 ? hi : lo;
+  const int slot = 4 * cube(wave, Lp) + xL;
+  const int xa = (slot * 4 + xr) * 4 + (kq ^ x_swz(slot, xr));
   // Y fragment: col (lane & 15) = 4p1' + p2' (p1' = hi, p2' = lo)
   const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
   const int yb = (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s));
   const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
   const Src<PREC> s1 = pass_src<PREC, (P + 1 < 18 ? P + 1 : 17)>(M, G, wave, lane);
-  const Src<PREC> s2 = pass_src<PREC, (P + 2 < 18 ? P + 2 : 17)>(M, G, wave, lane);
   for (int c = 0; c < nch0; ++c, ++j) {
     const bool more = j + 2 < total;
     if (more) {
-      const int c2 = c + 2;
-      const bool in0 = c2 < nch0, in1 = c2 - nch0 < nch1;
-      const int cc = in0 ? c2 : (in1 ? c2 - nch0 : c2 - nch0 - nch1);
-      load_chunk<PREC>(select_src<PREC>(in0, s0, in1, s1, s2), (long)cc * Pr::KC * Pr::ES,
-                       lds_base, buf == 0 ? 2 : buf - 1, wave);
+      const int c2 = c + 2, nb = buf == 0 ? 2 : buf - 1;
+      if (c2 < nch0) {
+        load_chunk<PREC>(s0, (long)c2 * Pr::KC * Pr::ES, lds_base, nb, wave);
+      } else if (c2 - nch0 < nch1) {
+        load_chunk<PREC>(s1, (long)(c2 - nch0) * Pr::KC * Pr::ES, lds_base, nb, wave);
+      } else {  // pass P+1 has a single chunk: chunk 0 of pass P+2
+        load_chunk<PREC>(pass_src<PREC, (P + 2 < 18 ? P + 2 : 17)>(M, G, wave, lane),
+                         (long)(c2 - nch0 - nch1) * Pr::KC * Pr::ES, lds_base, nb, wave);
+      }
     }
     chunk_mfma<PREC, H>(acc, lds + buf * buf_quads<PREC>(), xa, yb);
     // own loads of chunk j+1 have landed (only chunk j+2's may be pending)
@@ -367,7 +399,8 @@ __device__ __forceinline__ void pass_step(float (&acc)[64], const f4 *lds, unsig
 }
 
 template <int PREC, int P>
-__device__ __forceinline__ void all_passes(float (&acc)[64], const f4 *lds, unsigned lds_base,
+__device__ __forceinline__ void all_passes(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
+                                           unsigned lds_base,
                                            const MixedArgsG &M, const ItemGeo &G, int wave,
                                            int lane, int &j, int &buf, int total) {
   pass_step<PREC, P>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
@@ -403,7 +436,8 @@ __device__ __forceinline__ void stage_epilogue(double *e, const FusedArgsG &A, i
 // from the FP32 accumulators); den = d * factor / (e_h1 + e_h2 + e_h3 -
 // e_p4 - e_p5 - e_p6), E[T] += den * d, E(T) += den * (d + s) with the 9
 // singles products (ccsd_t_dot.F:107 and the s1 table, as fused_wg.hip).
-__device__ __forceinline__ void energy(const float (&acc)[64], const double *e,
+template <class AccT>
+__device__ __forceinline__ void energy(const AccT (&acc)[64], const double *e,
                                        const FusedArgsG &A, const TaskDev &T, int sa, int sb,
                                        int sc, int la, int lb, int lc, int i0, int j0, int k0,
                                        const ItemGeo &G, double &e4, double &e5) {
@@ -473,7 +507,7 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
   using Pr = Prec<PREC>;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier is reached by all 8 waves).
-  for (long g = blockIdx.x; g < nglobal; g += gridDim.x) {
+  for (long g = first_item(blockIdx.x, gridDim.x); g < nglobal; g += gridDim.x) {
     const MixedSegment &sg = segs[find_segment(segs, nseg, g)];
     const MixedArgsG M = load_args(sg.args);
     const FusedArgsG &A = M.f;
@@ -496,9 +530,9 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     G.b = PRange{T.op5 + b0, min(7, T.r5 - 1 - b0)};
     G.c = PRange{T.op6 + c0, min(7, T.r6 - 1 - c0)};
 
-    float acc[64];  // element (i, j, k) of the lane at acc[16 i + 4 j + k]
+    typename Pr::acc acc[64];  // element (i, j, k) of the lane at acc[16 i + 4 j + k]
 #pragma unroll
-    for (int r = 0; r < 64; ++r) acc[r] = 0.0f;
+    for (int r = 0; r < 64; ++r) acc[r] = 0;
 
     {  // chunks 0 and 1 (pass 0, or passes 0 and 1 when pass 0 has one chunk)
       const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
@@ -539,7 +573,10 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 // the precision's chunk (16 FP32 / 32 bf16 values); the job layer pads them.
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        hipStream_t s) {
-  if (precision == TCE_T_FP32) {
+  if (precision == TCE_T_FP64) {
+    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_FP64>, dim3((unsigned)grid), dim3(512), 0, s, segs,
+                       nseg, total);
+  } else if (precision == TCE_T_FP32) {
     hipLaunchKernelGGL(fused_triples_mixed<TCE_T_FP32>, dim3((unsigned)grid), dim3(512), 0, s, segs,
                        nseg, total);
   } else if (precision == TCE_T_BF16X3) {

@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(512) fused_triples_wg(const Segment *segs, int
   const int ca = (wave >> 2) & 1, cb = (wave >> 1) & 1, cc = wave & 1;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier below is reached by all 8 waves).
-  for (long g = blockIdx.x; g < total; g += gridDim.x) {
+  for (long g = first_item(blockIdx.x, gridDim.x); g < total; g += gridDim.x) {
     const Segment &sg = segs[find_segment(segs, nseg, g)];
     const FusedArgsG A = load_args(sg.args);
     const long item = sg.item0 + (g - sg.g0);

@@ -28,6 +28,7 @@ struct Job {
   long Ep = 0, Mp = 0;
   int grid = 0;
   int pbox = 4;  // particle box edge of a work item: 4 (fused_wave) or 8 (fused_wg)
+  bool k16 = false;  // FP64 on the 16x16 MFMA kernel (fused_triples_mixed, FP64 policy)
   int ppi = 1;   // energy partials per work item (waves per item)
   // reduced-precision path (fused_mixed.hip): 8-wide particle boxes always
   std::vector<TaskDev> tasks8;

@@ -225,6 +225,7 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
     if (const char *k = getenv("TCE_T_KERNEL")) {
       if (!strcmp(k, "wave")) J->pbox = 4;
       if (!strcmp(k, "wg")) J->pbox = 8;
+      if (!strcmp(k, "wg16")) J->pbox = 8, J->k16 = true;
     }
     J->ppi = J->pbox == 8 ? 8 : 1;
   }
@@ -528,17 +529,17 @@ static int ensure_table(Job &J) {
   return TCE_T_OK;
 }
 
-// Copy `args` and one segment covering items [0, nitems) into the job's
+// Copy `args` and one segment covering items [item0, item1) into the job's
 // launch table (stream-ordered); returns the device segment pointer.
 template <class Args>
-static const SegmentT<Args> *stage_table(Job &J, const Args &args, long nitems) {
+static const SegmentT<Args> *stage_table(Job &J, const Args &args, long item1, long item0 = 0) {
   char *h = (char *)J.tab_host;
   char *d = (char *)J.tab_dev;
   std::memcpy(h, &args, sizeof(Args));
   SegmentT<Args> sg;
   sg.args = (const Args *)d;
-  sg.item0 = 0;
-  sg.item1 = nitems;
+  sg.item0 = item0;
+  sg.item1 = item1;
   sg.g0 = 0;
   const size_t off = (sizeof(Args) + 15) / 16 * 16;
   std::memcpy(h + off, &sg, sizeof sg);
@@ -634,9 +635,16 @@ FusedArgs tce_t::job_args(const Job &J, bool wide) {
 }
 
 MixedArgs tce_t::job_margs(const Job &J, int precision) {
-  const Job::Mixed &m = J.mx[precision];
   MixedArgs M;
   M.f = job_args(J, true);
+  if (precision == TCE_T_FP64) {  // the 16x16 kernel on the FP64 tensors
+    M.x2[0] = J.dT2e, M.y2[0] = J.dVd2, M.x1[0] = J.dWd1, M.y1[0] = J.dT2m;
+    M.x2[1] = M.y2[1] = M.x1[1] = M.y1[1] = nullptr;
+    M.Ek = (int)J.Ep;
+    M.Mk = (int)J.Mp;
+    return M;
+  }
+  const Job::Mixed &m = J.mx[precision];
   for (int p = 0; p < 2; ++p) {
     M.x2[p] = m.x2[p];
     M.y2[p] = m.y2[p];
@@ -675,9 +683,13 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
   }
-  const bool mixed = precision != TCE_T_FP64;
-  if (mixed) {
+  // the 16x16 kernel: reduced precision, or FP64 when the job asks for it
+  const bool mixed = precision != TCE_T_FP64 || J.k16;
+  if (precision != TCE_T_FP64) {
     const int rc = prepare_mixed(J, precision);
+    if (rc != TCE_T_OK) return rc;
+  } else if (J.k16) {
+    const int rc = ensure_tasks8(J);
     if (rc != TCE_T_OK) return rc;
   }
   const long ntasks = (long)J.tasks.size();
@@ -734,6 +746,96 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
   return TCE_T_OK;
 }
 
+// Restartable (T), ccsd_t_restart.F:58-262: the tasks in blocks of one outer
+// virtual tile p4b = noab + b, b = *begin .. nvab.  Each block is one
+// contiguous range of work items (p4b is the outermost task index); its
+// E(T) (sum of its tasks' E(T) in task order) goes into table[b - 1] and
+// *begin becomes b + 1, as the reference writes tce:restart_triples_table
+// and tce:ccsd_t_restart_begin to the rtdb.  At most max_tiles blocks per
+// call (<= 0: all).  When every block is done, *t_energy = sum of table.
+extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin, double *table,
+                                     long max_tiles, double *t_energy) {
+  if (!job || !begin || !table) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (!J.uploaded) {
+    set_error("tce_t_job_run_restart before tce_t_job_upload");
+    return TCE_T_ERR_STATE;
+  }
+  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+    set_error("unsupported precision mode");
+    return TCE_T_ERR_ARG;
+  }
+  const long noab = J.T.noab, nvab = J.T.nvab;
+  if (*begin < 1) *begin = 1;
+  const bool mixed = precision != TCE_T_FP64 || J.k16;
+  if (precision != TCE_T_FP64) {
+    const int rc = prepare_mixed(J, precision);
+    if (rc != TCE_T_OK) return rc;
+  } else if (J.k16) {
+    const int rc = ensure_tasks8(J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  {
+    const int rc = ensure_table(J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
+  const bool wide = mixed || J.pbox == 8;
+  const FusedArgs A = job_args(J, wide);
+  const std::vector<TaskDev> &tab = wide ? J.tasks8 : J.tasks;
+  const long ntasks = (long)J.tasks.size(), nitems = A.nitems;
+  J.taskE.resize(2 * ntasks);
+  long done = 0;
+  while (*begin <= nvab && (max_tiles <= 0 || done < max_tiles)) {
+    const long p4b = noab + *begin;
+    long t0 = 0;
+    while (t0 < ntasks && J.task_tiles[6 * t0] < p4b) ++t0;
+    long t1 = t0;
+    while (t1 < ntasks && J.task_tiles[6 * t1] == p4b) ++t1;
+    double e = 0.0;
+    if (t1 > t0) {
+      const long i0 = tab[t0].item0, i1 = t1 < ntasks ? tab[t1].item0 : nitems;
+      int lrc;
+      if (mixed) {
+        const MixedSegment *sg = stage_table(J, job_margs(J, precision), i1, i0);
+        if (!sg) return TCE_T_ERR_HIP;
+        lrc = launch_fused_mixed(sg, 1, i1 - i0, precision,
+                                 (int)std::min<long>(std::max(1, cus), i1 - i0), J.stream);
+      } else {
+        const Segment *sg = stage_table(J, A, i1, i0);
+        if (!sg) return TCE_T_ERR_HIP;
+        const int grid = (int)std::min<long>(J.grid, i1 - i0);
+        lrc = J.pbox == 8 ? launch_fused_wg(sg, 1, i1 - i0, grid, J.stream)
+                          : launch_fused_wave(sg, 1, i1 - i0, grid, J.stream);
+      }
+      if (lrc != 0) {
+        set_error("fused kernel launch failed");
+        return TCE_T_ERR_HIP;
+      }
+      if (launch_task_reduce(A.tasks + t0, (int)(t1 - t0), i1, wide ? 8 : J.ppi, A.partials,
+                             J.dtaskE + 2 * t0, J.stream) != 0) {
+        set_error("task_reduce launch failed");
+        return TCE_T_ERR_HIP;
+      }
+      HIPCHK(hipMemcpyAsync(J.taskE.data() + 2 * t0, J.dtaskE + 2 * t0, 2 * (t1 - t0) * sizeof(double),
+                            hipMemcpyDeviceToHost, J.stream),
+             "D2H");
+      HIPCHK(hipStreamSynchronize(J.stream), "restart block");
+      for (long t = t0; t < t1; ++t) e += J.taskE[2 * t + 1];
+    }
+    table[*begin - 1] = e;
+    ++*begin;
+    ++done;
+  }
+  if (*begin > nvab && t_energy) {
+    double s5 = 0.0;
+    for (long b = 0; b < nvab; ++b) s5 += table[b];
+    *t_energy = s5;
+  }
+  return TCE_T_OK;
+}
+
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
   if (!job || !out) return TCE_T_ERR_ARG;
   const Job &J = *reinterpret_cast<const Job *>(job);
@@ -748,7 +850,8 @@ extern "C" double tce_t_job_last_kernel_ms(const tce_t_job *job) {
 
 extern "C" int tce_t_job_kernel(const tce_t_job *job) {
   if (!job) return TCE_T_ERR_ARG;
-  return reinterpret_cast<const Job *>(job)->pbox == 8 ? TCE_T_KERNEL_WG : TCE_T_KERNEL_WAVE;
+  const Job &J = *reinterpret_cast<const Job *>(job);
+  return J.k16 ? TCE_T_KERNEL_WG16 : (J.pbox == 8 ? TCE_T_KERNEL_WG : TCE_T_KERNEL_WAVE);
 }
 
 void tce_t::job_release(Job &J) {

@@ -63,6 +63,23 @@ class TriplesJob:
         check(_lib.lib().tce_t_job_run(self._h, precision, C.byref(e1), C.byref(e2)), "tce_t_job_run")
         return e1.value, e2.value
 
+    def run_restart(self, state: dict | None = None, precision: int = FP64,
+                    max_tiles: int = 0) -> dict:
+        """ccsd_t_restart.F semantics: `state` = {'begin', 'table'} from a
+        previous call (or None to start); returns the updated state plus
+        't_energy' (E(T)) once every outer virtual tile is done."""
+        nvab = int(self.tiles.range.size - self.tiles.noab)
+        begin = C.c_long(1 if state is None else int(state["begin"]))
+        table = np.zeros(nvab) if state is None else np.array(state["table"], dtype=float)
+        te = C.c_double(0.0)
+        check(_lib.lib().tce_t_job_run_restart(self._h, precision, C.byref(begin),
+                                               table.ctypes.data_as(DP), max_tiles, C.byref(te)),
+              "tce_t_job_run_restart")
+        out = {"begin": begin.value, "table": table}
+        if begin.value > nvab:
+            out["t_energy"] = te.value
+        return out
+
     def task_energies(self) -> np.ndarray:
         out = np.zeros((self.ntasks, 2))
         check(_lib.lib().tce_t_job_task_energies(self._h, out.ctypes.data_as(DP), self.ntasks),
@@ -81,7 +98,7 @@ class TriplesJob:
     def kernel(self) -> str:
         """Name of the contraction kernel the job runs."""
         k = int(_lib.lib().tce_t_job_kernel(self._h))
-        return {0: "fused_triples", 1: "fused_triples_wg"}[k]
+        return {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed"}[k]
 
     @staticmethod
     def kernel_for(precision: int, fp64_kernel: str) -> str:
@@ -106,7 +123,8 @@ class TriplesJob:
             pass
 
 
-FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed"}
+FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed",
+            3: "fused_triples_mixed(fp64)"}
 
 
 class TriplesBatch:
@@ -132,7 +150,7 @@ class TriplesBatch:
 
     def plan(self, precision: int) -> dict:
         fam, co = C.c_int(), C.c_int()
-        tms = np.zeros(4)
+        tms = np.zeros(6)
         check(_lib.lib().tce_t_batch_plan(self._h, precision, C.byref(fam), C.byref(co),
                                           tms.ctypes.data_as(DP)), "tce_t_batch_plan")
         return {"kernel": FAMILIES[fam.value], "cost_order": bool(co.value),

@@ -53,11 +53,12 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernel", ["wave", "wg"])
+@pytest.mark.parametrize("kernel", ["wave", "wg", "wg16"])
 @pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
 def test_job_matches_oracle(case, kernel, monkeypatch):
-    """Both fused kernels: one wave per 4x4x4 particle cube (fused_wave) and
-    the LDS-cooperative 8-wave workgroup (fused_wg)."""
+    """Every FP64 kernel: one wave per 4x4x4 particle cube (fused_wave), the
+    LDS-cooperative 8-wave workgroup on 4x4x4 MFMA (fused_wg) and on 16x16x4
+    MFMA (fused_mixed's FP64 policy)."""
     monkeypatch.setenv("TCE_T_KERNEL", kernel)
     no, nv, to, tv, seed, ro, rv = case
     tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
@@ -299,3 +300,31 @@ def test_server_evicts_idle_systems():
     for (e1, e2), r in zip(ref, rs):
         assert abs(r["E_bt"] - e1) <= tol(e1)
         assert abs(r["E_pt"] - e2) <= tol(e2)
+
+
+# ---- restart (ccsd_t_restart.F): blocks of one outer virtual tile
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "bf16x3"])
+def test_restart_blocks_match_task_sums(precision):
+    """Running the job in restartable blocks of outer virtual tiles, two per
+    call with the state passed along, fills the table with each tile's
+    E(T) (its tasks' oracle energies summed) and ends on the full E(T)."""
+    tm, b, ot = system(9, 41, 0, 0, 41, [6, 3], [16, 16, 9])
+    e1, e2, te = oracle_energies(ot, b)
+    tasks = tm.tasks()
+    nvab = tm.range.size - tm.noab
+    per_tile = np.zeros(nvab)
+    for (p4, p5, p6, h1, h2, h3), (t4, t5) in zip(tasks, te):
+        per_tile[p4 - tm.noab - 1] += t5
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        st, calls = None, 0
+        while st is None or "t_energy" not in st:
+            st = job.run_restart(st, precision, max_tiles=2)
+            calls += 1
+        full = job.run(precision)
+    assert calls == (nvab + 1) // 2
+    rel = 1e-12 if precision == 0 else MIXED_REL[precision]
+    scale = np.abs(per_tile).max()
+    assert np.all(np.abs(st["table"] - per_tile) <= rel * scale + 1e-10), (st["table"], per_tile)
+    assert abs(st["t_energy"] - e2) <= max(tol(e2), rel * abs(e2))
+    assert abs(st["t_energy"] - full[1]) <= rel * abs(full[1]) + 1e-12
