@@ -334,7 +334,7 @@ def main() -> int:
             "E_bt": m_bt, "E_pt": m_pt,
             "dE_bt_uEh": d_bt, "dE_pt_uEh": d_pt,
             "dE_rel": max(abs(m_bt - e_bt) / abs(e_bt), abs(m_pt - e_pt) / abs(e_pt)) if e_bt and e_pt else None,
-            "kernel": "fused_triples_mixed",
+            "kernel": f"fused_triples_mixed_{name}",
             "extra_device_bytes": job.precision_bytes(prec),
         }
     job.close()

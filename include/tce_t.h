@@ -183,13 +183,14 @@ int tce_t_pack_dense(const tce_t_tiles *tiles, const double *t1, const long *k_t
 
 /* Device time (ms) of the last run's contraction kernel(s), HIP events. */
 double tce_t_job_last_kernel_ms(const tce_t_job *job);
-/* Which contraction kernel the job runs: TCE_T_KERNEL_WAVE (one wavefront per
- * 4x4x4 particle cube, operands from L1/L2) or TCE_T_KERNEL_WG (8-wavefront
- * workgroup per 8x8x8 particle box, operands staged in LDS); chosen at
- * tce_t_job_create from the tile shapes.  Negative on a NULL job. */
+/* Which FP64 contraction kernel the job runs: TCE_T_KERNEL_WAVE (one
+ * wavefront per 4x4x4 particle cube, operands from L1/L2), TCE_T_KERNEL_WG
+ * (8-wavefront workgroup per 8x8x8 particle box, operands staged in LDS,
+ * 4x4x4 MFMA) or TCE_T_KERNEL_WG16 (the same boxes on 16x16x4 MFMA); chosen
+ * at tce_t_job_create from the tile shapes.  Negative on a NULL job. */
 #define TCE_T_KERNEL_WAVE 0
 #define TCE_T_KERNEL_WG 1
-#define TCE_T_KERNEL_WG16 2 /* 16x16 FP64 MFMA (fused_triples_mixed); TCE_T_KERNEL=wg16 */
+#define TCE_T_KERNEL_WG16 2
 int tce_t_job_kernel(const tce_t_job *job);
 int tce_t_job_destroy(tce_t_job *job);
 

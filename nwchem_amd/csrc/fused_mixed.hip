@@ -1,7 +1,11 @@
-// fused_mixed.hip -- the fused (T) kernel with the d1/d2 contractions in
-// reduced precision on 16x16 MFMA tiles, the denominators, their division,
-// the singles term and every energy sum in FP64.
+// fused_mixed.hip -- the fused (T) kernel on 16x16 MFMA tiles: the d1/d2
+// contractions in FP64 or in reduced precision; the denominators, their
+// division, the singles term and every energy sum in FP64.
 //
+//   TCE_T_FP64    v_mfma_f64_16x16x4_f64, FP64 operands and accumulation
+//                 (the default FP64 kernel of 8-wide tilings); its D layout
+//                 puts row i in lane group i%4, register i/4, so its X rows
+//                 are ordered 4r + L' (the roles of the two fields swapped)
 //   TCE_T_FP32    v_mfma_f32_16x16x4_f32, FP32 operands and accumulation
 //   TCE_T_BF16X3  v_mfma_f32_16x16x32_bf16, each operand x split into
 //                 hi = bf16(x), lo = bf16(x - hi); per product hi*hi + hi*lo
@@ -371,8 +375,10 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
   // Y fragment: col (lane & 15) = 4p1' + p2' (p1' = hi, p2' = lo)
   const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
   const int yb = (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s));
+  // load sources of this pass; those of the next two passes are formed only
+  // in the branches that need them (the last two chunks), which keeps the
+  // chunk loop's live registers to the accumulators, fragments and s0
   const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
-  const Src<PREC> s1 = pass_src<PREC, (P + 1 < 18 ? P + 1 : 17)>(M, G, wave, lane);
   for (int c = 0; c < nch0; ++c, ++j) {
     const bool more = j + 2 < total;
     if (more) {
@@ -380,7 +386,8 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       if (c2 < nch0) {
         load_chunk<PREC>(s0, (long)c2 * Pr::KC * Pr::ES, lds_base, nb, wave);
       } else if (c2 - nch0 < nch1) {
-        load_chunk<PREC>(s1, (long)(c2 - nch0) * Pr::KC * Pr::ES, lds_base, nb, wave);
+        load_chunk<PREC>(pass_src<PREC, (P + 1 < 18 ? P + 1 : 17)>(M, G, wave, lane),
+                         (long)(c2 - nch0) * Pr::KC * Pr::ES, lds_base, nb, wave);
       } else {  // pass P+1 has a single chunk: chunk 0 of pass P+2
         load_chunk<PREC>(pass_src<PREC, (P + 2 < 18 ? P + 2 : 17)>(M, G, wave, lane),
                          (long)(c2 - nch0 - nch1) * Pr::KC * Pr::ES, lds_base, nb, wave);

@@ -213,7 +213,9 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
   const long noab = T.noab, nvab = T.nvab;
   // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
   // shared through LDS) unless rounding the particle tiles up to multiples
-  // of 8 pads the work by more than 25% over 4-wide boxes.
+  // of 8 pads the work by more than 25% over 4-wide boxes.  TCE_T_KERNEL =
+  // wave | wg | wg16 forces fused_triples, fused_triples_wg or the FP64
+  // policy of fused_triples_mixed.
   {
     double w4 = 0, w8 = 0;
     for (long p = noab + 1; p <= noab + nvab; ++p) {
@@ -222,9 +224,12 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
       w8 += std::ceil(r / 8) * 8 / r;
     }
     J->pbox = (w8 <= 1.25 * w4) ? 8 : 4;
+    // 8-wide boxes run on the 16x16x4 FP64 kernel (measured 3% faster than
+    // the 4x4x4 one at no=40/nv=320)
+    J->k16 = J->pbox == 8;
     if (const char *k = getenv("TCE_T_KERNEL")) {
-      if (!strcmp(k, "wave")) J->pbox = 4;
-      if (!strcmp(k, "wg")) J->pbox = 8;
+      if (!strcmp(k, "wave")) J->pbox = 4, J->k16 = false;
+      if (!strcmp(k, "wg")) J->pbox = 8, J->k16 = false;
       if (!strcmp(k, "wg16")) J->pbox = 8, J->k16 = true;
     }
     J->ppi = J->pbox == 8 ? 8 : 1;

@@ -98,12 +98,13 @@ class TriplesJob:
     def kernel(self) -> str:
         """Name of the contraction kernel the job runs."""
         k = int(_lib.lib().tce_t_job_kernel(self._h))
-        return {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed"}[k]
+        return {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed_fp64"}[k]
 
     @staticmethod
     def kernel_for(precision: int, fp64_kernel: str) -> str:
         """Kernel that runs the contractions in `precision`."""
-        return fp64_kernel if precision == FP64 else "fused_triples_mixed"
+        return fp64_kernel if precision == FP64 else \
+            {FP32: "fused_triples_mixed_fp32", BF16X3: "fused_triples_mixed_bf16x3"}[precision]
 
     def close(self) -> None:
         if self._h:
@@ -124,7 +125,7 @@ class TriplesJob:
 
 
 FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed",
-            3: "fused_triples_mixed(fp64)"}
+            3: "fused_triples_mixed_fp64"}
 
 
 class TriplesBatch:

@@ -1,0 +1,180 @@
+"""Spin- and symmetry-labelled test systems, built from dense spin-orbital
+tensors so that a textbook (T) over the dense tensors pins the tiled result.
+
+NWChem tiles spin-orbitals by spin and irrep (tce_tile.F): alpha holes,
+beta holes, alpha particles, beta particles, each space split into tiles
+that carry one spin (k_spin: 1 alpha, 2 beta) and one irrep (k_sym, D2h
+subgroup labels combined by XOR).  Blocks that break spin or symmetry
+conservation are not stored (tce_t1_offset.F, tce_t2_offset.F,
+tce_mo2e_offset.F), and tasks that break it are skipped (ccsd_t_gpu.F:94-112).
+In a restricted run (closed-shell reference) all-beta blocks are read from
+their alpha images (tce_restricted.F), tasks with more beta than alpha
+spin-orbitals are skipped and the rest count twice (ccsd_t_gpu.F:153-157).
+
+Dense tensors here are random, antisymmetric, and zero wherever spin or
+irrep is not conserved, so the spin-adapted tiled sum must equal the dense
+sum over every orbital sextuple (the skipped blocks and tasks contribute
+exactly zero).  With spin-flip symmetric tensors (alpha <-> beta images
+equal) the restricted run must equal the unrestricted one.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from nwchem_amd import tiles as tl
+
+
+def _spin_orbitals(sizes_o, sizes_v, irreps_o, irreps_v):
+    """Tile list (alpha then beta in each space): ranges, spins, irreps,
+    alpha images (1-based) and per-orbital spin/irrep/spatial labels."""
+    ro = list(sizes_o) * 2
+    rv = list(sizes_v) * 2
+    spin = [1] * len(sizes_o) + [2] * len(sizes_o) + [1] * len(sizes_v) + [2] * len(sizes_v)
+    sym = list(irreps_o) * 2 + list(irreps_v) * 2
+    no_t, nv_t = len(sizes_o), len(sizes_v)
+    alpha = (list(range(1, no_t + 1)) * 2
+             + list(range(2 * no_t + 1, 2 * no_t + nv_t + 1)) * 2)
+    return np.array(ro + rv, np.int64), 2 * no_t, np.array(spin), np.array(sym), np.array(alpha)
+
+
+def _orbital_labels(ranges, spin, sym):
+    s, y = [], []
+    for r, sp, sy in zip(ranges, spin, sym):
+        s += [sp] * int(r)
+        y += [sy] * int(r)
+    return np.array(s), np.array(y)
+
+
+def _antisym(x, pairs):
+    for a, b in pairs:
+        x = x - np.swapaxes(x, a, b)
+    return x
+
+
+class SpinSystem:
+    """One spin-orbital (T) system: tiles, dense tensors, TCE blocks."""
+
+    def __init__(self, sizes_o, sizes_v, irreps_o=None, irreps_v=None, seed=0,
+                 flip_symmetric=False, restricted=False):
+        irreps_o = [0] * len(sizes_o) if irreps_o is None else list(irreps_o)
+        irreps_v = [0] * len(sizes_v) if irreps_v is None else list(irreps_v)
+        ranges, noab, spin, sym, alpha = _spin_orbitals(sizes_o, sizes_v, irreps_o, irreps_v)
+        self.no = int(ranges[:noab].sum())
+        self.nv = int(ranges[noab:].sum())
+        no, nv = self.no, self.nv
+        so, yo = _orbital_labels(ranges[:noab], spin[:noab], sym[:noab])
+        sv, yv = _orbital_labels(ranges[noab:], spin[noab:], sym[noab:])
+        rng = np.random.default_rng(seed)
+        no_s, nv_s = no // 2, nv // 2
+
+        # orbital energies: beta copies of the alpha ones
+        eo_a = np.sort(rng.uniform(-2.0, -0.3, no_s))
+        ev_a = np.sort(rng.uniform(0.2, 4.0, nv_s))
+        self.eo = np.concatenate([eo_a, eo_a])
+        self.ev = np.concatenate([ev_a, ev_a])
+
+        def flip(x, spaces):
+            """Image of x under alpha <-> beta on every index."""
+            for ax, sp in enumerate(spaces):
+                n = no_s if sp == "o" else nv_s
+                x = np.roll(x, n, axis=ax)
+            return x
+
+        def mask(spaces, signs):
+            """spin and irrep conservation over the indices (sign +1: ket
+            side, -1: bra side)."""
+            lab_s = [so if sp == "o" else sv for sp in spaces]
+            lab_y = [yo if sp == "o" else yv for sp in spaces]
+            grids_s = np.ix_(*lab_s)
+            grids_y = np.ix_(*lab_y)
+            tot = sum(sg * g for sg, g in zip(signs, grids_s))
+            xr = grids_y[0]
+            for g in grids_y[1:]:
+                xr = xr ^ g
+            return (tot == 0) & (xr == 0)
+
+        def make(shape_spaces, signs, pairs, sigma):
+            shape = tuple(no if sp == "o" else nv for sp in shape_spaces)
+            x = sigma * rng.standard_normal(shape)
+            x = _antisym(x, pairs)
+            if flip_symmetric:
+                x = 0.5 * (x + flip(x, shape_spaces))
+            return np.where(mask(shape_spaces, signs), x, 0.0)
+
+        # T1[p,h], T2[p,p',h,h'], W[m,p,h,h'] = <m p||h h'>,
+        # V[p,p',h,e] = <p p'||h e>, S[p,p',h,h'] = <p p'||h h'>
+        self.T1 = make("vo", (1, -1), [], 0.01)
+        self.T2 = make("vvoo", (1, 1, -1, -1), [(0, 1), (2, 3)], 0.02)
+        self.W = make("ovoo", (1, 1, -1, -1), [(2, 3)], 0.05)
+        self.V = make("vvov", (1, 1, -1, -1), [(0, 1)], 0.05)
+        self.S = make("vvoo", (1, 1, -1, -1), [(0, 1), (2, 3)], 0.05)
+
+        evl = np.concatenate([self.eo, self.ev])
+        self.tiles = tl.TileModel(range=ranges, noab=noab, evl=evl, spin=spin, sym=sym,
+                                  alpha=alpha, restricted=restricted)
+        self.blocks = self._fill_blocks()
+
+    def _fill_blocks(self) -> tl.Blocks:
+        """Canonical TCE blocks (C order over the block's index list) from
+        the dense tensors, at the offsets of the hash tables the library
+        builds (same key formulas as tce_t1/t2/mo2e_offset)."""
+        T = self.tiles
+        noab, nvab = T.noab, T.nvab
+        N = noab + nvab
+        rng_, ofs = T.range, T.offset
+        no = self.no
+
+        def sl(b):  # orbital slice of tile b (1-based) within its space
+            o = ofs[b - 1] - (no if b > noab else 0)
+            return slice(int(o), int(o + rng_[b - 1]))
+
+        def table(h):
+            L = int(h[0])
+            return [(int(h[1 + i]), int(h[1 + L + i])) for i in range(L)]
+
+        t1h, t2h, v2h = T.t1_hash(), T.t2_hash(), T.v2_hash()
+        s = T.sizes()
+        t1 = np.zeros(max(s["t1"], 1))
+        t2 = np.zeros(max(s["t2"], 1))
+        v2 = np.zeros(max(s["v2"], 1))
+        for key, off in table(t1h):
+            h6, p5 = key % noab + 1, key // noab + noab + 1
+            blk = self.T1[sl(p5), sl(h6)].ravel()
+            t1[off:off + blk.size] = blk
+        for key, off in table(t2h):
+            h4 = key % noab + 1
+            r = key // noab
+            h3 = r % noab + 1
+            r //= noab
+            p2 = r % nvab + noab + 1
+            p1 = r // nvab + noab + 1
+            blk = self.T2[sl(p1), sl(p2), sl(h3), sl(h4)].ravel()
+            t2[off:off + blk.size] = blk
+        for key, off in table(v2h):
+            g2 = key % N + 1
+            r = key // N
+            g1 = r % N + 1
+            r //= N
+            g4 = r % N + 1
+            g3 = r // N + 1
+            if g3 <= noab:
+                D = self.W
+            elif g2 > noab:
+                D = self.V
+            else:
+                D = self.S
+            blk = D[sl(g3), sl(g4), sl(g1), sl(g2)].ravel()
+            v2[off:off + blk.size] = blk
+        return tl.Blocks(t1, t1h, t2, t2h, v2, v2h)
+
+    def oracle_tiles(self):
+        import oracle
+
+        T = self.tiles
+        return oracle.Tiles(T.range, T.noab, T.evl, T.spin, T.sym, T.alpha, T.restricted)
+
+    def brute_force(self):
+        import oracle
+
+        return oracle.brute_force(self.no, self.nv, self.T1, self.T2, self.W, self.V, self.S,
+                                  self.eo, self.ev)

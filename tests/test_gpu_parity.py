@@ -6,12 +6,21 @@ floor only matters when |E| > 100 Eh (the random amplitudes of the synthetic
 inputs give |E| of order 10 Eh at no=8/nv=32 and far more at larger sizes,
 where FP64 summation of the reference itself cannot resolve 1e-10 Eh).
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle
 from nwchem_amd import TriplesJob, tiles as tl
 from nwchem_amd import _lib
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from golden import make_golden as mg  # noqa: E402
+import spin_systems  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -153,6 +162,81 @@ def test_mixed_precision_matches_oracle(case, precision):
     assert abs(e[1] - e2) <= rel * abs(e2) + 1e-12
     scale = np.abs(te).max()
     assert np.all(np.abs(gte - te) <= rel * scale + 1e-12)
+
+
+# ---- golden vectors (reference kernels, tests/golden) and spin/irrep/restricted tilings
+
+GOLDEN_SYSTEMS = list(mg.SYSTEMS)
+
+
+def golden(name):
+    with np.load(os.path.join(GOLD, "energies.npz")) as z:
+        tot, te = z[name + "_total"], z[name + "_tasks"]
+    return float(tot[0]), float(tot[1]), te
+
+
+@pytest.mark.parametrize("name", GOLDEN_SYSTEMS)
+def test_job_matches_golden(name):
+    """The default kernel choice against the energies the reference's own
+    kernels give on the same inputs (spin-labelled and restricted included)."""
+    kind, prm = mg.SYSTEMS[name]
+    ot, b, tm = mg.build_system(kind, prm)
+    print(f"golden {name}: ranges={list(tm.range)}", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        e = job.run()
+        te = job.task_energies()
+        print(f"  kernel {job.kernel}", flush=True)
+    assert_parity(e, te, golden(name))
+
+
+SPIN_GPU = [
+    dict(sizes_o=[3, 2], sizes_v=[4, 3], irreps_o=[0, 1], irreps_v=[1, 0], seed=5),
+    dict(sizes_o=[8], sizes_v=[16, 8], irreps_o=[0], irreps_v=[0, 1], seed=9),
+    dict(sizes_o=[4, 4], sizes_v=[8, 8], seed=10, flip_symmetric=True, restricted=True),
+]
+
+
+@pytest.mark.parametrize("kernel", ["wave", "wg", "wg16"])
+@pytest.mark.parametrize("prm", SPIN_GPU, ids=["spin_irreps", "spin_8_16", "restricted"])
+def test_spin_tilings_match_oracle(prm, kernel, monkeypatch):
+    """Tiles labelled by spin and irrep (blocks and tasks that break
+    conservation absent), and restricted runs, on every FP64 kernel."""
+    monkeypatch.setenv("TCE_T_KERNEL", kernel)
+    s = spin_systems.SpinSystem(**prm)
+    b = s.blocks
+    ref = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
+    print(f"spin case {prm} kernel={kernel} tasks={len(ref[2])}", flush=True)
+    with TriplesJob(s.tiles) as job:
+        job.upload(b)
+        e = job.run()
+        te = job.task_energies()
+    assert_parity(e, te, ref)
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
+def test_spin_tilings_reduced_precision(precision):
+    s = spin_systems.SpinSystem(**SPIN_GPU[1])
+    b = s.blocks
+    e1, e2, _ = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
+    rel = MIXED_REL[precision]
+    with TriplesJob(s.tiles) as job:
+        job.upload(b)
+        e = job.run(precision)
+    assert abs(e[0] - e1) <= rel * abs(e1) + 1e-12
+    assert abs(e[1] - e2) <= rel * abs(e2) + 1e-12
+
+
+def test_legacy_surface_spin_tiling():
+    """The per-call surface under the same spin/irrep task loop."""
+    s = spin_systems.SpinSystem(**SPIN_GPU[0])
+    b = s.blocks
+    ref = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
+    L = _lib.lib()
+    got = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                        backend=oracle.device_backend(L))
+    assert L.tce_t_legacy_status() == 0
+    assert_parity(got[:2], got[2], ref)
 
 
 def test_mixed_precision_deterministic_and_fp64_unchanged():
