@@ -9,13 +9,19 @@ stay resident in HBM.
 Default workload: no=40 occupied (tiles of 20), nv=320 virtual (tiles of 40),
 480 tasks, FP64, seeded synthetic inputs.
 
-Output: one JSON line.  `value` is algorithmic TFLOP/s of the FP64 job,
-F_ref = sum_tasks |t3_task| * 18 * (no + nv + 1) per job (the contraction
-flops the reference's tile dispatch performs; energy excluded).  The same
-line reports the reduced-precision modes (`--modes`): the d1/d2 contractions
-in FP32 and in 3-term split-BF16, each timed the same way on the same
-resident inputs, with energies/s, the speedup over FP64 and |dE| in micro-Eh
-against the FP64 result of the run.
+Output: one JSON line.  `value` is algorithmic TFLOP/s of the FP64 job:
+F = sum over the kernel tasks run of |t3 tile| * 18 * (no + nv + 1) (the
+reference's per-task contraction count, ccsd_t_kernels_omp.F; energy
+excluded) over the wall time of the job.  The kernel tasks are the box-sized
+sub-tiles of the reference tiles (the library's default), which compute the
+same energies without the symmetric duplicates a reference task computes
+inside equal tiles, so F is about a third of the reference tiling's count
+F_ref at this workload; both are reported, and `reference_tiling` times the
+same job with the reference's tiles as kernel tasks (F_ref of work).  The
+same line reports the reduced-precision modes (`--modes`): the d1/d2
+contractions in FP32 and in 3-term split-BF16, each timed the same way on the
+same resident inputs, with energies/s, the speedup over FP64 and |dE| in
+micro-Eh against the FP64 result of the run.
 """
 from __future__ import annotations
 
@@ -32,6 +38,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "(T) TFLOP/s & energies/s, FP64 vs split-BF16 at ≤1 µEh; serving p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
+
+
+def job_work(tm, precision: int = 0) -> float:
+    """Flops a run of this job performs (the kernel tasks the library
+    builds for it, tce_t_job_work); no upload needed."""
+    from nwchem_amd import TriplesJob
+
+    with TriplesJob(tm) as j:
+        return j.work(precision)[0]
 
 
 def f_ref(tm) -> float:
@@ -138,6 +153,7 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
 
     rng = np.random.default_rng(seed)
     jobs, F = [], 0.0
+    Fm = {p: 0.0 for p in modes}
     t0 = time.perf_counter()
     for k in range(njobs):
         no, nv = int(rng.integers(6, 31)), int(rng.integers(48, 257))
@@ -149,10 +165,12 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
         del b
         jobs.append(j)
         F += f_ref(tm)
+        for p in modes:
+            Fm[p] += j.work(p)[0]
         print(f"[bench] batch job {k}: no={no} nv={nv} tiles {to}/{tv} ({len(tm.tasks())} tasks) "
               f"ready at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
-    out = {"jobs": njobs, "flops_per_run": F, "prep_s": t_prep}
+    out = {"jobs": njobs, "flops_reference_tiling": F, "prep_s": t_prep}
     names = {0: "fp64", 1: "fp32", 2: "bf16x3"}
     with TriplesBatch(jobs) as batch:
         for p in modes:
@@ -180,11 +198,12 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
                     print(f"[bench] batch {names[p]}: one by one, {k + 1} jobs in {seq:.1f} s",
                           file=sys.stderr, flush=True)
             out[names[p]] = {"ms_per_run": wall * 1e3, "graph_ms": float(np.mean(ms)),
-                             "energies_per_s": njobs / wall, "fp64_equiv_tflops": F / wall / 1e12,
+                             "energies_per_s": njobs / wall, "flops_per_run": Fm[p],
+                             "fp64_equiv_tflops": Fm[p] / wall / 1e12,
                              "sequential_ms": seq * 1e3, "speedup_vs_sequential": seq / wall,
                              "plan": batch.plan(p), "plan_s": t_plan}
             print(f"[bench] batch {names[p]}: {wall*1e3:.1f} ms per run of {njobs} jobs "
-                  f"({njobs/wall:.1f} energies/s, {F/wall/1e12:.1f} TF/s), one by one {seq*1e3:.1f} ms; "
+                  f"({njobs/wall:.1f} energies/s, {Fm[p]/wall/1e12:.1f} TF/s), one by one {seq*1e3:.1f} ms; "
                   f"plan {batch.plan(p)}", file=sys.stderr, flush=True)
     for j in jobs:
         j.close()
@@ -205,7 +224,7 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -
         no = int(round(np.exp(rng.uniform(np.log(6), np.log(40)))))
         nv = int(round(np.exp(rng.uniform(np.log(48), np.log(320)))))
         tm = tl.TileModel.synthetic(no, nv, min(no, 20), min(nv, 40), seed + 100 + k)
-        pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), f_ref(tm)))
+        pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), {p: job_work(tm, p) for p in (0, 2)}))
         print(f"[bench] serve system {k}: no={no} nv={nv} ready at {time.perf_counter() - t0:.1f} s",
               file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
@@ -237,7 +256,7 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -
         st = srv.stats()
     for i, r in enumerate(res):
         lat.append(r["finish"] - r["submit"])
-        F += pool[picks[i]][2]
+        F += pool[picks[i]][2][int(precs[i])]
     lat = np.array(lat)
     out = {"requests": nreq, "lambda_per_s": lam, "pool_systems": npool,
            "precision_mix": {"fp64": int((precs == 0).sum()), "bf16x3": int((precs == 2).sum())},
@@ -269,6 +288,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--slice-ms", type=float, default=50.0)
+    ap.add_argument("--reference-tiling", type=int, default=1,
+                    help="also time the job with the reference tiles as kernel tasks (0: skip)")
     ap.add_argument("--modes", default="fp32,bf16x3",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
@@ -309,6 +330,9 @@ def main() -> int:
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     kernel = job.kernel
+    F_run, items = job.work(0)
+    print(f"[bench] kernel tasks: {items} items, F = {F_run:.4e} flops "
+          f"({F / F_run:.2f}x less than the reference tiling's F_ref)", file=sys.stderr, flush=True)
     modes = {}
     for name in [m for m in args.modes.split(",") if m]:
         prec = {"fp32": 1, "bf16x3": 2}[name]
@@ -330,7 +354,7 @@ def main() -> int:
             "kernel_ms": float(np.mean(mk)),
             "energies_per_s": args.steps / mwall,
             "speedup_vs_fp64": wall / mwall,
-            "fp64_equiv_tflops": F * args.steps / mwall / 1e12,
+            "fp64_equiv_tflops": job.work(prec)[0] * args.steps / mwall / 1e12,
             "E_bt": m_bt, "E_pt": m_pt,
             "dE_bt_uEh": d_bt, "dE_pt_uEh": d_pt,
             "dE_rel": max(abs(m_bt - e_bt) / abs(e_bt), abs(m_pt - e_pt) / abs(e_pt)) if e_bt and e_pt else None,
@@ -338,14 +362,39 @@ def main() -> int:
             "extra_device_bytes": job.precision_bytes(prec),
         }
     job.close()
+    reftile = None
+    if args.reference_tiling:
+        # the same job with the reference's tiles as kernel tasks: the work the
+        # reference's own task loop does, timed like the steps above
+        os.environ["TCE_T_SUBTILE"] = "0"
+        try:
+            rj = TriplesJob(tm)
+        finally:
+            del os.environ["TCE_T_SUBTILE"]
+        with rj:
+            rj.upload(blocks)
+            t1 = time.perf_counter()
+            r_bt, r_pt = rj.run()
+            rwall = time.perf_counter() - t1
+            rk = rj.last_kernel_ms
+            rF, ritems = rj.work(0)
+        reftile = {"ms_per_step": rwall * 1e3, "kernel_ms": rk, "items": ritems, "flops": rF,
+                   "tflops": rF / rwall / 1e12, "kernel_tflops": rF / (rk * 1e-3) / 1e12,
+                   "frac_of_peak": rF / (rk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                   "energies_per_s": 1.0 / rwall, "E_bt": r_bt, "E_pt": r_pt,
+                   "dE_vs_subtiled_rel": max(abs(r_bt - e_bt) / abs(e_bt), abs(r_pt - e_pt) / abs(e_pt))}
+        print(f"[bench] reference tiling: kernel {rk:.1f} ms, {ritems} items, "
+              f"{reftile['kernel_tflops']:.2f} TF/s ({100 * reftile['frac_of_peak']:.1f}% of peak), "
+              f"E[T]={r_bt:.12e} (rel. diff {reftile['dE_vs_subtiled_rel']:.2e})", file=sys.stderr, flush=True)
 
     ms_step = wall / args.steps * 1e3
-    value = F * args.steps / wall / 1e12
+    value = F_run * args.steps / wall / 1e12
     k_avg = float(np.mean(kms))
-    achieved = F / (k_avg * 1e-3) / 1e12
+    achieved = F_run / (k_avg * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload, kernel),
-                "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F}
+                "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F_run,
+                "items_per_launch": items}
     batch = bench_batch(args.batch_jobs, args.seed, max(1, args.steps)) if args.batch_jobs else None
     serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms) \
         if args.serve else None
@@ -377,7 +426,10 @@ def main() -> int:
         "energies_per_s": args.steps / wall,
         "E_bt": e_bt,
         "E_pt": e_pt,
+        "flops_per_job": F_run,
+        "flops_reference_tiling": F,
         "roofline": roofline,
+        "reference_tiling": reftile,
         "cpu_baseline": cpu,
         "precision_modes": modes or None,
         "batch": batch,

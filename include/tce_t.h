@@ -192,6 +192,17 @@ double tce_t_job_last_kernel_ms(const tce_t_job *job);
 #define TCE_T_KERNEL_WG 1
 #define TCE_T_KERNEL_WG16 2
 int tce_t_job_kernel(const tce_t_job *job);
+/* Work a run in `precision` performs: *flops = sum over the kernel tasks of
+ * |t3 tile| * 18 * (no + nv + 1) (the reference's per-task count of
+ * ccsd_t_kernels_omp.F: 9 d1 calls x 2 no, 9 d2 x 2 nv, 9 s1 x 2) and
+ * *items = work items (either may be NULL).  By default the kernel tasks are
+ * the box-sized sub-tiles of the reference tiles (4 occupied, 4 or 8
+ * virtual orbitals) in canonical order, which leaves out the symmetric
+ * duplicates a reference task computes inside equal tiles: the energies
+ * are the same sums (per reference task, up to FP64 rounding) at less work.
+ * TCE_T_SUBTILE=0 in the environment at tce_t_job_create keeps the
+ * reference tiles.  No reference counterpart. */
+int tce_t_job_work(const tce_t_job *job, int precision, double *flops, long *items);
 int tce_t_job_destroy(tce_t_job *job);
 
 /* ---- batches: many uploaded jobs in one persistent launch -------------- */

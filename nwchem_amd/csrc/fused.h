@@ -269,8 +269,11 @@ __device__ __forceinline__ void wave_sum(double &e4, double &e5) {
   }
 }
 
-// Locate the task of a work item (tasks sorted by item0) -- uniform.
+// Locate the task of a work item (tasks sorted by item0, item0[t] >= t) --
+// uniform.  Sub-tiled tables have one item per task (item0[t] == t): one
+// probe finds it.
 __device__ __forceinline__ int find_task(gptr<const TaskDev> tasks, int ntasks, long item) {
+  if (item < ntasks && tasks[item].item0 == item) return (int)item;
   int lo = 0, hi = ntasks - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;

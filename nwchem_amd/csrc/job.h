@@ -11,11 +11,21 @@
 
 namespace tce_t {
 
+// Kernel task tables.  A reference task (tile sextuple of the reference task
+// loop) runs as one kernel task with boxes over its tiles, or -- sub-tiled,
+// the default -- as the kernel tasks of the box-sized sub-tiles of its tiles
+// (canonical sub-tile order, factors by sub-tile equality), which skips the
+// symmetric duplicates and the padding inside equal tiles.  Either way a
+// reference task owns the contiguous items [bound[t], bound[t+1]), so its
+// energy is a fixed-order sum of its own items.
 struct Job {
   Tiles T;
   std::vector<TaskDev> tasks;
-  std::vector<long> task_tiles;  // 6 per task: p4b,p5b,p6b,h1b,h2b,h3b
+  std::vector<long> bound;       // item boundaries of the reference tasks in `tasks`
+  std::vector<long> task_tiles;  // 6 per reference task: p4b,p5b,p6b,h1b,h2b,h3b
   long nitems = 0;
+  bool subtile = true;
+  long ntask() const { return (long)task_tiles.size() / 6; }
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -32,9 +42,11 @@ struct Job {
   int ppi = 1;   // energy partials per work item (waves per item)
   // reduced-precision path (fused_mixed.hip): 8-wide particle boxes always
   std::vector<TaskDev> tasks8;
+  std::vector<long> bound8;
   long nitems8 = 0;
   TaskDev *dtasks8 = nullptr;  // == dtasks when pbox == 8
   double *dpart8 = nullptr;    // == dpart when pbox == 8
+  long *dbound = nullptr, *dbound8 = nullptr;  // dbound8 == dbound when pbox == 8
   // launch tables (job arguments + segments): pinned host staging, device copy
   void *tab_host = nullptr, *tab_dev = nullptr;
   struct Mixed {
@@ -52,10 +64,11 @@ bool hip_ok(hipError_t e, const char *what);
     if (!::tce_t::hip_ok((x), what)) return TCE_T_ERR_HIP; \
   } while (0)
 
-// Fixed-order per-task sums of a job's partials (ppi per item) into out
+// Fixed-order sums of a job's partials (ppi per item) over the items
+// [bounds[t], bounds[t+1]) of each of ntasks reference tasks into out
 // (2 doubles per task), on stream s.
-int launch_task_reduce(const TaskDev *tasks, int ntasks, long nitems, int ppi, const double *part,
-                       double *out, hipStream_t s);
+int launch_task_reduce(const long *bounds, int ntasks, int ppi, const double *part, double *out,
+                       hipStream_t s);
 // Device task table and partials of 8-wide particle boxes (the cooperative
 // and reduced-precision kernels); a no-op when the job already uses them.
 int ensure_tasks8(Job &J);
@@ -69,5 +82,7 @@ MixedArgs job_margs(const Job &J, int precision);
 void job_release(Job &J);
 // Per-item cost weight (contraction chunks of an item) for scheduling.
 double item_cost(const Job &J, int precision);
+// Device item boundaries of the reference tasks for the table a run uses.
+inline const long *job_bounds(const Job &J, bool wide) { return wide ? J.dbound8 : J.dbound; }
 
 }  // namespace tce_t

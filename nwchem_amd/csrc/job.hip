@@ -45,15 +45,14 @@
 namespace tce_t {
 namespace {
 
-// Per-task fixed-order reduction of the partials (ppi partials per item).
-__global__ void __launch_bounds__(256) task_reduce(const TaskDev *__restrict__ tasks, int ntasks,
-                                                   long nitems, int ppi,
-                                                   const double *__restrict__ part,
+// Per-task fixed-order reduction of the partials (ppi partials per item)
+// over the items [bounds[t], bounds[t+1]) of reference task t.
+__global__ void __launch_bounds__(256) task_reduce(const long *__restrict__ bounds, int ntasks,
+                                                   int ppi, const double *__restrict__ part,
                                                    double *__restrict__ out) {
   const int t = blockIdx.x;
   if (t >= ntasks) return;
-  const long i0 = tasks[t].item0 * ppi,
-             i1 = ((t + 1 < ntasks) ? tasks[t + 1].item0 : nitems) * ppi;
+  const long i0 = bounds[t] * ppi, i1 = bounds[t + 1] * ppi;
   double s4 = 0.0, s5 = 0.0;
   for (long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     s4 += part[2 * i];
@@ -136,10 +135,11 @@ bool hip_ok(hipError_t e, const char *what) {
   return false;
 }
 
-int launch_task_reduce(const TaskDev *tasks, int ntasks, long nitems, int ppi, const double *part,
-                       double *out, hipStream_t s) {
-  hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, s, tasks, ntasks, nitems,
-                     ppi, part, out);
+int launch_task_reduce(const long *bounds, int ntasks, int ppi, const double *part, double *out,
+                       hipStream_t s) {
+  if (ntasks <= 0) return 0;
+  hipLaunchKernelGGL(task_reduce, dim3((unsigned)ntasks), dim3(256), 0, s, bounds, ntasks, ppi,
+                     part, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 }  // namespace tce_t
@@ -157,13 +157,34 @@ extern "C" int tce_t_init(int device) {
   return TCE_T_OK;
 }
 
-// Task table in the order of the reference task loop (ccsd_t_gpu.F:87-92)
-// with work items of pb-wide particle boxes and 4-wide hole boxes.
-static long build_tasks(const Tiles &T, int pb, std::vector<TaskDev> &tasks,
-                        std::vector<long> *task_tiles) {
+// Kernel task table in the order of the reference task loop
+// (ccsd_t_gpu.F:87-92): work items of pb-wide particle boxes and 4-wide hole
+// boxes.  With sub = true each reference task is replaced by the kernel
+// tasks of its box-sized sub-tiles (one item each): sub-tiles qa <= qb <= qc
+// of equal reference tiles only, factors by sub-tile equality (1/2, 1/6) as
+// ccsd_t_gpu.F:153-167 applies them to tiles, so the reference task's energy
+// is the sum of its sub-tasks' and no symmetric duplicate is computed.
+// bound[t] = first item of reference task t (bound[ntask] = items).
+static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &tasks,
+                        std::vector<long> &bound, std::vector<long> *task_tiles) {
   const long noab = T.noab, nvab = T.nvab;
   long item = 0;
   tasks.clear();
+  bound.clear();
+  if (task_tiles) task_tiles->clear();
+  struct Piece {
+    int o, r;
+  };
+  // sub-tiles of tile b with edge e (the last one ragged)
+  auto pieces = [&](long b, int e, bool hole) {
+    std::vector<Piece> v;
+    const int o = (int)(T.ofs(b) - (hole ? 0 : T.no)), r = (int)T.rng(b);
+    for (int k = 0; k < r; k += e) v.push_back({o + k, std::min(e, r - k)});
+    return v;
+  };
+  auto sym_factor = [](bool e12, bool e23) {
+    return (e12 && e23) ? 1.0 / 6.0 : ((e12 || e23) ? 0.5 : 1.0);
+  };
   for (long p4b = noab + 1; p4b <= noab + nvab; ++p4b)
     for (long p5b = p4b; p5b <= noab + nvab; ++p5b)
       for (long p6b = p5b; p6b <= noab + nvab; ++p6b)
@@ -171,34 +192,64 @@ static long build_tasks(const Tiles &T, int pb, std::vector<TaskDev> &tasks,
           for (long h2b = h1b; h2b <= noab; ++h2b)
             for (long h3b = h2b; h3b <= noab; ++h3b) {
               if (!task_allowed(T, p4b, p5b, p6b, h1b, h2b, h3b)) continue;
-              TaskDev d;
-              d.op4 = (int)(T.ofs(p4b) - T.no);
-              d.op5 = (int)(T.ofs(p5b) - T.no);
-              d.op6 = (int)(T.ofs(p6b) - T.no);
-              d.oh1 = (int)T.ofs(h1b);
-              d.oh2 = (int)T.ofs(h2b);
-              d.oh3 = (int)T.ofs(h3b);
-              d.r4 = (int)T.rng(p4b);
-              d.r5 = (int)T.rng(p5b);
-              d.r6 = (int)T.rng(p6b);
-              d.r1 = (int)T.rng(h1b);
-              d.r2 = (int)T.rng(h2b);
-              d.r3 = (int)T.rng(h3b);
-              d.nb4 = (d.r4 + pb - 1) / pb;
-              d.nb5 = (d.r5 + pb - 1) / pb;
-              d.nb6 = (d.r6 + pb - 1) / pb;
-              d.nb1 = (d.r1 + 3) / 4;
-              d.nb2 = (d.r2 + 3) / 4;
-              d.nb3 = (d.r3 + 3) / 4;
-              d.item0 = item;
-              d.factor = task_factor(T.restricted, p4b, p5b, p6b, h1b, h2b, h3b);
-              item += (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
-              tasks.push_back(d);
+              bound.push_back(item);
               if (task_tiles) {
                 long tt[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
                 task_tiles->insert(task_tiles->end(), tt, tt + 6);
               }
+              if (!sub) {
+                TaskDev d;
+                d.op4 = (int)(T.ofs(p4b) - T.no);
+                d.op5 = (int)(T.ofs(p5b) - T.no);
+                d.op6 = (int)(T.ofs(p6b) - T.no);
+                d.oh1 = (int)T.ofs(h1b);
+                d.oh2 = (int)T.ofs(h2b);
+                d.oh3 = (int)T.ofs(h3b);
+                d.r4 = (int)T.rng(p4b);
+                d.r5 = (int)T.rng(p5b);
+                d.r6 = (int)T.rng(p6b);
+                d.r1 = (int)T.rng(h1b);
+                d.r2 = (int)T.rng(h2b);
+                d.r3 = (int)T.rng(h3b);
+                d.nb4 = (d.r4 + pb - 1) / pb;
+                d.nb5 = (d.r5 + pb - 1) / pb;
+                d.nb6 = (d.r6 + pb - 1) / pb;
+                d.nb1 = (d.r1 + 3) / 4;
+                d.nb2 = (d.r2 + 3) / 4;
+                d.nb3 = (d.r3 + 3) / 4;
+                d.item0 = item;
+                d.factor = task_factor(T.restricted, p4b, p5b, p6b, h1b, h2b, h3b);
+                item += (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
+                tasks.push_back(d);
+                continue;
+              }
+              const auto A = pieces(p4b, pb, false), B = pieces(p5b, pb, false),
+                         C = pieces(p6b, pb, false), I = pieces(h1b, 4, true),
+                         J = pieces(h2b, 4, true), K = pieces(h3b, 4, true);
+              const bool pab = p4b == p5b, pbc = p5b == p6b, hij = h1b == h2b, hjk = h2b == h3b;
+              const double rf = T.restricted ? 2.0 : 1.0;
+              // holes outermost, particle c fastest: the item order of one
+              // reference task's boxes
+              for (size_t i = 0; i < I.size(); ++i)
+                for (size_t j = hij ? i : 0; j < J.size(); ++j)
+                  for (size_t k = hjk ? j : 0; k < K.size(); ++k) {
+                    const double hf = sym_factor(hij && i == j, hjk && j == k);
+                    for (size_t a = 0; a < A.size(); ++a)
+                      for (size_t b = pab ? a : 0; b < B.size(); ++b)
+                        for (size_t c = pbc ? b : 0; c < C.size(); ++c) {
+                          TaskDev d;
+                          d.op4 = A[a].o, d.op5 = B[b].o, d.op6 = C[c].o;
+                          d.oh1 = I[i].o, d.oh2 = J[j].o, d.oh3 = K[k].o;
+                          d.r4 = A[a].r, d.r5 = B[b].r, d.r6 = C[c].r;
+                          d.r1 = I[i].r, d.r2 = J[j].r, d.r3 = K[k].r;
+                          d.nb4 = d.nb5 = d.nb6 = d.nb1 = d.nb2 = d.nb3 = 1;
+                          d.item0 = item++;
+                          d.factor = rf * hf * sym_factor(pab && a == b, pbc && b == c);
+                          tasks.push_back(d);
+                        }
+                  }
             }
+  bound.push_back(item);
   return item;
 }
 
@@ -234,11 +285,14 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
     }
     J->ppi = J->pbox == 8 ? 8 : 1;
   }
-  J->nitems = build_tasks(T, J->pbox, J->tasks, &J->task_tiles);
+  // TCE_T_SUBTILE=0 runs the reference tiles as kernel tasks (the work of
+  // the reference's own task loop, padding and symmetric duplicates included)
+  if (const char *e = getenv("TCE_T_SUBTILE")) J->subtile = strcmp(e, "0") != 0;
+  J->nitems = build_tasks(T, J->pbox, J->subtile, J->tasks, J->bound, &J->task_tiles);
   if (J->pbox == 8)
-    J->tasks8 = J->tasks, J->nitems8 = J->nitems;
+    J->tasks8 = J->tasks, J->bound8 = J->bound, J->nitems8 = J->nitems;
   else
-    J->nitems8 = build_tasks(T, 8, J->tasks8, nullptr);
+    J->nitems8 = build_tasks(T, 8, J->subtile, J->tasks8, J->bound8, nullptr);
   J->Ep = round8(T.nv);
   J->Mp = round8(T.no);
   *out = reinterpret_cast<tce_t_job *>(J.release());
@@ -246,7 +300,20 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
 }
 
 extern "C" long tce_t_job_num_tasks(const tce_t_job *job) {
-  return job ? (long)reinterpret_cast<const Job *>(job)->tasks.size() : -1;
+  return job ? reinterpret_cast<const Job *>(job)->ntask() : -1;
+}
+
+extern "C" int tce_t_job_work(const tce_t_job *job, int precision, double *flops, long *items) {
+  if (!job || precision < TCE_T_FP64 || precision > TCE_T_BF16X3) return TCE_T_ERR_ARG;
+  const Job &J = *reinterpret_cast<const Job *>(job);
+  const bool wide = precision != TCE_T_FP64 || J.pbox == 8;
+  const std::vector<TaskDev> &tab = wide ? J.tasks8 : J.tasks;
+  double f = 0.0;
+  for (const TaskDev &d : tab)
+    f += (double)d.r4 * d.r5 * d.r6 * d.r1 * d.r2 * d.r3;
+  if (flops) *flops = f * 18.0 * (double)(J.T.no + J.T.nv + 1);
+  if (items) *items = wide ? J.nitems8 : J.nitems;
+  return TCE_T_OK;
 }
 
 static long dense_bytes(const Job &J) {
@@ -264,7 +331,8 @@ static long dense_bytes(const Job &J) {
 extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
   if (!job) return -1;
   const Job &J = *reinterpret_cast<const Job *>(job);
-  return dense_bytes(J) + J.nitems * J.ppi * 16 + (long)J.tasks.size() * (sizeof(TaskDev) + 16) + 64;
+  return dense_bytes(J) + J.nitems * J.ppi * 16 + (long)J.tasks.size() * sizeof(TaskDev) +
+         (J.ntask() + 1) * 8 + J.ntask() * 16 + 64;
 }
 
 namespace {
@@ -503,13 +571,17 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
       !up(J.devl, T.evl, "evl"))
     return TCE_T_ERR_HIP;
   if (J.dtasks) (void)hipFree(J.dtasks);
-  HIPCHK(hipMalloc(&J.dtasks, J.tasks.size() * sizeof(TaskDev)), "tasks");
+  HIPCHK(hipMalloc(&J.dtasks, std::max<size_t>(1, J.tasks.size()) * sizeof(TaskDev)), "tasks");
   HIPCHK(hipMemcpy(J.dtasks, J.tasks.data(), J.tasks.size() * sizeof(TaskDev), hipMemcpyHostToDevice),
          "tasks");
+  if (J.dbound) (void)hipFree(J.dbound);
+  HIPCHK(hipMalloc(&J.dbound, J.bound.size() * sizeof(long)), "bounds");
+  HIPCHK(hipMemcpy(J.dbound, J.bound.data(), J.bound.size() * sizeof(long), hipMemcpyHostToDevice),
+         "bounds");
   if (J.dpart) (void)hipFree(J.dpart);
   HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * J.ppi * 2 * sizeof(double)), "partials");
   if (J.dtaskE) (void)hipFree(J.dtaskE);
-  HIPCHK(hipMalloc(&J.dtaskE, std::max<size_t>(1, J.tasks.size()) * 2 * sizeof(double)), "taskE");
+  HIPCHK(hipMalloc(&J.dtaskE, std::max(1L, J.ntask()) * 2 * sizeof(double)), "taskE");
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   // persistent grid: 8 waves per CU (2 per SIMD)
@@ -518,6 +590,7 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   if (J.pbox == 8) {
     J.dtasks8 = J.dtasks;
     J.dpart8 = J.dpart;
+    J.dbound8 = J.dbound;
   }
   return TCE_T_OK;
 }
@@ -574,12 +647,17 @@ int tce_t::ensure_tasks8(Job &J) {
   if (J.pbox == 8) {
     J.dtasks8 = J.dtasks;
     J.dpart8 = J.dpart;
+    J.dbound8 = J.dbound;
     return TCE_T_OK;
   }
   HIPCHK(hipMalloc(&J.dtasks8, std::max<size_t>(1, J.tasks8.size()) * sizeof(TaskDev)), "tasks8");
   HIPCHK(hipMemcpy(J.dtasks8, J.tasks8.data(), J.tasks8.size() * sizeof(TaskDev),
                    hipMemcpyHostToDevice),
          "tasks8");
+  HIPCHK(hipMalloc(&J.dbound8, J.bound8.size() * sizeof(long)), "bounds8");
+  HIPCHK(hipMemcpy(J.dbound8, J.bound8.data(), J.bound8.size() * sizeof(long),
+                   hipMemcpyHostToDevice),
+         "bounds8");
   HIPCHK(hipMalloc(&J.dpart8, std::max(1L, J.nitems8) * 8 * 2 * sizeof(double)), "partials8");
   return TCE_T_OK;
 }
@@ -629,7 +707,7 @@ FusedArgs tce_t::job_args(const Job &J, bool wide) {
   A.S = J.dS;
   A.evl = J.devl;
   A.tasks = wide ? J.dtasks8 : J.dtasks;
-  A.ntasks = (int)J.tasks.size();
+  A.ntasks = (int)(wide ? J.tasks8.size() : J.tasks.size());
   A.nitems = wide ? J.nitems8 : J.nitems;
   A.no = (int)J.T.no;
   A.nv = (int)J.T.nv;
@@ -697,7 +775,7 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     const int rc = ensure_tasks8(J);
     if (rc != TCE_T_OK) return rc;
   }
-  const long ntasks = (long)J.tasks.size();
+  const long ntasks = J.ntask();
   J.taskE.assign(2 * ntasks, 0.0);
   if (ntasks == 0 || J.nitems == 0) {
     if (e_bt) *e_bt = 0.0;
@@ -711,7 +789,8 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     if (rc != TCE_T_OK) return rc;
   }
   int lrc;
-  const FusedArgs A = job_args(J, mixed || J.pbox == 8);
+  const bool wide = mixed || J.pbox == 8;
+  const FusedArgs A = job_args(J, wide);
   if (mixed) {
     const MixedSegment *sg = stage_table(J, job_margs(J, precision), J.nitems8);
     if (!sg) return TCE_T_ERR_HIP;
@@ -731,8 +810,8 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     return TCE_T_ERR_HIP;
   }
   HIPCHK(hipEventRecord(J.ev1, J.stream), "event");
-  if (launch_task_reduce(A.tasks, (int)ntasks, A.nitems, mixed ? 8 : J.ppi, A.partials, J.dtaskE,
-                         J.stream) != 0) {
+  if (launch_task_reduce(job_bounds(J, wide), (int)ntasks, wide ? 8 : J.ppi, A.partials,
+                         J.dtaskE, J.stream) != 0) {
     set_error("task_reduce launch failed");
     return TCE_T_ERR_HIP;
   }
@@ -788,8 +867,8 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   const bool wide = mixed || J.pbox == 8;
   const FusedArgs A = job_args(J, wide);
-  const std::vector<TaskDev> &tab = wide ? J.tasks8 : J.tasks;
-  const long ntasks = (long)J.tasks.size(), nitems = A.nitems;
+  const std::vector<long> &bnd = wide ? J.bound8 : J.bound;
+  const long ntasks = J.ntask();
   J.taskE.resize(2 * ntasks);
   long done = 0;
   while (*begin <= nvab && (max_tiles <= 0 || done < max_tiles)) {
@@ -800,7 +879,7 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
     while (t1 < ntasks && J.task_tiles[6 * t1] == p4b) ++t1;
     double e = 0.0;
     if (t1 > t0) {
-      const long i0 = tab[t0].item0, i1 = t1 < ntasks ? tab[t1].item0 : nitems;
+      const long i0 = bnd[t0], i1 = bnd[t1];
       int lrc;
       if (mixed) {
         const MixedSegment *sg = stage_table(J, job_margs(J, precision), i1, i0);
@@ -818,8 +897,8 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
         set_error("fused kernel launch failed");
         return TCE_T_ERR_HIP;
       }
-      if (launch_task_reduce(A.tasks + t0, (int)(t1 - t0), i1, wide ? 8 : J.ppi, A.partials,
-                             J.dtaskE + 2 * t0, J.stream) != 0) {
+      if (launch_task_reduce(job_bounds(J, wide) + t0, (int)(t1 - t0), wide ? 8 : J.ppi,
+                             A.partials, J.dtaskE + 2 * t0, J.stream) != 0) {
         set_error("task_reduce launch failed");
         return TCE_T_ERR_HIP;
       }
@@ -844,7 +923,7 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
   if (!job || !out) return TCE_T_ERR_ARG;
   const Job &J = *reinterpret_cast<const Job *>(job);
-  if ((long)J.tasks.size() != ntasks || (long)J.taskE.size() != 2 * ntasks) return TCE_T_ERR_STATE;
+  if (J.ntask() != ntasks || (long)J.taskE.size() != 2 * ntasks) return TCE_T_ERR_STATE;
   std::memcpy(out, J.taskE.data(), 2 * ntasks * sizeof(double));
   return TCE_T_OK;
 }
@@ -863,8 +942,12 @@ void tce_t::job_release(Job &J) {
   if (J.stream) (void)hipStreamSynchronize(J.stream);
   if (J.dtasks8 && J.dtasks8 != J.dtasks) (void)hipFree(J.dtasks8);
   if (J.dpart8 && J.dpart8 != J.dpart) (void)hipFree(J.dpart8);
+  if (J.dbound8 && J.dbound8 != J.dbound) (void)hipFree(J.dbound8);
   J.dtasks8 = nullptr;
   J.dpart8 = nullptr;
+  J.dbound8 = nullptr;
+  if (J.dbound) (void)hipFree(J.dbound);
+  J.dbound = nullptr;
   double **bufs[] = {&J.dT1, &J.dT2e, &J.dT2m, &J.dVd2, &J.dWd1, &J.dS, &J.devl, &J.dpart, &J.dtaskE};
   for (double **b : bufs) {
     if (*b) (void)hipFree(*b);

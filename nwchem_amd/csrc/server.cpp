@@ -91,7 +91,7 @@ struct Server {
 
 Job &job_of(Server &S, int sys) { return *reinterpret_cast<Job *>(S.systems[sys].job); }
 
-long request_bytes(const Job &J) { return std::max(1L, J.nitems8) * 8 * 16 + (long)J.tasks.size() * 16; }
+long request_bytes(const Job &J) { return std::max(1L, J.nitems8) * 8 * 16 + J.ntask() * 16; }
 
 // Bytes request r would add to the device (caller holds the lock).
 long admission_bytes(Server &S, const Request &r) {
@@ -120,7 +120,7 @@ int activate(Server &S, Request &r) {
   if (rc != TCE_T_OK) return rc;
   r.nitems = J.nitems8;
   HIPCHK(hipMalloc(&r.dpart, std::max(1L, J.nitems8) * 8 * 2 * sizeof(double)), "request partials");
-  HIPCHK(hipMalloc(&r.dtaskE, std::max<size_t>(1, J.tasks.size()) * 2 * sizeof(double)),
+  HIPCHK(hipMalloc(&r.dtaskE, std::max(1L, J.ntask()) * 2 * sizeof(double)),
          "request energies");
   return TCE_T_OK;
 }
@@ -204,10 +204,10 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
 // Reduce a finished request's partials and read its energies.
 int finish(Server &S, Request &r) {
   Job &J = job_of(S, r.sys);
-  const long nt = (long)J.tasks.size();
+  const long nt = J.ntask();
   std::vector<double> te(2 * std::max(1L, nt));
   if (nt > 0) {
-    if (launch_task_reduce(J.dtasks8, (int)nt, J.nitems8, 8, r.dpart, r.dtaskE, S.stream) != 0) {
+    if (launch_task_reduce(J.dbound8, (int)nt, 8, r.dpart, r.dtaskE, S.stream) != 0) {
       set_error("request task_reduce failed");
       return TCE_T_ERR_HIP;
     }

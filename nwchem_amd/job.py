@@ -86,6 +86,14 @@ class TriplesJob:
               "tce_t_job_task_energies")
         return out
 
+    def work(self, precision: int = FP64) -> tuple[float, int]:
+        """(flops, work items) of a run in `precision`: the reference's
+        per-task flop count summed over the kernel tasks actually run (the
+        box-sized sub-tiles of the reference tiles unless TCE_T_SUBTILE=0)."""
+        f, n = C.c_double(0.0), C.c_long(0)
+        check(_lib.lib().tce_t_job_work(self._h, precision, C.byref(f), C.byref(n)), "tce_t_job_work")
+        return f.value, n.value
+
     def precision_bytes(self, precision: int) -> int:
         """Extra device bytes the first run in `precision` allocates."""
         return int(_lib.lib().tce_t_job_precision_bytes(self._h, precision))

@@ -80,6 +80,30 @@ def test_job_matches_oracle(case, kernel, monkeypatch):
     assert_parity(e, te, ref)
 
 
+@pytest.mark.parametrize("kernel", ["wave", "wg16"])
+@pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
+def test_reference_tiling_matches_oracle(case, kernel, monkeypatch):
+    """TCE_T_SUBTILE=0: the reference tiles themselves as kernel tasks (boxes
+    over each tile, the symmetric duplicates inside equal tiles computed and
+    weighted as the reference does)."""
+    monkeypatch.setenv("TCE_T_KERNEL", kernel)
+    monkeypatch.setenv("TCE_T_SUBTILE", "0")
+    no, nv, to, tv, seed, ro, rv = case
+    tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
+    ref = oracle_energies(ot, b)
+    print(f"case no={no} nv={nv} ranges={list(tm.range)} kernel={kernel} reference tiling", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        e = job.run()
+        te = job.task_energies()
+        f, n = job.work()
+    r = tm.range
+    f_ref = sum(float(r[a - 1] * r[b - 1] * r[c - 1] * r[i - 1] * r[j - 1] * r[k - 1])
+                for a, b, c, i, j, k in tm.tasks()) * 18 * (no + nv + 1)
+    assert f == f_ref and n >= len(ref[2])
+    assert_parity(e, te, ref)
+
+
 def test_job_is_deterministic_and_reusable():
     tm, b, ot = system(8, 32, 8, 16, 99)
     with TriplesJob(tm) as job:

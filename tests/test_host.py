@@ -166,3 +166,59 @@ def test_task_list_matches_oracle():
         assert len(tm.tasks()) == ot.count_tasks()
     tm = tl.TileModel.synthetic(40, 320, 20, 40, 1)
     assert len(tm.tasks()) == 480
+
+
+def _job_work(tm, subtile: bool, precision: int = 0):
+    """(reference tasks, flops, items) of a job plan (tce_t_job_create needs
+    no device)."""
+    L = _lib.lib()
+    old = os.environ.get("TCE_T_SUBTILE")
+    os.environ["TCE_T_SUBTILE"] = "1" if subtile else "0"
+    try:
+        h = C.c_void_p()
+        _lib.check(L.tce_t_job_create(C.byref(tm.c()), C.byref(h)), "tce_t_job_create")
+    finally:
+        if old is None:
+            del os.environ["TCE_T_SUBTILE"]
+        else:
+            os.environ["TCE_T_SUBTILE"] = old
+    f, n = C.c_double(0), C.c_long(0)
+    _lib.check(L.tce_t_job_work(h, precision, C.byref(f), C.byref(n)), "tce_t_job_work")
+    nt = L.tce_t_job_num_tasks(h)
+    L.tce_t_job_destroy(h)
+    return nt, f.value, n.value
+
+
+def _f_ref(tm):
+    r = tm.range
+    return float(sum(r[a - 1] * r[b - 1] * r[c - 1] * r[i - 1] * r[j - 1] * r[k - 1]
+                     for a, b, c, i, j, k in tm.tasks())) * 18 * (tm.no + tm.nv + 1)
+
+
+@pytest.mark.parametrize("no,nv,to,tv", [(40, 320, 20, 40), (8, 32, 8, 16), (12, 48, 12, 24)])
+def test_subtiled_work_is_the_box_tiling(no, nv, to, tv):
+    """Sub-tiling a job whose tiles are whole numbers of boxes does exactly
+    the work of the reference task loop run on box-sized tiles (4 occupied,
+    8 virtual orbitals): same flops, one item per sub-task; the reference
+    task count (per-task energies) is unchanged."""
+    tm = tl.TileModel.synthetic(no, nv, to, tv, 1)
+    fine = tl.TileModel.synthetic(no, nv, 4, 8, 1)
+    nt, f, n = _job_work(tm, True)
+    nt0, f0, n0 = _job_work(tm, False)
+    assert nt == nt0 == len(tm.tasks())
+    assert f0 == _f_ref(tm)
+    assert f == _f_ref(fine)
+    assert n == len(fine.tasks())
+    assert f < f0 and n < n0
+
+
+def test_subtiled_work_ragged_tiles_counts_every_orbital_sextuple_once():
+    """Ragged tiles: the sub-tiled work is the reference count of a tiling
+    refined to the sub-tiles, which never exceeds the reference tiling's."""
+    for ro, rv in [([6, 3], [16, 16, 9]), ([9], [21, 21, 21, 21, 21, 21]), ([5, 5, 5], [13, 13])]:
+        tm = tl.TileModel.synthetic(sum(ro), sum(rv), 0, 0, 3, ro, rv)
+        refine = lambda rs, e: [min(e, r - k) for r in rs for k in range(0, r, e)]
+        fine = tl.TileModel.synthetic(sum(ro), sum(rv), 0, 0, 3, refine(ro, 4), refine(rv, 8))
+        nt, f, n = _job_work(tm, True, precision=2)  # 8-wide particle sub-tiles
+        assert f == _f_ref(fine) and n == len(fine.tasks())
+        assert f <= _f_ref(tm)
