@@ -40,12 +40,13 @@ METRIC = "(T) TFLOP/s & energies/s, FP64 vs split-BF16 at ≤1 µEh; serving p50
 PEAK_FP64_TFLOPS = 78.6
 
 
-def job_work(tm, precision: int = 0) -> float:
-    """Flops a run of this job performs (the kernel tasks the library
-    builds for it, tce_t_job_work); no upload needed."""
+def job_work(tm, precision: int = 0, tiling: int = 2) -> float:
+    """Flops a run of this job performs in `tiling` (the kernel tasks the
+    library builds for it, tce_t_job_work; 2 = orbital, what the server
+    uses); no upload needed."""
     from nwchem_amd import TriplesJob
 
-    with TriplesJob(tm) as j:
+    with TriplesJob(tm, tiling=tiling) as j:
         return j.work(precision)[0]
 
 
@@ -148,8 +149,8 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
     """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256}, occupied tiles
     6-20, virtual tiles 16-48, ragged last tiles) in one persistent launch per
     run (autotuned, replayed as a hipGraph), against the same jobs run one
-    after another."""
-    from nwchem_amd import TriplesBatch, TriplesJob, tiles as tl
+    after another.  The jobs use the orbital tiling (E[T], E(T) per job)."""
+    from nwchem_amd import TILING_ORBITAL, TriplesBatch, TriplesJob, tiles as tl
 
     rng = np.random.default_rng(seed)
     jobs, F = [], 0.0
@@ -160,7 +161,7 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
         to, tv = int(rng.integers(6, 21)), int(rng.integers(16, 49))
         tm = tl.TileModel.synthetic(no, nv, 0, 0, seed + k, ragged(no, to), ragged(nv, tv))
         b = tl.synth_blocks(tm, seed + k)
-        j = TriplesJob(tm)
+        j = TriplesJob(tm, tiling=TILING_ORBITAL)  # a batch returns each job's E[T], E(T) only
         j.upload(b)
         del b
         jobs.append(j)

@@ -133,8 +133,26 @@ int tce_t_init(int device);
 int tce_t_device_memory(long *free_bytes, long *total_bytes);
 
 /* Plan a job: enumerate the tile sextuples of ccsd_t_gpu.F:87-112 and size
- * the device tensors.  The tile arrays are copied. */
+ * the device tensors.  The tile arrays are copied.  Kernel tasks: the
+ * default tiling (TCE_T_TILING_SUBTILE, or TCE_T_SUBTILE=0|1|2 in the
+ * environment for REFERENCE | SUBTILE | ORBITAL). */
 int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **job);
+/* Tiling of the kernel tasks (no reference counterpart; the energies are
+ * the same sums, up to FP64 rounding, in every mode):
+ *   REFERENCE  the reference tiles, boxes over each (the reference's work);
+ *   SUBTILE    box-sized sub-tiles of each reference tile (4 occupied, 4 or
+ *              8 virtual orbitals) in canonical order: no symmetric
+ *              duplicates inside equal tiles; per-task energies and restart
+ *              blocks as REFERENCE;
+ *   ORBITAL    box-sized pieces of each run of consecutive tiles of one spin
+ *              and irrep, ignoring the reference tile edges (less padding
+ *              with ragged tiles); E[T] and E(T) only: task energies and
+ *              restart blocks are TCE_T_ERR_STATE. */
+#define TCE_T_TILING_REFERENCE 0
+#define TCE_T_TILING_SUBTILE 1
+#define TCE_T_TILING_ORBITAL 2
+int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce_t_job **job);
+int tce_t_job_tiling(const tce_t_job *job);
 long tce_t_job_num_tasks(const tce_t_job *job);
 /* Device bytes the job holds once uploaded (admission control). */
 long tce_t_job_device_bytes(const tce_t_job *job);
@@ -199,9 +217,8 @@ int tce_t_job_kernel(const tce_t_job *job);
  * the box-sized sub-tiles of the reference tiles (4 occupied, 4 or 8
  * virtual orbitals) in canonical order, which leaves out the symmetric
  * duplicates a reference task computes inside equal tiles: the energies
- * are the same sums (per reference task, up to FP64 rounding) at less work.
- * TCE_T_SUBTILE=0 in the environment at tce_t_job_create keeps the
- * reference tiles.  No reference counterpart. */
+ * are the same sums (per reference task, up to FP64 rounding) at less work
+ * (see tce_t_job_create_tiling).  No reference counterpart. */
 int tce_t_job_work(const tce_t_job *job, int precision, double *flops, long *items);
 int tce_t_job_destroy(tce_t_job *job);
 

@@ -90,6 +90,8 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_kernel.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.restype = C.c_int
     L.tce_t_job_work.argtypes = [C.c_void_p, C.c_int, DP, C.POINTER(C.c_long)]
+    L.tce_t_job_create_tiling.argtypes = [C.POINTER(TilesC), C.c_int, C.POINTER(C.c_void_p)]
+    L.tce_t_job_tiling.argtypes = [C.c_void_p]
     L.tce_t_job_destroy.argtypes = [C.c_void_p]
     L.tce_t_pack_dense.argtypes = [C.POINTER(TilesC), DP, LP, DP, LP, DP, LP] + [DP] * 6
     L.tce_t_tile_ranges.argtypes = [C.c_long, C.c_long, LP]

@@ -162,7 +162,7 @@ int enqueue(Batch &B, int precision, const Plan &P) {
   }
   for (size_t j = 0; j < n; ++j) {
     const Job &J = *B.jobs[j];
-    const long nt = J.ntask();
+    const long nt = J.ngroups();
     if (nt == 0) continue;
     const FusedArgs A = job_args(J, wide);
     if (launch_task_reduce(job_bounds(J, wide), (int)nt, wide ? 8 : J.ppi, A.partials, J.dtaskE,
@@ -259,7 +259,7 @@ extern "C" int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch
     }
     B->jobs.push_back(J);
     B->eoff.push_back(B->ntasks);
-    B->ntasks += J->ntask();
+    B->ntasks += J->ngroups();
     dev = J->device;
   }
   auto fail = [&](hipError_t e, const char *what) {
@@ -300,7 +300,7 @@ extern "C" int tce_t_batch_run(tce_t_batch *batch, int precision, double *energi
   HIPCHK(hipEventElapsedTime(&B.last_ms, B.ev0, B.ev1), "elapsed");
   for (size_t j = 0; j < B.jobs.size(); ++j) {
     Job &J = *B.jobs[j];
-    const long nt = J.ntask();
+    const long nt = J.ngroups();
     J.taskE.assign(B.hE + 2 * B.eoff[j], B.hE + 2 * (B.eoff[j] + nt));
     double s4 = 0.0, s5 = 0.0;
     for (long t = 0; t < nt; ++t) {

@@ -11,21 +11,27 @@
 
 namespace tce_t {
 
-// Kernel task tables.  A reference task (tile sextuple of the reference task
-// loop) runs as one kernel task with boxes over its tiles, or -- sub-tiled,
-// the default -- as the kernel tasks of the box-sized sub-tiles of its tiles
-// (canonical sub-tile order, factors by sub-tile equality), which skips the
-// symmetric duplicates and the padding inside equal tiles.  Either way a
-// reference task owns the contiguous items [bound[t], bound[t+1]), so its
-// energy is a fixed-order sum of its own items.
+// Kernel task tables (tiling modes TCE_T_TILING_*).  A reference task (tile
+// sextuple of the reference task loop) runs as one kernel task with boxes
+// over its tiles (REFERENCE), or as the kernel tasks of the box-sized
+// sub-tiles of its tiles (SUBTILE, the default: canonical sub-tile order,
+// factors by sub-tile equality), which skips the symmetric duplicates and
+// the padding inside equal tiles; either way reference task t owns the
+// contiguous items [bound[t], bound[t+1]), so its energy is a fixed-order
+// sum of its own items.  ORBITAL re-tiles each run of consecutive tiles of
+// one spin and irrep into box-sized pieces regardless of the reference tile
+// edges (one ragged piece per run instead of one per tile); only the job's
+// totals are defined then (bound = {0, items}, one reduction group).
 struct Job {
   Tiles T;
   std::vector<TaskDev> tasks;
-  std::vector<long> bound;       // item boundaries of the reference tasks in `tasks`
+  std::vector<long> bound;       // item boundaries of the reduction groups in `tasks`
   std::vector<long> task_tiles;  // 6 per reference task: p4b,p5b,p6b,h1b,h2b,h3b
   long nitems = 0;
-  bool subtile = true;
+  int tiling = TCE_T_TILING_SUBTILE;
   long ntask() const { return (long)task_tiles.size() / 6; }
+  // reduction groups: the reference tasks, or the whole job (ORBITAL)
+  long ngroups() const { return (long)bound.size() - 1; }
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -253,9 +253,68 @@ static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &
   return item;
 }
 
+// Orbital tiling: each run of consecutive tiles with one spin and irrep
+// (a contiguous orbital range) cut into pieces of e4 occupied or ep virtual
+// orbitals, the last one ragged; spin, irrep, orbital energies and the
+// restricted flag carry over, so the task filters and factors of the task
+// loop apply to the pieces unchanged.
+static Tiles orbital_tiles(const Tiles &T, int ep) {
+  Tiles R;
+  R.restricted = T.restricted;
+  R.no = T.no;
+  R.nv = T.nv;
+  R.norb = T.norb;
+  R.evl = T.evl;
+  auto space = [&](long first, long last, int e) {
+    for (long b = first; b <= last;) {
+      long c = b;
+      while (c + 1 <= last && T.spn(c + 1) == T.spn(b) && T.sy(c + 1) == T.sy(b)) ++c;
+      const long o = T.ofs(b), end = T.ofs(c) + T.rng(c);
+      for (long k = o; k < end; k += e) {
+        R.range.push_back(std::min<long>(e, end - k));
+        R.offset.push_back(k);
+        R.spin.push_back(T.spn(b));
+        R.sym.push_back(T.sy(b));
+      }
+      b = c + 1;
+    }
+  };
+  space(1, T.noab, 4);
+  R.noab = (long)R.range.size();
+  space(T.noab + 1, T.noab + T.nvab, ep);
+  R.nvab = (long)R.range.size() - R.noab;
+  for (long i = 1; i <= R.noab + R.nvab; ++i) R.alpha.push_back(i);
+  return R;
+}
+
+// Kernel task table of a job in its tiling mode (pb: particle box).
+static long job_tasks(const Job &J, int pb, std::vector<TaskDev> &tasks, std::vector<long> &bound) {
+  if (J.tiling == TCE_T_TILING_ORBITAL) {
+    const long n = build_tasks(orbital_tiles(J.T, pb), pb, false, tasks, bound, nullptr);
+    bound.assign({0L, n});
+    return n;
+  }
+  return build_tasks(J.T, pb, J.tiling == TCE_T_TILING_SUBTILE, tasks, bound, nullptr);
+}
+
 extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
+  int tiling = TCE_T_TILING_SUBTILE;
+  if (const char *e = getenv("TCE_T_SUBTILE")) tiling = atoi(e);
+  return tce_t_job_create_tiling(tiles, tiling, out);
+}
+
+extern "C" int tce_t_job_tiling(const tce_t_job *job) {
+  return job ? reinterpret_cast<const Job *>(job)->tiling : TCE_T_ERR_ARG;
+}
+
+extern "C" int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce_t_job **out) {
   if (!out) return TCE_T_ERR_ARG;
+  if (tiling < TCE_T_TILING_REFERENCE || tiling > TCE_T_TILING_ORBITAL) {
+    set_error("unknown tiling mode");
+    return TCE_T_ERR_ARG;
+  }
   auto J = std::make_unique<Job>();
+  J->tiling = tiling;
   if (!J->T.load(tiles)) {
     set_error("invalid tile description");
     return TCE_T_ERR_ARG;
@@ -285,14 +344,16 @@ extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
     }
     J->ppi = J->pbox == 8 ? 8 : 1;
   }
-  // TCE_T_SUBTILE=0 runs the reference tiles as kernel tasks (the work of
-  // the reference's own task loop, padding and symmetric duplicates included)
-  if (const char *e = getenv("TCE_T_SUBTILE")) J->subtile = strcmp(e, "0") != 0;
-  J->nitems = build_tasks(T, J->pbox, J->subtile, J->tasks, J->bound, &J->task_tiles);
+  {  // the reference task list (per-task energies, restart blocks)
+    std::vector<TaskDev> ref;
+    std::vector<long> rb;
+    build_tasks(T, J->pbox, false, ref, rb, &J->task_tiles);
+  }
+  J->nitems = job_tasks(*J, J->pbox, J->tasks, J->bound);
   if (J->pbox == 8)
     J->tasks8 = J->tasks, J->bound8 = J->bound, J->nitems8 = J->nitems;
   else
-    J->nitems8 = build_tasks(T, 8, J->subtile, J->tasks8, J->bound8, nullptr);
+    J->nitems8 = job_tasks(*J, 8, J->tasks8, J->bound8);
   J->Ep = round8(T.nv);
   J->Mp = round8(T.no);
   *out = reinterpret_cast<tce_t_job *>(J.release());
@@ -332,7 +393,7 @@ extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
   if (!job) return -1;
   const Job &J = *reinterpret_cast<const Job *>(job);
   return dense_bytes(J) + J.nitems * J.ppi * 16 + (long)J.tasks.size() * sizeof(TaskDev) +
-         (J.ntask() + 1) * 8 + J.ntask() * 16 + 64;
+         (J.ngroups() + 1) * 8 + J.ngroups() * 16 + 64;
 }
 
 namespace {
@@ -581,7 +642,7 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   if (J.dpart) (void)hipFree(J.dpart);
   HIPCHK(hipMalloc(&J.dpart, std::max(1L, J.nitems) * J.ppi * 2 * sizeof(double)), "partials");
   if (J.dtaskE) (void)hipFree(J.dtaskE);
-  HIPCHK(hipMalloc(&J.dtaskE, std::max(1L, J.ntask()) * 2 * sizeof(double)), "taskE");
+  HIPCHK(hipMalloc(&J.dtaskE, std::max(1L, J.ngroups()) * 2 * sizeof(double)), "taskE");
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   // persistent grid: 8 waves per CU (2 per SIMD)
@@ -775,7 +836,7 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     const int rc = ensure_tasks8(J);
     if (rc != TCE_T_OK) return rc;
   }
-  const long ntasks = J.ntask();
+  const long ntasks = J.ngroups();
   J.taskE.assign(2 * ntasks, 0.0);
   if (ntasks == 0 || J.nitems == 0) {
     if (e_bt) *e_bt = 0.0;
@@ -848,6 +909,10 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
   if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
+  }
+  if (J.tiling == TCE_T_TILING_ORBITAL) {
+    set_error("restart blocks need the reference tiles: the job uses TCE_T_TILING_ORBITAL");
+    return TCE_T_ERR_STATE;
   }
   const long noab = J.T.noab, nvab = J.T.nvab;
   if (*begin < 1) *begin = 1;
@@ -923,6 +988,10 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
   if (!job || !out) return TCE_T_ERR_ARG;
   const Job &J = *reinterpret_cast<const Job *>(job);
+  if (J.tiling == TCE_T_TILING_ORBITAL) {
+    set_error("per-task energies need the reference tiles: the job uses TCE_T_TILING_ORBITAL");
+    return TCE_T_ERR_STATE;
+  }
   if (J.ntask() != ntasks || (long)J.taskE.size() != 2 * ntasks) return TCE_T_ERR_STATE;
   std::memcpy(out, J.taskE.data(), 2 * ntasks * sizeof(double));
   return TCE_T_OK;

@@ -91,7 +91,7 @@ struct Server {
 
 Job &job_of(Server &S, int sys) { return *reinterpret_cast<Job *>(S.systems[sys].job); }
 
-long request_bytes(const Job &J) { return std::max(1L, J.nitems8) * 8 * 16 + J.ntask() * 16; }
+long request_bytes(const Job &J) { return std::max(1L, J.nitems8) * 8 * 16 + J.ngroups() * 16; }
 
 // Bytes request r would add to the device (caller holds the lock).
 long admission_bytes(Server &S, const Request &r) {
@@ -120,7 +120,7 @@ int activate(Server &S, Request &r) {
   if (rc != TCE_T_OK) return rc;
   r.nitems = J.nitems8;
   HIPCHK(hipMalloc(&r.dpart, std::max(1L, J.nitems8) * 8 * 2 * sizeof(double)), "request partials");
-  HIPCHK(hipMalloc(&r.dtaskE, std::max(1L, J.ntask()) * 2 * sizeof(double)),
+  HIPCHK(hipMalloc(&r.dtaskE, std::max(1L, J.ngroups()) * 2 * sizeof(double)),
          "request energies");
   return TCE_T_OK;
 }
@@ -204,7 +204,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
 // Reduce a finished request's partials and read its energies.
 int finish(Server &S, Request &r) {
   Job &J = job_of(S, r.sys);
-  const long nt = J.ntask();
+  const long nt = J.ngroups();
   std::vector<double> te(2 * std::max(1L, nt));
   if (nt > 0) {
     if (launch_task_reduce(J.dbound8, (int)nt, 8, r.dpart, r.dtaskE, S.stream) != 0) {
@@ -343,7 +343,11 @@ extern "C" int tce_t_server_add_system(tce_t_server *server, const tce_t_tiles *
   if (!server || !tiles || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
   Server &S = *reinterpret_cast<Server *>(server);
   System sy;
-  const int rc = tce_t_job_create(tiles, &sy.job);
+  // requests return E[T] and E(T) only: the orbital tiling (least padding)
+  // unless TCE_T_SUBTILE names another
+  const int rc = getenv("TCE_T_SUBTILE")
+                     ? tce_t_job_create(tiles, &sy.job)
+                     : tce_t_job_create_tiling(tiles, TCE_T_TILING_ORBITAL, &sy.job);
   if (rc != TCE_T_OK) return rc;
   sy.t1 = t1, sy.t1h = t1h, sy.t2 = t2, sy.t2h = t2h, sy.v2 = v2, sy.v2h = v2h;
   std::lock_guard<std::mutex> lk(S.mu);

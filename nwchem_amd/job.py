@@ -22,6 +22,11 @@ FP64 = 0    # d1/d2 contractions on FP64 MFMA
 FP32 = 1    # FP32 operands, FP32 MFMA and accumulation
 BF16X3 = 2  # operands split into bf16 hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulation
 
+# Kernel-task tilings (tce_t_job_create_tiling)
+TILING_REFERENCE = 0  # the reference tiles (the reference's work)
+TILING_SUBTILE = 1    # box-sized sub-tiles of each reference tile (default)
+TILING_ORBITAL = 2    # box-sized pieces of each spin/irrep run of orbitals; totals only
+
 _initialised = False
 
 
@@ -40,12 +45,21 @@ def device_memory() -> tuple[int, int]:
 
 
 class TriplesJob:
-    def __init__(self, tiles: TileModel, device: int = 0):
+    def __init__(self, tiles: TileModel, device: int = 0, tiling: int | None = None):
+        """tiling: TILING_REFERENCE, TILING_SUBTILE or TILING_ORBITAL (None:
+        the library default, TILING_SUBTILE unless TCE_T_SUBTILE says
+        otherwise).  TILING_ORBITAL jobs give E[T] and E(T) only."""
         init(device)
         self.tiles = tiles
         self._h = C.c_void_p()
-        check(_lib.lib().tce_t_job_create(C.byref(tiles.c()), C.byref(self._h)), "tce_t_job_create")
-        self.ntasks = int(_lib.lib().tce_t_job_num_tasks(self._h))
+        L = _lib.lib()
+        if tiling is None:
+            check(L.tce_t_job_create(C.byref(tiles.c()), C.byref(self._h)), "tce_t_job_create")
+        else:
+            check(L.tce_t_job_create_tiling(C.byref(tiles.c()), int(tiling), C.byref(self._h)),
+                  "tce_t_job_create_tiling")
+        self.ntasks = int(L.tce_t_job_num_tasks(self._h))
+        self.tiling = int(L.tce_t_job_tiling(self._h))
 
     @property
     def device_bytes(self) -> int:

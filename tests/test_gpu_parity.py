@@ -104,6 +104,44 @@ def test_reference_tiling_matches_oracle(case, kernel, monkeypatch):
     assert_parity(e, te, ref)
 
 
+@pytest.mark.parametrize("kernel", ["wave", "wg16"])
+@pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
+def test_orbital_tiling_matches_oracle(case, kernel, monkeypatch):
+    """TILING_ORBITAL: box-sized pieces of the whole occupied and virtual
+    spaces as kernel tasks, E[T] and E(T) against the oracle's sums over the
+    reference tiles; per-task energies and restart blocks are refused."""
+    from nwchem_amd import TILING_ORBITAL
+    monkeypatch.setenv("TCE_T_KERNEL", kernel)
+    no, nv, to, tv, seed, ro, rv = case
+    tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
+    e1, e2, _ = oracle_energies(ot, b)
+    print(f"case no={no} nv={nv} ranges={list(tm.range)} kernel={kernel} orbital tiling", flush=True)
+    with TriplesJob(tm, tiling=TILING_ORBITAL) as job:
+        job.upload(b)
+        e = job.run()
+        assert abs(e[0] - e1) <= tol(e1) and abs(e[1] - e2) <= tol(e2), (e, e1, e2)
+        with pytest.raises(RuntimeError):
+            job.task_energies()
+        with pytest.raises(RuntimeError):
+            job.run_restart()
+        eb = job.run(2)
+        assert abs(eb[0] - e1) <= MIXED_REL[2] * abs(e1) + 1e-12
+
+
+@pytest.mark.parametrize("prm", [dict(sizes_o=[3, 2], sizes_v=[4, 3], irreps_o=[0, 0], irreps_v=[1, 1], seed=5),
+                                 dict(sizes_o=[2, 2], sizes_v=[5, 6], seed=10, flip_symmetric=True,
+                                      restricted=True)], ids=["spin_irreps", "restricted"])
+def test_orbital_tiling_spin_systems(prm):
+    from nwchem_amd import TILING_ORBITAL
+    s = spin_systems.SpinSystem(**prm)
+    b = s.blocks
+    e1, e2, _ = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
+    with TriplesJob(s.tiles, tiling=TILING_ORBITAL) as job:
+        job.upload(b)
+        e = job.run()
+    assert abs(e[0] - e1) <= tol(e1) and abs(e[1] - e2) <= tol(e2), (e, e1, e2)
+
+
 def test_job_is_deterministic_and_reusable():
     tm, b, ot = system(8, 32, 8, 16, 99)
     with TriplesJob(tm) as job:
@@ -317,6 +355,33 @@ def test_batch_matches_oracle_and_single_runs():
                 print(p, "plan", batch.plan(p), flush=True)
                 for k in range(len(jobs)):
                     assert tuple(ep[k]) == singles[p][k], (p, k, ep[k], singles[p][k])
+    finally:
+        for j in jobs:
+            j.close()
+
+
+def test_batch_of_mixed_tilings():
+    """Jobs of different tilings (orbital, sub-tile, reference) in one
+    batch: FP64 totals match the oracle, reduced-precision totals equal each
+    job's own run bit for bit."""
+    from nwchem_amd import TILING_ORBITAL, TILING_REFERENCE, TILING_SUBTILE, TriplesBatch
+    systems = _batch_systems()
+    tilings = [TILING_ORBITAL, TILING_SUBTILE, TILING_REFERENCE, TILING_ORBITAL, TILING_SUBTILE]
+    jobs = []
+    try:
+        for (tm, b, ot), t in zip(systems, tilings):
+            j = TriplesJob(tm, tiling=t)
+            j.upload(b)
+            jobs.append(j)
+        singles = [j.run(2) for j in jobs]
+        with TriplesBatch(jobs) as batch:
+            e64 = batch.run(0)
+            for k, (tm, b, ot) in enumerate(systems):
+                e1, e2, _ = oracle_energies(ot, b)
+                assert abs(e64[k, 0] - e1) <= tol(e1) and abs(e64[k, 1] - e2) <= tol(e2), k
+            ep = batch.run(2)
+            for k in range(len(jobs)):
+                assert tuple(ep[k]) == singles[k], (k, ep[k], singles[k])
     finally:
         for j in jobs:
             j.close()

@@ -222,3 +222,54 @@ def test_subtiled_work_ragged_tiles_counts_every_orbital_sextuple_once():
         nt, f, n = _job_work(tm, True, precision=2)  # 8-wide particle sub-tiles
         assert f == _f_ref(fine) and n == len(fine.tasks())
         assert f <= _f_ref(tm)
+
+
+def _work_tiling(tm, tiling: int, precision: int = 0):
+    L = _lib.lib()
+    h = C.c_void_p()
+    _lib.check(L.tce_t_job_create_tiling(C.byref(tm.c()), tiling, C.byref(h)), "create_tiling")
+    f, n = C.c_double(0), C.c_long(0)
+    _lib.check(L.tce_t_job_work(h, precision, C.byref(f), C.byref(n)), "tce_t_job_work")
+    assert L.tce_t_job_tiling(h) == tiling
+    L.tce_t_job_destroy(h)
+    return f.value, n.value
+
+
+@pytest.mark.parametrize("ro,rv", [([6, 3], [16, 16, 9]), ([9], [21] * 6), ([13, 13, 2], [42, 21])])
+def test_orbital_tiling_is_the_box_tiling_of_each_space(ro, rv):
+    """TCE_T_TILING_ORBITAL on a C1 spin-orbital tiling (one spin, one irrep):
+    the work of the reference task loop on the whole occupied space cut into
+    4s and the whole virtual space into 8s, whatever the reference tiles."""
+    no, nv = sum(ro), sum(rv)
+    tm = tl.TileModel.synthetic(no, nv, 0, 0, 5, ro, rv)
+    cut = lambda n, e: [min(e, n - k) for k in range(0, n, e)]
+    fine = tl.TileModel.synthetic(no, nv, 0, 0, 5, cut(no, 4), cut(nv, 8))
+    f, n = _work_tiling(tm, 2, precision=2)
+    assert f == _f_ref(fine) and n == len(fine.tasks())
+    f1, n1 = _work_tiling(tm, 1, precision=2)
+    assert n <= n1
+
+
+def test_orbital_tiling_keeps_spin_and_irrep_runs():
+    """Spin-labelled tiles: consecutive tiles of one spin and irrep merge
+    into one run before the cut, and pieces never straddle a change of spin
+    or irrep, so the task filters still apply piece by piece."""
+    ranges = [2, 2, 2, 2, 5, 6, 5, 6]       # holes a a b b, particles a a b b
+    spin = [1, 1, 2, 2, 1, 1, 2, 2]
+    evl = np.sort(np.random.default_rng(1).uniform(-2, 4, sum(ranges)))
+    tm = tl.TileModel(range=ranges, noab=4, evl=evl, spin=spin)
+    fine = tl.TileModel(range=[4, 4, 8, 3, 8, 3], noab=2, evl=evl, spin=[1, 2, 1, 1, 2, 2])
+    f, n = _work_tiling(tm, 2, precision=2)
+    assert f == _f_ref(fine) and n == len(fine.tasks())
+    irr = tl.TileModel(range=ranges, noab=4, evl=evl, spin=spin, sym=[0, 1, 0, 1, 0, 0, 0, 0])
+    fine_irr = tl.TileModel(range=[2, 2, 2, 2, 8, 3, 8, 3], noab=4, evl=evl, spin=[1, 1, 2, 2, 1, 1, 2, 2],
+                            sym=[0, 1, 0, 1, 0, 0, 0, 0])
+    f, n = _work_tiling(irr, 2, precision=2)
+    assert f == _f_ref(fine_irr) and n == len(fine_irr.tasks())
+
+
+def test_unknown_tiling_is_an_error():
+    L = _lib.lib()
+    tm = tl.TileModel.synthetic(8, 32, 8, 16, 1)
+    h = C.c_void_p()
+    assert L.tce_t_job_create_tiling(C.byref(tm.c()), 7, C.byref(h)) == -1
