@@ -148,6 +148,20 @@ __device__ __forceinline__ long first_item(unsigned b, unsigned G) {
   return (G % 8u) ? (long)b : (long)(b % 8u) * (G / 8u) + b / 8u;
 }
 
+// Item of workgroup b in round r of a persistent launch (round r covers
+// items r*G .. r*G + G-1): the XCD-aware position, mirrored in odd rounds
+// when `mirror` (launches over several jobs).  In a batch the items of one
+// job follow those of the previous (costlier, in cost order) job, so in each
+// round where the job changes the low positions draw the costlier items;
+// mirroring every other round keeps any workgroup from drawing them every
+// time.  One job's items cost the same, and there the unmirrored order keeps
+// each XCD on the same slice of consecutive rounds (half the L2 misses).
+// Uniform across the workgroup.
+__device__ __forceinline__ long round_item(long r, unsigned b, unsigned G, bool mirror) {
+  const long f = first_item(b, G);
+  return r * (long)G + ((mirror && (r & 1)) ? (long)G - 1 - f : f);
+}
+
 // Segment of global index g (uniform): the last with g0 <= g.
 template <class Args>
 __device__ __forceinline__ int find_segment(const SegmentT<Args> *s, int n, long g) {

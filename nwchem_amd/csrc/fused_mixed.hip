@@ -29,12 +29,12 @@
 // the lane layout and changing L (a -> b -> c) is one lane permutation.
 // Pass order: L=a {h=k, h=j, h=i} x {d2, d1}, L=b, L=c (18 passes).
 //
-// Operands stream through LDS in chunks of KC contraction values (16 f32 or
-// 32 bf16 = four 16-byte quads per row) with three buffers: direct
-// global->LDS loads (global_load_lds_dwordx4) fetch chunk j+2 while the
-// waves run chunk j from LDS; the workgroup waits for its own loads of chunk
-// j+1 and synchronises once per chunk.  A pass with a minus sign negates the
-// accumulators before and after it.
+// Operands stream through LDS in chunks of KC contraction values (8 f64, 16
+// f32 or 32 bf16 = four 16-byte quads per row) with three buffers: direct
+// global->LDS loads (global_load_lds_dwordx4) fetch chunk j+2 (j+3 in the
+// FP32 loop) while the waves run chunk j; the workgroup waits for its own
+// loads of chunk j+1 and synchronises once per chunk (pass_step).  A pass
+// with a minus sign negates the accumulators before and after it.
 //
 // LDS layout of one buffer part (hi or lo; quads, kq = quad 0..3 of a row):
 //   X  [4 s][8 L slots][4 r][4 kq ^ ((slot & 3) ^ (r & 2))]         512 quads
@@ -57,6 +57,8 @@ struct Prec<TCE_T_FP64> {
   static constexpr int KC = 8;     // contraction values per chunk
   static constexpr int NPART = 1;  // operand parts
   static constexpr int ES = 8;     // bytes per element
+  static constexpr int NBUF = 3;   // LDS chunk buffers
+  static constexpr bool SPLIT = false;  // chunk loop: see pass_step
   typedef double acc;
 };
 template <>
@@ -64,6 +66,8 @@ struct Prec<TCE_T_FP32> {
   static constexpr int KC = 16;
   static constexpr int NPART = 1;
   static constexpr int ES = 4;
+  static constexpr int NBUF = 3;
+  static constexpr bool SPLIT = true;  // measured 0.7% faster than the direct loop
   typedef float acc;
 };
 template <>
@@ -71,13 +75,14 @@ struct Prec<TCE_T_BF16X3> {
   static constexpr int KC = 32;
   static constexpr int NPART = 2;  // hi, lo
   static constexpr int ES = 2;
+  static constexpr int NBUF = 3;
+  static constexpr bool SPLIT = false;  // measured 20% slower split (64 fragment VGPRs)
   typedef float acc;
 };
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
 constexpr int X_QUADS = 512, Y_QUADS = 1024, PART_QUADS = X_QUADS + Y_QUADS;
-constexpr int NBUF = 3;
 
 // Epilogue staging (doubles), overlaying the chunk buffers after the last
 // chunk:
@@ -94,7 +99,8 @@ constexpr int buf_quads() {
 }
 template <int PREC>
 constexpr int lds_quads() {
-  return NBUF * buf_quads<PREC>() > EPI_QUADS ? NBUF * buf_quads<PREC>() : EPI_QUADS;
+  constexpr int nb = Prec<PREC>::NBUF;
+  return nb * buf_quads<PREC>() > EPI_QUADS ? nb * buf_quads<PREC>() : EPI_QUADS;
 }
 
 // The 18 passes: kind (0 = d2, 1 = d1), lone particle L (a b c), lone hole
@@ -251,74 +257,104 @@ __device__ __forceinline__ constexpr int acc_at(int s, int h, int r) {
                           : 16 * r + 4 * h + s);  // r = i, s = k, h = j
 }
 
-// The MFMAs of one chunk.  xa/yb: this lane's X / Y quad offsets for s = 0 /
-// h = 0 (s adds 128 quads, h adds 256).
-template <int PREC, int H>
-__device__ __forceinline__ void chunk_mfma(typename Prec<PREC>::acc (&a)[64], const f4 *cb, int xa,
-                                           int yb) {
-  if constexpr (PREC == TCE_T_FP64) {
-    const dv2 *cd = reinterpret_cast<const dv2 *>(cb);
-    dv2 xf[4], yf[4];
+// Operand fragments of one chunk in registers: X quads s = 0..3 and Y quads
+// h = 0..3 of this lane (bf16: hi and lo parts).
+template <int PREC>
+struct Frag;
+template <>
+struct Frag<TCE_T_FP64> {
+  dv2 x[4], y[4];
+};
+template <>
+struct Frag<TCE_T_FP32> {
+  f4 x[4], y[4];
+};
+template <>
+struct Frag<TCE_T_BF16X3> {
+  bf8 x[4], y[4], xl[4], yl[4];
+};
+
+// X quads S0..S1-1 (offset xa for s = 0, s adds 128 quads) and the Y quads
+// (offset yb for h = 0, h adds 256) of the chunk in buffer cb.
+template <int PREC, int S0, int S1>
+__device__ __forceinline__ void read_x(Frag<PREC> &F, const f4 *cb, int xa) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) xf[s] = cd[xa + 128 * s];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) yf[h] = cd[X_QUADS + yb + 256 * h];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          d4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                  a[acc_at<H>(s, h, 3)]};
-          c = __builtin_amdgcn_mfma_f64_16x16x4f64(xf[s][t], yf[h][t], c, 0, 0, 0);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
-        }
-  } else if constexpr (PREC == TCE_T_FP32) {
-    f4 xf[4], yf[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) xf[s] = cb[xa + 128 * s];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) yf[h] = cb[X_QUADS + yb + 256 * h];
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                  a[acc_at<H>(s, h, 3)]};
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(xf[s][t], yf[h][t], c, 0, 0, 0);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
-        }
-  } else {
-    const bf8 *cq = reinterpret_cast<const bf8 *>(cb);
-    bf8 xh[4], xl[4], yh[4], yl[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      xh[s] = cq[xa + 128 * s];
-      xl[s] = cq[PART_QUADS + xa + 128 * s];
+  for (int s = S0; s < S1; ++s) {
+    if constexpr (PREC == TCE_T_FP64) {
+      F.x[s] = reinterpret_cast<const dv2 *>(cb)[xa + 128 * s];
+    } else if constexpr (PREC == TCE_T_FP32) {
+      F.x[s] = cb[xa + 128 * s];
+    } else {
+      F.x[s] = reinterpret_cast<const bf8 *>(cb)[xa + 128 * s];
+      F.xl[s] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + xa + 128 * s];
     }
+  }
+}
+template <int PREC>
+__device__ __forceinline__ void read_y(Frag<PREC> &F, const f4 *cb, int yb) {
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      yh[h] = cq[X_QUADS + yb + 256 * h];
-      yl[h] = cq[PART_QUADS + X_QUADS + yb + 256 * h];
+  for (int h = 0; h < 4; ++h) {
+    if constexpr (PREC == TCE_T_FP64) {
+      F.y[h] = reinterpret_cast<const dv2 *>(cb)[X_QUADS + yb + 256 * h];
+    } else if constexpr (PREC == TCE_T_FP32) {
+      F.y[h] = cb[X_QUADS + yb + 256 * h];
+    } else {
+      F.y[h] = reinterpret_cast<const bf8 *>(cb)[X_QUADS + yb + 256 * h];
+      F.yl[h] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + X_QUADS + yb + 256 * h];
     }
+  }
+}
+
+// The MFMAs of X quads S0..S1-1 with every Y quad (lone hole H).
+template <int PREC, int H, int S0, int S1>
+__device__ __forceinline__ void mma_x(typename Prec<PREC>::acc (&a)[64], const Frag<PREC> &F) {
+  constexpr int NT = PREC == TCE_T_FP64 ? 2 : (PREC == TCE_T_FP32 ? 4 : 1);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int s = S0; s < S1; ++s)
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                a[acc_at<H>(s, h, 3)]};
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[s], yh[h], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[s], yl[h], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[s], yh[h], c, 0, 0, 0);
+        if constexpr (PREC == TCE_T_FP64) {
+          d4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                  a[acc_at<H>(s, h, 3)]};
+          c = __builtin_amdgcn_mfma_f64_16x16x4f64(F.x[s][t], F.y[h][t], c, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+        } else if constexpr (PREC == TCE_T_FP32) {
+          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                  a[acc_at<H>(s, h, 3)]};
+          c = __builtin_amdgcn_mfma_f32_16x16x4f32(F.x[s][t], F.y[h][t], c, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+        } else {
+          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                  a[acc_at<H>(s, h, 3)]};
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.xl[s], F.y[h], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.x[s], F.yl[h], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.x[s], F.y[h], c, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
+        }
       }
-  }
+}
+
+// This lane's fragment offsets in pass P: X quad (s = 0), Y quad (h = 0).
+// X row (lane & 15) = 4L' + r (L' = hi, r = lo), quad kq.  The f64 MFMA
+// writes row i to lane group i % 4, register i / 4 (the f32 and bf16 ones
+// to group i / 4, register i % 4), so for FP64 the rows are ordered 4r + L'
+// instead: either way the result lands at lane 16L' + col, component r.
+// Y col (lane & 15) = 4p1' + p2' (p1' = hi, p2' = lo).
+template <int PREC, int P>
+__device__ __forceinline__ int2 frag_offsets(int wave, int lane) {
+  constexpr PassInfo pi = PASS[P];
+  constexpr int Lp = pi.L, p1 = Lp == 0 ? 1 : 0, p2 = Lp == 2 ? 1 : 2;
+  const int kq = lane >> 4, hi = (lane & 15) >> 2, lo = lane & 3;
+  const int xL = PREC == TCE_T_FP64 ? lo : hi, xr = PREC == TCE_T_FP64 ? hi : lo;
+  const int slot = 4 * cube(wave, Lp) + xL;
+  const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
+  return make_int2((slot * 4 + xr) * 4 + (kq ^ x_swz(slot, xr)),
+                   (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s)));
 }
 
 __device__ __forceinline__ void permute_lanes(float (&a)[64], int src) {
@@ -342,19 +378,51 @@ __device__ __forceinline__ constexpr int chunks_of(const MixedArgsG &M, int p) {
   return p >= 18 ? 0 : (PASS[p].kind ? M.Mk : M.Ek) / Prec<PREC>::KC;
 }
 
-// Pass P: layout change, fragment offsets, then its chunks.  Each chunk
-// first issues the loads of chunk j+2 (of this pass or of the next two),
-// then runs its MFMAs from buffer j % 3, waits for this thread's loads of
-// chunk j+1 and synchronises.
+// Load global chunk `c2` of the pass sequence that starts at pass P (chunk
+// c2 of pass P, or of one of the next D passes when P has fewer chunks;
+// every pass has at least one) into buffer nb.
+template <int PREC, int P, int D>
+__device__ __forceinline__ void load_ahead(const Src<PREC> &s0, const MixedArgsG &M,
+                                           const ItemGeo &G, int wave, int lane, int c2,
+                                           unsigned lds_base, int nb) {
+  using Pr = Prec<PREC>;
+  const int n = chunks_of<PREC>(M, P);
+  if (c2 < n) {
+    load_chunk<PREC>(s0, (long)c2 * Pr::KC * Pr::ES, lds_base, nb, wave);
+  } else if constexpr (D > 0 && P + 1 < 18) {
+    load_ahead<PREC, P + 1, D - 1>(pass_src<PREC, P + 1>(M, G, wave, lane), M, G, wave, lane,
+                                   c2 - n, lds_base, nb);
+  }
+}
+
+// s_waitcnt lgkmcnt(0) with the other counters left alone (gfx9 encoding).
+__device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// Pass P: layout change, then its chunks, with three LDS chunk buffers.
+//
+// SPLIT (FP32): on entry F holds the fragments of the pass's
+// first chunk j.  Per chunk: the MFMAs of X quads 0-1; this thread's loads
+// of chunk j+1 and its LDS reads of chunk j complete, then the barrier that
+// publishes chunk j+1 and frees buffer j % 3, which receives the loads of
+// chunk j+3; the fragments of X quads 0-1 of chunk j+1 are read while the
+// MFMAs of X quads 2-3 of chunk j run, then the rest of chunk j+1's, so the
+// matrix core never waits for an LDS read after the barrier.
+//
+// Otherwise (FP64, whose 64 double accumulators leave no room to keep a
+// chunk's fragments live across the barrier, and split-BF16, whose hi and lo
+// fragments take 64 registers): per chunk the loads of chunk
+// j+2 go to the buffer freed by the last barrier, the fragments of chunk j
+// are read and its MFMAs run, then this thread's loads of chunk j+1
+// complete and the barrier publishes it.
 template <int PREC, int P>
 __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
                                           unsigned lds_base,
                                           const MixedArgsG &M, const ItemGeo &G, int wave,
-                                          int lane, int &j, int &buf, int total) {
+                                          int lane, int &j, int &buf, int total,
+                                          Frag<PREC> &F) {
   using Pr = Prec<PREC>;
   constexpr PassInfo pi = PASS[P];
-  constexpr int Lp = pi.L, H = pi.h;
-  constexpr int p1 = Lp == 0 ? 1 : 0, p2 = Lp == 2 ? 1 : 2;
+  constexpr int H = pi.h;
   constexpr int NLOAD = 3 * Pr::NPART;  // load instructions per chunk
   const int kq = lane >> 4, hi = (lane & 15) >> 2, lo = lane & 3;
   if constexpr (P == 6) {  // L: a -> b; lane (4b'+c') + 16a' -> (4a'+c') + 16b'
@@ -363,44 +431,53 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
     permute_lanes(acc, 4 * hi + kq + 16 * lo);
   }
   if constexpr (pi.minus) negate(acc);
-  const int nch0 = chunks_of<PREC>(M, P), nch1 = chunks_of<PREC>(M, P + 1);
-  // X fragment: row (lane & 15) = 4L' + r (L' = hi, r = lo), quad kq.  The
-  // f64 MFMA writes row i to lane group i % 4, register i / 4 (the f32 and
-  // bf16 ones to group i / 4, register i % 4), so for FP64 the rows are
-  // ordered 4r + L' instead: either way the result lands at lane 16L' + col,
-  // component r.
-  const int xL = PREC == TCE_T_FP64 ? lo : hi, xr = PREC == TCE_T_FP64 ? hi : lo;
-  const int slot = 4 * cube(wave, Lp) + xL;
-  const int xa = (slot * 4 + xr) * 4 + (kq ^ x_swz(slot, xr));
-  // Y fragment: col (lane & 15) = 4p1' + p2' (p1' = hi, p2' = lo)
-  const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
-  const int yb = (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s));
-  // load sources of this pass; those of the next two passes are formed only
-  // in the branches that need them (the last two chunks), which keeps the
+  const int nch0 = chunks_of<PREC>(M, P);
+  // load sources of this pass; those of the next passes are formed only in
+  // the branches that need them (the pass's last chunks), which keeps the
   // chunk loop's live registers to the accumulators, fragments and s0
   const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
-  for (int c = 0; c < nch0; ++c, ++j) {
-    const bool more = j + 2 < total;
-    if (more) {
-      const int c2 = c + 2, nb = buf == 0 ? 2 : buf - 1;
-      if (c2 < nch0) {
-        load_chunk<PREC>(s0, (long)c2 * Pr::KC * Pr::ES, lds_base, nb, wave);
-      } else if (c2 - nch0 < nch1) {
-        load_chunk<PREC>(pass_src<PREC, (P + 1 < 18 ? P + 1 : 17)>(M, G, wave, lane),
-                         (long)(c2 - nch0) * Pr::KC * Pr::ES, lds_base, nb, wave);
-      } else {  // pass P+1 has a single chunk: chunk 0 of pass P+2
-        load_chunk<PREC>(pass_src<PREC, (P + 2 < 18 ? P + 2 : 17)>(M, G, wave, lane),
-                         (long)(c2 - nch0 - nch1) * Pr::KC * Pr::ES, lds_base, nb, wave);
+  if constexpr (Pr::SPLIT) {
+    for (int c = 0; c < nch0; ++c, ++j) {
+      const bool next = j + 1 < total;
+      mma_x<PREC, H, 0, 2>(acc, F);
+      if (j + 2 < total)
+        wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
+      else
+        wait_vm<0>();
+      wait_lgkm0();
+      block_barrier();
+      if (j + 3 < total) load_ahead<PREC, P, 3>(s0, M, G, wave, lane, c + 3, lds_base, buf);
+      const int nbuf = buf == 2 ? 0 : buf + 1;
+      const f4 *cbn = lds + nbuf * buf_quads<PREC>();
+      // chunk j+1: this pass or the first chunk of pass P+1
+      const int2 o = c + 1 < nch0 ? frag_offsets<PREC, P>(wave, lane)
+                                  : frag_offsets<PREC, (P + 1 < 18 ? P + 1 : 17)>(wave, lane);
+      if (next) read_x<PREC, 0, 2>(F, cbn, o.x);
+      mma_x<PREC, H, 2, 4>(acc, F);
+      if (next) {
+        read_x<PREC, 2, 4>(F, cbn, o.x);
+        read_y<PREC>(F, cbn, o.y);
       }
+      buf = nbuf;
     }
-    chunk_mfma<PREC, H>(acc, lds + buf * buf_quads<PREC>(), xa, yb);
-    // own loads of chunk j+1 have landed (only chunk j+2's may be pending)
-    if (more)
-      wait_vm<NLOAD>();
-    else
-      wait_vm<0>();
-    block_barrier();
-    buf = buf == 2 ? 0 : buf + 1;
+  } else {
+    const int2 o = frag_offsets<PREC, P>(wave, lane);
+    for (int c = 0; c < nch0; ++c, ++j) {
+      const bool more = j + 2 < total;
+      if (more)  // buffer (j + 2) % 3, free since the last barrier
+        load_ahead<PREC, P, 2>(s0, M, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
+      Frag<PREC> f;
+      const f4 *cb = lds + buf * buf_quads<PREC>();
+      read_x<PREC, 0, 4>(f, cb, o.x);
+      read_y<PREC>(f, cb, o.y);
+      mma_x<PREC, H, 0, 4>(acc, f);
+      if (more)
+        wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
+      else
+        wait_vm<0>();
+      block_barrier();
+      buf = buf == 2 ? 0 : buf + 1;
+    }
   }
   if constexpr (pi.minus) negate(acc);
 }
@@ -409,9 +486,11 @@ template <int PREC, int P>
 __device__ __forceinline__ void all_passes(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
                                            unsigned lds_base,
                                            const MixedArgsG &M, const ItemGeo &G, int wave,
-                                           int lane, int &j, int &buf, int total) {
-  pass_step<PREC, P>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
-  if constexpr (P + 1 < 18) all_passes<PREC, P + 1>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
+                                           int lane, int &j, int &buf, int total,
+                                           Frag<PREC> &F) {
+  pass_step<PREC, P>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
+  if constexpr (P + 1 < 18)
+    all_passes<PREC, P + 1>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
 }
 
 // Stage the item's singles operands into LDS (FP64, as fused_wg.hip).
@@ -514,7 +593,9 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
   using Pr = Prec<PREC>;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier is reached by all 8 waves).
-  for (long g = first_item(blockIdx.x, gridDim.x); g < nglobal; g += gridDim.x) {
+  for (long r = 0; r * (long)gridDim.x < nglobal; ++r) {
+    const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
+    if (g >= nglobal) continue;  // last round: uniform per workgroup
     const MixedSegment &sg = segs[find_segment(segs, nseg, g)];
     const MixedArgsG M = load_args(sg.args);
     const FusedArgsG &A = M.f;
@@ -541,20 +622,29 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 #pragma unroll
     for (int r = 0; r < 64; ++r) acc[r] = 0;
 
-    {  // chunks 0 and 1 (pass 0, or passes 0 and 1 when pass 0 has one chunk)
+    Frag<PREC> F;
+    {  // chunks 0 .. 2 (SPLIT) or 0 .. 1 of the pass sequence into buffers
+       // 0 .. 2 / 0 .. 1; SPLIT: then the fragments of chunk 0
+      constexpr int PRO = Pr::SPLIT ? 3 : 2;
       const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
-      const Src<PREC> s1 = pass_src<PREC, 1>(M, G, wave, lane);
-      const int n0 = chunks_of<PREC>(M, 0);
-      load_chunk<PREC>(s0, 0, lds_base, 0, wave);
-      if (n0 > 1)
-        load_chunk<PREC>(s0, (long)Pr::KC * Pr::ES, lds_base, 1, wave);
+#pragma unroll
+      for (int k = 0; k < PRO; ++k)
+        if (k < total) load_ahead<PREC, 0, PRO>(s0, M, G, wave, lane, k, lds_base, k);
+      // chunk 0 has landed once at most the loads of the later ones are
+      // pending (fewer chunks in the item: wait for all)
+      if (total >= PRO)
+        wait_vm<3 * Pr::NPART * (PRO - 1)>();
       else
-        load_chunk<PREC>(s1, 0, lds_base, 1, wave);
-      wait_vm<3 * Pr::NPART>();
+        wait_vm<0>();
       block_barrier();
+      if constexpr (Pr::SPLIT) {
+        const int2 o = frag_offsets<PREC, 0>(wave, lane);
+        read_x<PREC, 0, 4>(F, lds, o.x);
+        read_y<PREC>(F, lds, o.y);
+      }
     }
     int j = 0, buf = 0;
-    all_passes<PREC, 0>(acc, lds, lds_base, M, G, wave, lane, j, buf, total);
+    all_passes<PREC, 0>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
 
     // Epilogue: the chunk buffers are free (the last chunk ended with a
     // barrier and no load is outstanding).

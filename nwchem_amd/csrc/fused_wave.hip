@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(64, 2) fused_triples(const Segment *segs, int 
   // Persistent loop, static round-robin over the launch's items: the item
   // index is wave-uniform by construction and every wave leaves after at
   // most ceil(total / gridDim.x) items.
-  for (long g = first_item(blockIdx.x, gridDim.x); g < total; g += gridDim.x) {
+  for (long r = 0; r * (long)gridDim.x < total; ++r) {
+    const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
+    if (g >= total) continue;  // last round: uniform per workgroup
     const Segment &sg = segs[find_segment(segs, nseg, g)];
     const FusedArgsG A = load_args(sg.args);
     const long item = sg.item0 + (g - sg.g0);

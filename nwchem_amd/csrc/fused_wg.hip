@@ -300,7 +300,9 @@ __global__ void __launch_bounds__(512) fused_triples_wg(const Segment *segs, int
   const int ca = (wave >> 2) & 1, cb = (wave >> 1) & 1, cc = wave & 1;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier below is reached by all 8 waves).
-  for (long g = first_item(blockIdx.x, gridDim.x); g < total; g += gridDim.x) {
+  for (long r = 0; r * (long)gridDim.x < total; ++r) {
+    const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
+    if (g >= total) continue;  // last round: uniform per workgroup
     const Segment &sg = segs[find_segment(segs, nseg, g)];
     const FusedArgsG A = load_args(sg.args);
     const long item = sg.item0 + (g - sg.g0);
