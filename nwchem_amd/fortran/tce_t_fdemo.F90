@@ -14,7 +14,9 @@ program tce_t_fdemo
   real(c_double), allocatable :: t1(:), t2(:), v2(:)
   type(tce_t_tiles) :: tiles
   type(c_ptr) :: job
-  real(c_double) :: e1, e2
+  real(c_double) :: e1, e2, et
+  real(c_double), allocatable :: table(:)
+  integer(c_long) :: begin
   integer(c_int) :: rc
   character(len=32) :: arg
   integer(c_long_long), parameter :: seed = 1234_c_long_long
@@ -59,5 +61,15 @@ program tce_t_fdemo
   if (rc /= TCE_T_OK) stop 6
   write(*, '(A,I0,A,I0,A,ES24.16,A,ES24.16)') 'tasks=', tce_t_job_num_tasks(job), &
       ' blocks(t2)=', l2, ' E[T]=', e1, ' E(T)=', e2
+  ! ccsd_t_restart.F pattern: one outer virtual tile per call, the state
+  ! (begin, table) carried between calls as the rtdb would keep it
+  allocate(table(nvab))
+  table = 0.0d0
+  begin = 1
+  do while (begin <= nvab)
+    rc = tce_t_job_run_restart(job, TCE_T_FP64, begin, table, 1_c_long, et)
+    if (rc /= TCE_T_OK) stop 7
+  end do
+  write(*, '(A,I0,A,ES24.16)') 'restart blocks=', nvab, ' E(T)=', et
   rc = tce_t_job_destroy(job)
 end program tce_t_fdemo

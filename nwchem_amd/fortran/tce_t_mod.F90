@@ -16,6 +16,8 @@ module tce_t_mi355x
 
   integer(c_int), parameter :: TCE_T_OK = 0
   integer(c_int), parameter :: TCE_T_FP64 = 0
+  integer(c_int), parameter :: TCE_T_TILING_REFERENCE = 0, TCE_T_TILING_SUBTILE = 1, &
+                               TCE_T_TILING_ORBITAL = 2
 
   ! Mirror of tce_t_tiles: the tce.fh / MA arrays the (T) path reads.
   type, bind(C) :: tce_t_tiles
@@ -63,6 +65,29 @@ module tce_t_mi355x
       type(c_ptr), value :: job
       integer(c_int), value :: precision
       real(c_double), intent(out) :: energy1, energy2
+    end function
+
+    ! ccsd_t_restart.F: blocks of one outer virtual tile; begin and table
+    ! are what the caller keeps in the rtdb (tce:ccsd_t_restart_begin,
+    ! tce:restart_triples_table); max_tiles <= 0 runs every remaining block
+    integer(c_int) function tce_t_job_run_restart(job, precision, begin, table, max_tiles, &
+        t_energy) bind(C, name="tce_t_job_run_restart")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      integer(c_int), value :: precision
+      integer(c_long), intent(inout) :: begin
+      real(c_double), intent(inout) :: table(*)
+      integer(c_long), value :: max_tiles
+      real(c_double), intent(out) :: t_energy
+    end function
+
+    ! kernel-task tiling: TCE_T_TILING_REFERENCE, _SUBTILE (default), _ORBITAL
+    integer(c_int) function tce_t_job_create_tiling(tiles, tiling, job) &
+        bind(C, name="tce_t_job_create_tiling")
+      import :: c_int, c_ptr, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_int), value :: tiling
+      type(c_ptr), intent(out) :: job
     end function
 
     integer(c_int) function tce_t_job_task_energies(job, out, ntasks) &

@@ -195,6 +195,10 @@ def test_fortran_caller_matches_oracle():
     tm, b, ot = system(8, 32, 8, 16, 1234)
     e1, e2, _ = oracle_energies(ot, b)
     assert abs(float(m.group(1)) - e1) <= tol(e1) and abs(float(m.group(2)) - e2) <= tol(e2)
+    # the ccsd_t_restart.F call pattern from Fortran: one outer tile per call
+    r = re.search(r"restart blocks=\s*(\d+)\s+E\(T\)=\s*(\S+)", out)
+    assert r, out
+    assert int(r.group(1)) == tm.nvab and abs(float(r.group(2)) - e2) <= tol(e2)
 
 
 # Reduced-precision modes: the d1/d2 products in FP32, or split-BF16 with
