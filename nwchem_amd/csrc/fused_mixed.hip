@@ -410,10 +410,10 @@ __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F
 //
 // Otherwise (FP64, whose 64 double accumulators leave no room to keep a
 // chunk's fragments live across the barrier, and split-BF16, whose hi and lo
-// fragments take 64 registers): per chunk the loads of chunk
-// j+2 go to the buffer freed by the last barrier, the fragments of chunk j
-// are read and its MFMAs run, then this thread's loads of chunk j+1
-// complete and the barrier publishes it.
+// fragments take 64 registers): per chunk the fragments of chunk j are read
+// and its MFMAs run, with the loads of chunk j+2 (into the buffer freed by
+// the last barrier) issued halfway through them; then this thread's loads
+// of chunk j+1 complete and the barrier publishes it.
 template <int PREC, int P>
 __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
                                           unsigned lds_base,
@@ -464,13 +464,17 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
     const int2 o = frag_offsets<PREC, P>(wave, lane);
     for (int c = 0; c < nch0; ++c, ++j) {
       const bool more = j + 2 < total;
-      if (more)  // buffer (j + 2) % 3, free since the last barrier
-        load_ahead<PREC, P, 2>(s0, M, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
       Frag<PREC> f;
       const f4 *cb = lds + buf * buf_quads<PREC>();
       read_x<PREC, 0, 4>(f, cb, o.x);
       read_y<PREC>(f, cb, o.y);
-      mma_x<PREC, H, 0, 4>(acc, f);
+      // the loads of chunk j+2 (buffer (j + 2) % 3, free since the last
+      // barrier) go between the two halves of the MFMAs, so their address
+      // arithmetic issues while the matrix core works (measured 0.7% faster
+      // than before the MFMAs, 3% faster than after them)
+      mma_x<PREC, H, 0, 2>(acc, f);
+      if (more) load_ahead<PREC, P, 2>(s0, M, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
+      mma_x<PREC, H, 2, 4>(acc, f);
       if (more)
         wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
       else
