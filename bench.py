@@ -145,6 +145,42 @@ def ragged(n: int, ts: int) -> list:
     return r
 
 
+def config1(threads: int) -> dict | None:
+    """Config 1 (no=8 in one tile, nv=32 in tiles of 16, 4 tasks) whole: the
+    reference CPU path (its compiled Fortran kernels under the ccsd_t_gpu.F
+    task loop at usedevice = 0) on one core and on `threads` OpenMP threads,
+    and the GPU job on the same inputs."""
+    import oracle
+    from nwchem_amd import TriplesJob, tiles as tl
+
+    tm = tl.TileModel.synthetic(8, 32, 8, 16, 1234)
+    b = tl.synth_blocks(tm, 1234)
+    ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
+    out = {"workload": "no=8 (1 tile), nv=32 (2 tiles of 16), 4 tasks, seed 1234", "flops": f_ref(tm)}
+    try:
+        for name, omp in (("cpu_1core", False), ("cpu_all_cores", True)):
+            be = oracle.ref_backend(omp=omp)
+            t0 = time.perf_counter()
+            e = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be)
+            out[name] = {"seconds": time.perf_counter() - t0, "E_bt": e[0], "E_pt": e[1],
+                         "build": "-O3 -fopenmp" if omp else "-O2, no OpenMP, no FP contraction"}
+    except (FileNotFoundError, OSError):
+        return None
+    out["cpu_all_cores"]["threads"] = int(os.environ.get("OMP_NUM_THREADS", threads))
+    with TriplesJob(tm) as j:
+        j.upload(b)
+        j.run()  # first launch of the kernel (code load) outside the timing
+        t0 = time.perf_counter()
+        e = j.run()
+        out["gpu"] = {"seconds": time.perf_counter() - t0, "kernel_ms": j.last_kernel_ms,
+                      "E_bt": e[0], "E_pt": e[1]}
+    out["dE_gpu_vs_cpu"] = [e[0] - out["cpu_1core"]["E_bt"], e[1] - out["cpu_1core"]["E_pt"]]
+    print(f"[bench] config 1: reference CPU {out['cpu_1core']['seconds']:.2f} s on 1 core, "
+          f"{out['cpu_all_cores']['seconds']:.2f} s on {out['cpu_all_cores']['threads']} threads; "
+          f"GPU {out['gpu']['seconds'] * 1e3:.2f} ms; dE {out['dE_gpu_vs_cpu']}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
     """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256}, occupied tiles
     6-20, virtual tiles 16-48, ragged last tiles) in one persistent launch per
@@ -400,10 +436,12 @@ def main() -> int:
     serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms) \
         if args.serve else None
     cpu = None
+    cfg1 = None
     if args.cpu_baseline:
         print(f"[bench] cpu baseline on {args.cpu_threads} threads", file=sys.stderr, flush=True)
         del blocks
         cpu = cpu_baseline(tm, args.cpu_threads)
+        cfg1 = config1(args.cpu_threads)
         if cpu:
             print(f"[bench] cpu baseline {cpu['value']*1e3:.2f} GFLOP/s ({cpu['seconds']:.1f}s)",
                   file=sys.stderr, flush=True)
@@ -432,6 +470,7 @@ def main() -> int:
         "roofline": roofline,
         "reference_tiling": reftile,
         "cpu_baseline": cpu,
+        "config1": cfg1,
         "precision_modes": modes or None,
         "batch": batch,
         "serving": serve,

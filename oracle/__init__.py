@@ -140,10 +140,10 @@ class Tiles:
         return int(lib().ora_count_tasks(C.byref(self.c)))
 
 
-def ref_backend() -> OraBackend:
+def ref_backend(omp: bool = False) -> OraBackend:
     """Kernel back-end made of the reference's compiled Fortran kernels
-    (CPU argument order, usedevice = 0)."""
-    R = ref_lib()
+    (CPU argument order, usedevice = 0); omp: the -O3 -fopenmp build."""
+    R = ref_lib(omp)
     be = OraBackend()
     be.usedevice = 0
     for n in range(9):
