@@ -22,8 +22,9 @@
 //              reduction; its energies, like a lone run's, depend only on its
 //              own work items, never on what shared the slices.
 //
-// FP64 requests run on fused_triples_wg, reduced precision on
-// fused_triples_mixed (both with 8-wide particle boxes).
+// Every request runs on fused_triples_mixed (8-wide particle boxes; FP64 by
+// its FP64 policy).  A system without allowed tasks has no items: its
+// requests finish at their first slice without a launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -145,9 +146,10 @@ bool make_room(Server &S, long need, int keep) {
 
 // One slice of the requests of one precision: returns the launch time (ms).
 int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float &ms) {
-  const bool mixed = precision != TCE_T_FP64;
-  const size_t asz = ((mixed ? sizeof(MixedArgs) : sizeof(FusedArgs)) + 15) / 16 * 16;
-  const size_t ssz = mixed ? sizeof(MixedSegment) : sizeof(Segment);
+  // every precision on the 16x16 kernel (FP64: its FP64 policy, as a job
+  // with 8-wide boxes runs it)
+  const size_t asz = (sizeof(MixedArgs) + 15) / 16 * 16;
+  const size_t ssz = sizeof(MixedSegment);
   const size_t n = rs.size();
   char *h = (char *)S.tab_host;
   char *d = (char *)S.tab_dev;
@@ -155,39 +157,38 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
   // items per CU for each request)
   double total_cost = 0;
   long g = 0;
+  size_t ns = 0;  // segments: requests with items left (a system without
+                  // allowed tasks has none and finishes without a launch)
   for (size_t k = 0; k < n; ++k) {
     Request &r = *rs[k];
     const Job &J = job_of(S, r.sys);
+    const long left = r.nitems - r.cursor;
+    if (left <= 0) continue;
     const double c = item_cost(J, precision);
     long take;
     if (S.ms_per_cost[precision] > 0)
       take = (long)(S.slice_ms / S.ms_per_cost[precision] / (double)n / c);
     else
       take = 2L * S.cus;
-    take = std::max(1L, std::min(take, r.nitems - r.cursor));
-    if (mixed) {
-      MixedArgs M = job_margs(J, precision);
-      M.f.partials = r.dpart;
-      std::memcpy(h + k * asz, &M, sizeof M);
-      MixedSegment sg{(const MixedArgs *)(d + k * asz), r.cursor, r.cursor + take, g};
-      std::memcpy(h + n * asz + k * ssz, &sg, sizeof sg);
-    } else {
-      FusedArgs A = job_args(J, true);
-      A.partials = r.dpart;
-      std::memcpy(h + k * asz, &A, sizeof A);
-      Segment sg{(const FusedArgs *)(d + k * asz), r.cursor, r.cursor + take, g};
-      std::memcpy(h + n * asz + k * ssz, &sg, sizeof sg);
-    }
+    take = std::max(1L, std::min(take, left));
+    MixedArgs M = job_margs(J, precision);
+    M.f.partials = r.dpart;
+    std::memcpy(h + ns * asz, &M, sizeof M);
+    MixedSegment sg{(const MixedArgs *)(d + ns * asz), r.cursor, r.cursor + take, g};
+    std::memcpy(h + n * asz + ns * ssz, &sg, sizeof sg);
+    ++ns;
     r.cursor += take;
     g += take;
     total_cost += c * (double)take;
   }
-  HIPCHK(hipMemcpyAsync(d, h, n * (asz + ssz), hipMemcpyHostToDevice, S.stream), "slice table");
+  ms = 0.f;
+  if (ns == 0) return TCE_T_OK;
+  // the segments follow the argument slots of all n requests (n * asz)
+  HIPCHK(hipMemcpyAsync(d, h, n * asz + ns * ssz, hipMemcpyHostToDevice, S.stream), "slice table");
   HIPCHK(hipEventRecord(S.ev0, S.stream), "event");
   const int grid = (int)std::min<long>(S.cus, g);
-  const int rc = mixed ? launch_fused_mixed((const MixedSegment *)(d + n * asz), (int)n, g, precision,
-                                            grid, S.stream)
-                       : launch_fused_wg((const Segment *)(d + n * asz), (int)n, g, grid, S.stream);
+  const int rc =
+      launch_fused_mixed((const MixedSegment *)(d + n * asz), (int)ns, g, precision, grid, S.stream);
   if (rc != 0) {
     set_error("slice launch failed");
     return TCE_T_ERR_HIP;

@@ -505,3 +505,47 @@ def test_restart_blocks_match_task_sums(precision):
     assert np.all(np.abs(st["table"] - per_tile) <= rel * scale + 1e-10), (st["table"], per_tile)
     assert abs(st["t_energy"] - e2) <= max(tol(e2), rel * abs(e2))
     assert abs(st["t_energy"] - full[1]) <= rel * abs(full[1]) + 1e-12
+
+
+# ---- a system without allowed tasks (every block breaks spin conservation)
+def _empty_system():
+    evl = np.sort(np.random.default_rng(1).uniform(-2.0, 4.0, 12))
+    tm = tl.TileModel(range=[2, 2, 4, 4], noab=2, evl=evl, spin=[1, 1, 2, 2])
+    return tm, tl.synth_blocks(tm, 3)
+
+
+def test_empty_system_job_batch_and_server():
+    """Holes all alpha, particles all beta: no block is stored and no task
+    is allowed.  A job gives zero energies and an empty task list, a batch
+    that holds it gives zeros for it, and a server request for it finishes
+    without a launch; the other jobs are unaffected."""
+    from nwchem_amd import TILING_ORBITAL, TriplesBatch, TriplesServer
+    tm, b = _empty_system()
+    assert len(tm.tasks()) == 0
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        assert job.run() == (0.0, 0.0)
+        assert job.task_energies().shape == (0, 2)
+        assert job.run(2) == (0.0, 0.0)
+        assert job.work() == (0.0, 0)
+    tm2, b2, ot2 = system(8, 32, 8, 16, 1234)
+    e1, e2, _ = oracle_energies(ot2, b2)
+    jobs = [TriplesJob(tm), TriplesJob(tm2, tiling=TILING_ORBITAL)]
+    try:
+        jobs[0].upload(b)
+        jobs[1].upload(b2)
+        with TriplesBatch(jobs) as batch:
+            for p in (0, 2):
+                e = batch.run(p)
+                assert tuple(e[0]) == (0.0, 0.0)
+                rel = 1e-12 if p == 0 else MIXED_REL[2]
+                assert abs(e[1, 0] - e1) <= rel * abs(e1) + 1e-10
+    finally:
+        for j in jobs:
+            j.close()
+    with TriplesServer(slice_ms=2.0, max_active=4, reserve_bytes=0) as srv:
+        s0, s1 = srv.add_system(tm, b), srv.add_system(tm2, b2)
+        ts = [srv.submit(s0, 0), srv.submit(s1, 0), srv.submit(s0, 2)]
+        r = [srv.wait(t) for t in ts]
+    assert (r[0]["E_bt"], r[0]["E_pt"]) == (0.0, 0.0) and (r[2]["E_bt"], r[2]["E_pt"]) == (0.0, 0.0)
+    assert abs(r[1]["E_bt"] - e1) <= tol(e1) and abs(r[1]["E_pt"] - e2) <= tol(e2)

@@ -273,3 +273,14 @@ def test_unknown_tiling_is_an_error():
     tm = tl.TileModel.synthetic(8, 32, 8, 16, 1)
     h = C.c_void_p()
     assert L.tce_t_job_create_tiling(C.byref(tm.c()), 7, C.byref(h)) == -1
+
+
+def test_job_plan_without_allowed_tasks():
+    """Holes all alpha, particles all beta: empty block tables, no task, no
+    work item, in every tiling."""
+    evl = np.sort(np.random.default_rng(1).uniform(-2.0, 4.0, 12))
+    tm = tl.TileModel(range=[2, 2, 4, 4], noab=2, evl=evl, spin=[1, 1, 2, 2])
+    assert tm.sizes() == {"t1": 0, "t2": 0, "v2": 0} and len(tm.tasks()) == 0
+    for tiling in (0, 1, 2):
+        for precision in (0, 2):
+            assert _work_tiling(tm, tiling, precision) == (0.0, 0)
