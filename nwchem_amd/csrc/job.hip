@@ -297,6 +297,46 @@ static long job_tasks(const Job &J, int pb, std::vector<TaskDev> &tasks, std::ve
   return build_tasks(J.T, pb, J.tiling == TCE_T_TILING_SUBTILE, tasks, bound, nullptr);
 }
 
+// Work items of a job's kernel tasks with pb-wide particle boxes (counted,
+// not stored).
+static long count_items(const Job &J, int pb) {
+  const Tiles R = J.tiling == TCE_T_TILING_ORBITAL ? orbital_tiles(J.T, pb) : J.T;
+  const bool sub = J.tiling == TCE_T_TILING_SUBTILE;
+  const long noab = R.noab, nvab = R.nvab;
+  auto nb = [](long r, long e) { return (r + e - 1) / e; };
+  // pieces x (ways to choose ordered sub-tiles) per reference task: for
+  // sub-tiles within equal tiles only the canonical (non-decreasing) choices
+  auto choose = [](long n, int k) {  // multisets of size k from n pieces
+    return k == 1 ? n : (k == 2 ? n * (n + 1) / 2 : n * (n + 1) * (n + 2) / 6);
+  };
+  long items = 0;
+  for (long p4b = noab + 1; p4b <= noab + nvab; ++p4b)
+    for (long p5b = p4b; p5b <= noab + nvab; ++p5b)
+      for (long p6b = p5b; p6b <= noab + nvab; ++p6b)
+        for (long h1b = 1; h1b <= noab; ++h1b)
+          for (long h2b = h1b; h2b <= noab; ++h2b)
+            for (long h3b = h2b; h3b <= noab; ++h3b) {
+              if (!task_allowed(R, p4b, p5b, p6b, h1b, h2b, h3b)) continue;
+              const long a = nb(R.rng(p4b), pb), b = nb(R.rng(p5b), pb), c = nb(R.rng(p6b), pb);
+              const long i = nb(R.rng(h1b), 4), j = nb(R.rng(h2b), 4), k = nb(R.rng(h3b), 4);
+              if (!sub) {
+                items += a * b * c * i * j * k;
+                continue;
+              }
+              long pp, hh;
+              if (p4b == p5b && p5b == p6b) pp = choose(a, 3);
+              else if (p4b == p5b) pp = choose(a, 2) * c;
+              else if (p5b == p6b) pp = a * choose(b, 2);
+              else pp = a * b * c;
+              if (h1b == h2b && h2b == h3b) hh = choose(i, 3);
+              else if (h1b == h2b) hh = choose(i, 2) * k;
+              else if (h2b == h3b) hh = i * choose(j, 2);
+              else hh = i * j * k;
+              items += pp * hh;
+            }
+  return items;
+}
+
 extern "C" int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **out) {
   int tiling = TCE_T_TILING_SUBTILE;
   if (const char *e = getenv("TCE_T_SUBTILE")) tiling = atoi(e);
@@ -322,18 +362,15 @@ extern "C" int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce
   const Tiles &T = J->T;
   const long noab = T.noab, nvab = T.nvab;
   // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
-  // shared through LDS) unless rounding the particle tiles up to multiples
-  // of 8 pads the work by more than 25% over 4-wide boxes.  TCE_T_KERNEL =
-  // wave | wg | wg16 forces fused_triples, fused_triples_wg or the FP64
-  // policy of fused_triples_mixed.
+  // shared through LDS, the 16x16x4 FP64 kernel) unless their padded work
+  // exceeds that of 4-wide boxes (one wave per box, fused_triples) by more
+  // than the measured rate ratio of the two kernels (63.4 vs 27.9 TF/s at
+  // no=40/nv=320: 2.2x), both counted in the job's own kernel tasks.
+  // TCE_T_KERNEL = wave | wg | wg16 forces fused_triples, fused_triples_wg
+  // or the FP64 policy of fused_triples_mixed.
   {
-    double w4 = 0, w8 = 0;
-    for (long p = noab + 1; p <= noab + nvab; ++p) {
-      double r = (double)T.rng(p);
-      w4 += std::ceil(r / 4) * 4 / r;
-      w8 += std::ceil(r / 8) * 8 / r;
-    }
-    J->pbox = (w8 <= 1.25 * w4) ? 8 : 4;
+    const double cells4 = 64.0 * count_items(*J, 4), cells8 = 512.0 * count_items(*J, 8);
+    J->pbox = (cells8 <= 2.2 * cells4) ? 8 : 4;
     // 8-wide boxes run on the 16x16x4 FP64 kernel (measured 3% faster than
     // the 4x4x4 one at no=40/nv=320)
     J->k16 = J->pbox == 8;

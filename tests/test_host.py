@@ -284,3 +284,37 @@ def test_job_plan_without_allowed_tasks():
     for tiling in (0, 1, 2):
         for precision in (0, 2):
             assert _work_tiling(tm, tiling, precision) == (0.0, 0)
+
+
+def _kernel_and_items(tm, tiling, kernel=None):
+    L = _lib.lib()
+    old = os.environ.pop("TCE_T_KERNEL", None)
+    if kernel:
+        os.environ["TCE_T_KERNEL"] = kernel
+    try:
+        h = C.c_void_p()
+        _lib.check(L.tce_t_job_create_tiling(C.byref(tm.c()), tiling, C.byref(h)), "create")
+    finally:
+        os.environ.pop("TCE_T_KERNEL", None)
+        if old is not None:
+            os.environ["TCE_T_KERNEL"] = old
+    f, n = C.c_double(0), C.c_long(0)
+    L.tce_t_job_work(h, 0, C.byref(f), C.byref(n))
+    k = L.tce_t_job_kernel(h)
+    L.tce_t_job_destroy(h)
+    return k, n.value
+
+
+@pytest.mark.parametrize("ro,rv", [([8, 8], [16] * 4), ([9], [21] * 6), ([19], [31] * 4 + [1]),
+                                   ([6, 4], [19] * 10 + [1]), ([8], [16, 16])])
+@pytest.mark.parametrize("tiling", [0, 1, 2])
+def test_particle_box_choice_follows_the_padded_work(ro, rv, tiling):
+    """8-wide particle boxes (the 16x16x4 kernel) unless their padded work
+    (items x 8^3) exceeds 2.2x that of 4-wide boxes (items x 4^3), counted in
+    the job's own kernel tasks."""
+    tm = tl.TileModel.synthetic(sum(ro), sum(rv), 0, 0, 2, ro, rv)
+    _, n4 = _kernel_and_items(tm, tiling, "wave")
+    _, n8 = _kernel_and_items(tm, tiling, "wg16")
+    k, n = _kernel_and_items(tm, tiling)
+    want8 = 512 * n8 <= 2.2 * 64 * n4
+    assert (k == 2) == want8 and n == (n8 if want8 else n4)
