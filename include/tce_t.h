@@ -185,6 +185,21 @@ int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt);
  * blocks without allowed tasks are 0. */
 int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin, double *table,
                           long max_tiles, double *t_energy);
+/* Triples tiles of one task, for the consumers of the reference's CPU
+ * drivers other than the (T) energy (cr_ccsd_t.F:139-144, lambda_ccsd_t.F,
+ * emb/emb2.F:293, mrcc/ccsd_t/tce_mrcc_ccsdpt_subg.F:202).  Replaces
+ * ccsd_t_singles_l(singles, ..., t_h1b..t_p6b, 2) (ccsd_t_singles_l.F:1-28)
+ * and ccsd_t_doubles_l(doubles, ..., t_h1b..t_p6b, 2) (ccsd_t_doubles_l.F:
+ * 1-100) on the job's resident tensors: adds t3s to singles and t3d to
+ * doubles, host arrays of r(h1b) r(h2b) r(h3b) r(p4b) r(p5b) r(p6b) doubles in
+ * triplesx(h3,h2,h1,p6,p5,p4) order (Fortran), as the drivers accumulate
+ * into the caller's zeroed buffers.  Either array may be NULL.  Tiles are
+ * 1-based (holes 1..noab, particles noab+1..), h1b <= h2b <= h3b,
+ * p4b <= p5b <= p6b, and the sextuple must pass the task loop's spin, irrep
+ * and restricted tests (else TCE_T_ERR_ARG).  Any tiling; d1/d2 products in
+ * `precision`.  Synchronous. */
+int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b, long h3b,
+                            long p4b, long p5b, long p6b, double *singles, double *doubles);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);

@@ -86,6 +86,7 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_int)]
     L.tce_t_server_destroy.argtypes = [C.c_void_p]
     L.tce_t_job_run_restart.argtypes = [C.c_void_p, C.c_int, LP, DP, C.c_long, DP]
+    L.tce_t_job_triples_tiles.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 6 + [DP, DP]
     L.tce_t_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.tce_t_job_kernel.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.restype = C.c_int

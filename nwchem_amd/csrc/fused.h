@@ -162,18 +162,15 @@ __device__ __forceinline__ long round_item(long r, unsigned b, unsigned G, bool 
   return r * (long)G + ((mirror && (r & 1)) ? (long)G - 1 - f : f);
 }
 
-// Segment of global index g (uniform): the last with g0 <= g.
+// Segment of global index g for a workgroup whose previous item lay in
+// segment s (uniform).  A workgroup's global indices increase from round to
+// round, so its segment only moves forward: one read of the next segment's
+// start per item replaces a binary search of the table.  Empty segments are
+// stepped over.
 template <class Args>
-__device__ __forceinline__ int find_segment(const SegmentT<Args> *s, int n, long g) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (s[mid].g0 <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
+__device__ __forceinline__ int next_segment(const SegmentT<Args> *segs, int nseg, int s, long g) {
+  while (s + 1 < nseg && segs[s + 1].g0 <= g) ++s;
+  return s;
 }
 
 typedef double acc_t;
@@ -304,6 +301,10 @@ __device__ __forceinline__ int find_task(gptr<const TaskDev> tasks, int ntasks, 
 int launch_fused_wave(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
 int launch_fused_wg(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
+                       hipStream_t s);
+// One task's t3 tiles (fused_mixed.hip): `total` items of the single task at
+// args->f.tasks[0], tiles written to args->f.partials (doubles, then singles).
+int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
                        hipStream_t s);
 
 }  // namespace tce_t

@@ -587,7 +587,69 @@ __device__ __forceinline__ void energy(const AccT (&acc)[64], const double *e,
   e5 = e4 + x;
 }
 
-template <int PREC>
+// Tile export: the task's doubles d and singles s (the products of the
+// energy's singles term, without the denominator), element (a, b, c, i, j, k)
+// of the task at ((((a r5 + b) r6 + c) r1 + i) r2 + j) r3 + k, i.e. the
+// reference's triplesx(h3,h2,h1,p6,p5,p4); doubles at A.partials, singles
+// n = |tile| doubles after them.  One task per launch (A.tasks[0]).
+template <class AccT>
+__device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double *e,
+                                            const FusedArgsG &A, const TaskDev &T, int sa, int sb,
+                                            int sc, int la, int lb, int lc, int i0, int j0,
+                                            int k0) {
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  const long n = (long)T.r1 * T.r2 * T.r3 * T.r4 * T.r5 * T.r6;
+  const gptr<double> td = A.partials, ts = A.partials + n;
+  const long base = (((long)la * T.r5 + lb) * T.r6 + lc) * T.r1;
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        if (i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3)
+          td[((base + i0 + ii) * T.r2 + j0 + jj) * T.r3 + k0 + kk] = (double)acc[16 * ii + 4 * jj + kk];
+  double sv[64];
+#pragma unroll
+  for (int r = 0; r < 64; ++r) sv[r] = 0.0;
+  const int obc = 0 * 64 + sb * 8 + sc, oac = 1 * 64 + sa * 8 + sc, oab = 2 * 64 + sa * 8 + sb;
+  const double *t1 = e + EPI_S;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    double ta[4], tb[4], tc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ta[u] = t1[((0 * 3 + t) * 8 + sa) * 4 + u];
+      tb[u] = t1[((1 * 3 + t) * 8 + sb) * 4 + u];
+      tc[u] = t1[((2 * 3 + t) * 8 + sc) * 4 + u];
+    }
+    const double sg = t == 1 ? -1.0 : 1.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const double *sp = e + (t * 16 + u * 4 + w) * 192;
+        const double sbc = sg * sp[obc], sac = sg * sp[oac], sab = sg * sp[oab];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int r = t == 0 ? m * 16 + u * 4 + w : (t == 1 ? u * 16 + m * 4 + w : u * 16 + w * 4 + m);
+          sv[r] += ta[m] * sbc - tb[m] * sac + tc[m] * sab;
+        }
+      }
+  }
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        if (i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3)
+          ts[((base + i0 + ii) * T.r2 + j0 + jj) * T.r3 + k0 + kk] = sv[16 * ii + 4 * jj + kk];
+}
+
+// The item loop of both kernels below: TILES = false writes each item's
+// energy partials, TILES = true the task's t3 tiles (store_tiles).
+template <int PREC, bool TILES>
 __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *segs, int nseg,
                                                            long nglobal) {
   __shared__ f4 lds[lds_quads<PREC>()];
@@ -597,13 +659,15 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
   using Pr = Prec<PREC>;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier is reached by all 8 waves).
+  int seg = 0;  // segment of the workgroup's current item
   for (long r = 0; r * (long)gridDim.x < nglobal; ++r) {
     const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
     if (g >= nglobal) continue;  // last round: uniform per workgroup
-    const MixedSegment &sg = segs[find_segment(segs, nseg, g)];
+    seg = next_segment(segs, nseg, seg, g);
+    const MixedSegment &sg = segs[seg];
+    const long item = sg.item0 + (g - sg.g0);
     const MixedArgsG M = load_args(sg.args);
     const FusedArgsG &A = M.f;
-    const long item = sg.item0 + (g - sg.g0);
     const int total = 9 * (M.Ek / Pr::KC) + 9 * (M.Mk / Pr::KC);  // chunks per item
     const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));
     long box = item - T.item0;
@@ -657,12 +721,17 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     __syncthreads();
     // final layout: L = c, lane = 4a' + b' + 16c'
     const int sa = 4 * cube(wave, 0) + hi, sb = 4 * cube(wave, 1) + lo, sc = 4 * cube(wave, 2) + kq;
-    double e4, e5;
-    energy(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G, e4, e5);
-    wave_sum(e4, e5);
-    if (lane == 0) {
-      A.partials[2 * (item * 8 + wave)] = e4;
-      A.partials[2 * (item * 8 + wave) + 1] = e5;
+    if constexpr (TILES) {
+      store_tiles(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk);
+    } else {
+      double e4, e5;
+      energy(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G, e4,
+             e5);
+      wave_sum(e4, e5);
+      if (lane == 0) {
+        A.partials[2 * (item * 8 + wave)] = e4;
+        A.partials[2 * (item * 8 + wave) + 1] = e5;
+      }
     }
     __syncthreads();  // the next item's first loads overwrite the staged data
   }
@@ -675,14 +744,31 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        hipStream_t s) {
   if (precision == TCE_T_FP64) {
-    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_FP64>, dim3((unsigned)grid), dim3(512), 0, s, segs,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP64, false>), dim3((unsigned)grid), dim3(512), 0, s, segs,
                        nseg, total);
   } else if (precision == TCE_T_FP32) {
-    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_FP32>, dim3((unsigned)grid), dim3(512), 0, s, segs,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP32, false>), dim3((unsigned)grid), dim3(512), 0, s, segs,
                        nseg, total);
   } else if (precision == TCE_T_BF16X3) {
-    hipLaunchKernelGGL(fused_triples_mixed<TCE_T_BF16X3>, dim3((unsigned)grid), dim3(512), 0, s,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_BF16X3, false>), dim3((unsigned)grid), dim3(512), 0, s,
                        segs, nseg, total);
+  } else {
+    return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
+                       hipStream_t s) {
+  if (precision == TCE_T_FP64) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP64, true>), dim3((unsigned)grid), dim3(512), 0, s, segs,
+                       1, total);
+  } else if (precision == TCE_T_FP32) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP32, true>), dim3((unsigned)grid), dim3(512), 0, s, segs,
+                       1, total);
+  } else if (precision == TCE_T_BF16X3) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_BF16X3, true>), dim3((unsigned)grid), dim3(512), 0, s,
+                       segs, 1, total);
   } else {
     return -1;
   }

@@ -58,12 +58,14 @@ __global__ void __launch_bounds__(64, 2) fused_triples(const Segment *segs, int 
   // Persistent loop, static round-robin over the launch's items: the item
   // index is wave-uniform by construction and every wave leaves after at
   // most ceil(total / gridDim.x) items.
+  int seg = 0;  // segment of the workgroup's current item
   for (long r = 0; r * (long)gridDim.x < total; ++r) {
     const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
     if (g >= total) continue;  // last round: uniform per workgroup
-    const Segment &sg = segs[find_segment(segs, nseg, g)];
-    const FusedArgsG A = load_args(sg.args);
+    seg = next_segment(segs, nseg, seg, g);
+    const Segment &sg = segs[seg];
     const long item = sg.item0 + (g - sg.g0);
+    const FusedArgsG A = load_args(sg.args);
     const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
     const long nvE = (long)nv * Ep, nvM = (long)nv * Mp;
     const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));

@@ -300,12 +300,14 @@ __global__ void __launch_bounds__(512) fused_triples_wg(const Segment *segs, int
   const int ca = (wave >> 2) & 1, cb = (wave >> 1) & 1, cc = wave & 1;
   // Persistent loop, static round-robin over the launch's items (uniform
   // across the workgroup, so every barrier below is reached by all 8 waves).
+  int seg = 0;  // segment of the workgroup's current item
   for (long r = 0; r * (long)gridDim.x < total; ++r) {
     const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
     if (g >= total) continue;  // last round: uniform per workgroup
-    const Segment &sg = segs[find_segment(segs, nseg, g)];
-    const FusedArgsG A = load_args(sg.args);
+    seg = next_segment(segs, nseg, seg, g);
+    const Segment &sg = segs[seg];
     const long item = sg.item0 + (g - sg.g0);
+    const FusedArgsG A = load_args(sg.args);
     const int no = A.no, nv = A.nv, Ep = A.Ep, Mp = A.Mp;
     const long sT2e = (long)nv * Ep;      // stride of (h*no+h') in T2e
     const long sW = (long)nv * Mp;        // stride of (h*no+h') in Wd1

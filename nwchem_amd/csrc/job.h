@@ -55,6 +55,8 @@ struct Job {
   long *dbound = nullptr, *dbound8 = nullptr;  // dbound8 == dbound when pbox == 8
   // launch tables (job arguments + segments): pinned host staging, device copy
   void *tab_host = nullptr, *tab_dev = nullptr;
+  void *dtile = nullptr;  // tile export: task entry, then doubles and singles tiles
+  size_t dtile_bytes = 0;
   struct Mixed {
     void *x2[2] = {nullptr, nullptr}, *y2[2] = {nullptr, nullptr};  // T2e, Vd2
     void *x1[2] = {nullptr, nullptr}, *y1[2] = {nullptr, nullptr};  // Wd1, T2m

@@ -360,7 +360,6 @@ extern "C" int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce
     return TCE_T_ERR_ARG;
   }
   const Tiles &T = J->T;
-  const long noab = T.noab, nvab = T.nvab;
   // Kernel choice: 8-wide particle boxes (cooperative workgroups, operands
   // shared through LDS, the 16x16x4 FP64 kernel) unless their padded work
   // exceeds that of 4-wide boxes (one wave per box, fused_triples) by more
@@ -1022,6 +1021,117 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
   return TCE_T_OK;
 }
 
+// Triples tiles of one task for the other consumers of the reference's
+// drivers: cr_ccsd_t.F:139-144, lambda_ccsd_t.F (ccsd_t_doubles),
+// emb/emb2.F:293 and mrcc/ccsd_t/tce_mrcc_ccsdpt_subg.F:202 call
+// ccsd_t_singles_l (ccsd_t_singles_l.F:1-28) and ccsd_t_doubles_l
+// (ccsd_t_doubles_l.F:1-100, toggle 2) to add t3s / t3d of the task with
+// tiles (t_h1b..t_p6b) into their triplesx(h3,h2,h1,p6,p5,p4) buffers, then
+// apply their own functionals.  Here the task's items run through the fused
+// kernel on the resident tensors (store_tiles epilogue) and the tiles come
+// back to the host, where they are added to *singles / *doubles (either may
+// be NULL).  Synchronous on the job's stream.
+extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b,
+                                       long h3b, long p4b, long p5b, long p6b, double *singles,
+                                       double *doubles) {
+  if (!job) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (!J.uploaded) {
+    set_error("tce_t_job_triples_tiles before tce_t_job_upload");
+    return TCE_T_ERR_STATE;
+  }
+  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+    set_error("unsupported precision mode");
+    return TCE_T_ERR_ARG;
+  }
+  const Tiles &T = J.T;
+  const long noab = T.noab, nvab = T.nvab;
+  if (!(1 <= h1b && h1b <= h2b && h2b <= h3b && h3b <= noab && noab < p4b && p4b <= p5b &&
+        p5b <= p6b && p6b <= noab + nvab) ||
+      !task_allowed(T, p4b, p5b, p6b, h1b, h2b, h3b)) {
+    set_error("tile sextuple is not a task of the job");
+    return TCE_T_ERR_ARG;
+  }
+  if (precision != TCE_T_FP64) {
+    const int rc = prepare_mixed(J, precision);
+    if (rc != TCE_T_OK) return rc;
+  }
+  TaskDev d;
+  d.op4 = (int)(T.ofs(p4b) - T.no);
+  d.op5 = (int)(T.ofs(p5b) - T.no);
+  d.op6 = (int)(T.ofs(p6b) - T.no);
+  d.oh1 = (int)T.ofs(h1b);
+  d.oh2 = (int)T.ofs(h2b);
+  d.oh3 = (int)T.ofs(h3b);
+  d.r4 = (int)T.rng(p4b);
+  d.r5 = (int)T.rng(p5b);
+  d.r6 = (int)T.rng(p6b);
+  d.r1 = (int)T.rng(h1b);
+  d.r2 = (int)T.rng(h2b);
+  d.r3 = (int)T.rng(h3b);
+  d.nb4 = (d.r4 + 7) / 8;
+  d.nb5 = (d.r5 + 7) / 8;
+  d.nb6 = (d.r6 + 7) / 8;
+  d.nb1 = (d.r1 + 3) / 4;
+  d.nb2 = (d.r2 + 3) / 4;
+  d.nb3 = (d.r3 + 3) / 4;
+  d.item0 = 0;
+  d.factor = 1.0;
+  const long n = (long)d.r1 * d.r2 * d.r3 * d.r4 * d.r5 * d.r6;
+  const long items = (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
+  constexpr size_t HEAD = 256;  // the task entry, then the tiles
+  static_assert(sizeof(TaskDev) <= HEAD, "tile export header");
+  const size_t need = HEAD + 2 * (size_t)n * sizeof(double);
+  if (J.dtile_bytes < need) {
+    if (J.dtile) HIPCHK(hipFree(J.dtile), "tile buffer");
+    J.dtile = nullptr;
+    J.dtile_bytes = 0;
+    HIPCHK(hipMalloc(&J.dtile, need), "tile buffer");
+    J.dtile_bytes = need;
+  }
+  {
+    const int rc = ensure_table(J);
+    if (rc != TCE_T_OK) return rc;
+  }
+  // the task entry travels in the pinned launch table, after the arguments
+  // and the segment
+  constexpr size_t TASK_AT = 512;
+  static_assert((sizeof(MixedArgs) + 15) / 16 * 16 + sizeof(MixedSegment) <= TASK_AT &&
+                    TASK_AT + sizeof(TaskDev) <= TAB_BYTES,
+                "tile export table");
+  std::memcpy((char *)J.tab_host + TASK_AT, &d, sizeof d);
+  HIPCHK(hipMemcpyAsync(J.dtile, (char *)J.tab_host + TASK_AT, sizeof d, hipMemcpyHostToDevice,
+                        J.stream),
+         "task entry");
+  MixedArgs M = job_margs(J, precision);
+  M.f.tasks = (const TaskDev *)J.dtile;
+  M.f.ntasks = 1;
+  M.f.nitems = items;
+  M.f.partials = (double *)((char *)J.dtile + HEAD);
+  const MixedSegment *sg = stage_table(J, M, items);
+  if (!sg) {
+    set_error("launch table copy failed");
+    return TCE_T_ERR_HIP;
+  }
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
+  if (launch_fused_tiles(sg, items, precision, (int)std::min<long>(std::max(1, cus), items),
+                         J.stream) != 0) {
+    set_error("tile kernel launch failed");
+    return TCE_T_ERR_HIP;
+  }
+  std::vector<double> h(2 * (size_t)n);
+  HIPCHK(hipMemcpyAsync(h.data(), M.f.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        J.stream),
+         "D2H");
+  HIPCHK(hipStreamSynchronize(J.stream), "fused_triples_tiles");
+  if (doubles)
+    for (long i = 0; i < n; ++i) doubles[i] += h[i];
+  if (singles)
+    for (long i = 0; i < n; ++i) singles[i] += h[n + i];
+  return TCE_T_OK;
+}
+
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
   if (!job || !out) return TCE_T_ERR_ARG;
   const Job &J = *reinterpret_cast<const Job *>(job);
@@ -1061,6 +1171,9 @@ void tce_t::job_release(Job &J) {
   }
   if (J.dtasks) (void)hipFree(J.dtasks);
   J.dtasks = nullptr;
+  if (J.dtile) (void)hipFree(J.dtile);
+  J.dtile = nullptr;
+  J.dtile_bytes = 0;
   for (auto &m : J.mx) {
     for (void **b : {&m.x2[0], &m.x2[1], &m.y2[0], &m.y2[1], &m.x1[0], &m.x1[1], &m.y1[0], &m.y1[1]}) {
       if (*b) (void)hipFree(*b);

@@ -1,7 +1,9 @@
 ! tce_t_fdemo.F90 -- a Fortran caller of the whole-job API: builds the tile
 ! model the way tce_tile.F does (holes first, then particles; C1, one spin),
 ! fills T1/T2/V2 canonical blocks with the seeded generator, uploads them with
-! their hash tables and prints E[T], E(T).
+! their hash tables and prints E[T], E(T); then E(T) by restart blocks, and
+! E[T], E(T) once more from the task tiles the library hands back, summed
+! here as the tiles' other consumers (cr_ccsd_t.F, lambda_ccsd_t.F) do.
 ! Usage: tce_t_fdemo [no nv tile_o tile_v]   (default 8 32 8 16)
 program tce_t_fdemo
   use iso_c_binding
@@ -17,6 +19,9 @@ program tce_t_fdemo
   real(c_double) :: e1, e2, et
   real(c_double), allocatable :: table(:)
   integer(c_long) :: begin
+  integer(c_long) :: p4b, p5b, p6b, h1b, h2b, h3b, a4, a5, a6, i1, i2, i3, sz
+  real(c_double), allocatable :: singles(:), doubles(:)
+  real(c_double) :: factor, den, ta, tb
   integer(c_int) :: rc
   character(len=32) :: arg
   integer(c_long_long), parameter :: seed = 1234_c_long_long
@@ -71,5 +76,57 @@ program tce_t_fdemo
     if (rc /= TCE_T_OK) stop 7
   end do
   write(*, '(A,I0,A,ES24.16)') 'restart blocks=', nvab, ' E(T)=', et
+  ! the task loop of ccsd_t_gpu.F (C1, one spin: every sextuple is a task)
+  ! with the tiles from tce_t_job_triples_tiles and the energy of
+  ! ccsd_t_gpu.F:153-200 evaluated by the caller
+  ta = 0.0d0; tb = 0.0d0
+  do p4b = noab + 1, n
+  do p5b = p4b, n
+  do p6b = p5b, n
+  do h1b = 1, noab
+  do h2b = h1b, noab
+  do h3b = h2b, noab
+    sz = range(p4b) * range(p5b) * range(p6b) * range(h1b) * range(h2b) * range(h3b)
+    allocate(singles(sz), doubles(sz))
+    singles = 0.0d0; doubles = 0.0d0
+    rc = tce_t_job_triples_tiles(job, TCE_T_FP64, h1b, h2b, h3b, p4b, p5b, p6b, singles, doubles)
+    if (rc /= TCE_T_OK) stop 8
+    factor = 1.0d0
+    if (p4b == p5b .and. p5b == p6b) then
+      factor = factor / 6.0d0
+    else if (p4b == p5b .or. p5b == p6b) then
+      factor = factor / 2.0d0
+    end if
+    if (h1b == h2b .and. h2b == h3b) then
+      factor = factor / 6.0d0
+    else if (h1b == h2b .or. h2b == h3b) then
+      factor = factor / 2.0d0
+    end if
+    i = 0
+    do a4 = 1, range(p4b)
+    do a5 = 1, range(p5b)
+    do a6 = 1, range(p6b)
+    do i1 = 1, range(h1b)
+    do i2 = 1, range(h2b)
+    do i3 = 1, range(h3b)
+      i = i + 1
+      den = evl(offset(h1b) + i1) + evl(offset(h2b) + i2) + evl(offset(h3b) + i3) &
+          - evl(offset(p4b) + a4) - evl(offset(p5b) + a5) - evl(offset(p6b) + a6)
+      ta = ta + factor * doubles(i) * doubles(i) / den
+      tb = tb + factor * doubles(i) * (doubles(i) + singles(i)) / den
+    end do
+    end do
+    end do
+    end do
+    end do
+    end do
+    deallocate(singles, doubles)
+  end do
+  end do
+  end do
+  end do
+  end do
+  end do
+  write(*, '(A,ES24.16,A,ES24.16)') 'tiles E[T]=', ta, ' E(T)=', tb
   rc = tce_t_job_destroy(job)
 end program tce_t_fdemo

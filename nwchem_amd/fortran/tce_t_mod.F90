@@ -90,6 +90,18 @@ module tce_t_mi355x
       type(c_ptr), intent(out) :: job
     end function
 
+    ! ccsd_t_singles_l / ccsd_t_doubles_l (toggle 2) on the resident tensors:
+    ! adds t3s to singles and t3d to doubles (triplesx(h3,h2,h1,p6,p5,p4) of
+    ! the task with these 1-based tiles), for cr_ccsd_t, lambda_ccsd_t, emb2
+    integer(c_int) function tce_t_job_triples_tiles(job, precision, h1b, h2b, h3b, p4b, p5b, &
+        p6b, singles, doubles) bind(C, name="tce_t_job_triples_tiles")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      integer(c_int), value :: precision
+      integer(c_long), value :: h1b, h2b, h3b, p4b, p5b, p6b
+      real(c_double), intent(inout) :: singles(*), doubles(*)
+    end function
+
     integer(c_int) function tce_t_job_task_energies(job, out, ntasks) &
         bind(C, name="tce_t_job_task_energies")
       import :: c_int, c_long, c_double, c_ptr

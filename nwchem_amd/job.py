@@ -100,6 +100,25 @@ class TriplesJob:
               "tce_t_job_task_energies")
         return out
 
+    def triples_tiles(self, h1b: int, h2b: int, h3b: int, p4b: int, p5b: int, p6b: int,
+                      precision: int = FP64) -> tuple[np.ndarray, np.ndarray]:
+        """(t3s, t3d) of the task with these 1-based tiles, as
+        ccsd_t_singles_l / ccsd_t_doubles_l (toggle 2) leave them: arrays of
+        shape (r(p4b), r(p5b), r(p6b), r(h1b), r(h2b), r(h3b)), i.e. the
+        reference's triplesx(h3,h2,h1,p6,p5,p4) in Fortran order."""
+        tiles = (p4b, p5b, p6b, h1b, h2b, h3b)
+        if not all(1 <= t <= len(self.tiles.range) for t in tiles):
+            # not tiles of the job: the library refuses the call (no buffers)
+            check(_lib.lib().tce_t_job_triples_tiles(self._h, precision, h1b, h2b, h3b, p4b, p5b,
+                                                     p6b, None, None), "tce_t_job_triples_tiles")
+        shape = tuple(int(self.tiles.range[b - 1]) for b in tiles)
+        s = np.zeros(shape)
+        d = np.zeros(shape)
+        check(_lib.lib().tce_t_job_triples_tiles(self._h, precision, h1b, h2b, h3b, p4b, p5b, p6b,
+                                                 s.ctypes.data_as(DP), d.ctypes.data_as(DP)),
+              "tce_t_job_triples_tiles")
+        return s, d
+
     def work(self, precision: int = FP64) -> tuple[float, int]:
         """(flops, work items) of a run in `precision`: the reference's
         per-task flop count summed over the kernel tasks actually run (the

@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
         L.ora_set_mutation.argtypes = [C.c_int, C.c_int]
         L.ora_set_mutation.restype = None
         L.ora_brute_force.argtypes = [C.c_long, C.c_long] + [DP] * 7 + [DP, DP]
+        L.ora_task_tiles.argtypes = [C.POINTER(OraTiles), DP, LP, DP, LP, DP, LP, LP, DP, DP]
+        L.ora_task_tiles.restype = C.c_long
         L._declared = True
     return L
 
@@ -188,6 +190,22 @@ def ccsd_t(tiles: Tiles, t1, t1h, t2, t2h, v2, v2h, backend: OraBackend | None =
     if rc < 0:
         raise RuntimeError(f"ora_ccsd_t_gpu failed: {rc}")
     return e1.value, e2.value, te[:rc]
+
+
+def task_tiles(tiles: Tiles, t1, t1h, t2, t2h, v2, v2h, task):
+    """(t3s, t3d) of the task with tiles task = (p4b, p5b, p6b, h1b, h2b, h3b),
+    1-based, as the reference's singles and doubles drivers leave them:
+    arrays of shape (r4, r5, r6, r1, r2, r3) (triplesx(h3,h2,h1,p6,p5,p4))."""
+    L = lib()
+    tk = np.ascontiguousarray(task, np.int64)
+    shape = tuple(int(tiles.range[b - 1]) for b in tk)
+    s = np.zeros(shape)
+    d = np.zeros(shape)
+    rc = L.ora_task_tiles(C.byref(tiles.c), _dp(t1), _lp(t1h), _dp(t2), _lp(t2h), _dp(v2), _lp(v2h),
+                          _lp(tk), _dp(s), _dp(d))
+    if rc < 0:
+        raise ValueError(f"ora_task_tiles({list(tk)}) failed: {rc}")
+    return s, d
 
 
 def brute_force(no, nv, T1, T2, W, V, S, eo, ev):

@@ -720,33 +720,63 @@ long ora_count_tasks(const ora_tiles_t *t) {
   return count;
 }
 
+/* Dispatch state over the caller's blocks, with scratch for the largest
+ * block any fetch can return (4 tile ranges).  0 or -100 (no memory). */
+static int dispatch_init(dispatch_t *D, const ora_tiles_t *t, const double *t1,
+                         const long *t1_hash, const double *t2, const long *t2_hash,
+                         const double *v2, const long *v2_hash, const ora_backend_t *be) {
+  memset(D, 0, sizeof(*D));
+  D->t = t;
+  D->t1 = t1;
+  D->t2 = t2;
+  D->v2 = v2;
+  D->t1_hash = t1_hash;
+  D->t2_hash = t2_hash;
+  D->v2_hash = v2_hash;
+  D->be = be ? be : &builtin_backend;
+  long maxr = 0;
+  for (long b = 1; b <= t->noab + t->nvab; b++)
+    if (RNG(t, b) > maxr) maxr = RNG(t, b);
+  D->scratch_len = maxr * maxr * maxr * maxr;
+  for (int i = 0; i < 3; i++) {
+    D->scratch[i] = (double *)malloc((size_t)D->scratch_len * sizeof(double));
+    if (!D->scratch[i]) return -100;
+  }
+  return 0;
+}
+
+long ora_task_tiles(const ora_tiles_t *t, const double *t1, const long *t1_hash,
+                    const double *t2, const long *t2_hash, const double *v2,
+                    const long *v2_hash, const long tiles[6], double *t3s,
+                    double *t3d) {
+  const long p4b = tiles[0], p5b = tiles[1], p6b = tiles[2];
+  const long h1b = tiles[3], h2b = tiles[4], h3b = tiles[5];
+  const long noab = t->noab, nvab = t->nvab;
+  if (!(1 <= h1b && h1b <= h2b && h2b <= h3b && h3b <= noab && noab < p4b && p4b <= p5b &&
+        p5b <= p6b && p6b <= noab + nvab))
+    return -1;
+  if (!task_ok(t, p4b, p5b, p6b, h1b, h2b, h3b)) return -1;
+  dispatch_t D;
+  if (dispatch_init(&D, t, t1, t1_hash, t2, t2_hash, v2, v2_hash, NULL)) return -100;
+  const long task[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
+  const long size = RNG(t, p4b) * RNG(t, p5b) * RNG(t, p6b) * RNG(t, h1b) * RNG(t, h2b) *
+                    RNG(t, h3b);
+  memset(t3s, 0, (size_t)size * sizeof(double));
+  memset(t3d, 0, (size_t)size * sizeof(double));
+  singles(&D, t3s, task);
+  doubles_1(&D, t3d, task);
+  doubles_2(&D, t3d, task);
+  for (int i = 0; i < 3; i++) free(D.scratch[i]);
+  return D.err ? D.err : size;
+}
+
 long ora_ccsd_t_gpu(const ora_tiles_t *t, const double *t1, const long *t1_hash,
                     const double *t2, const long *t2_hash, const double *v2,
                     const long *v2_hash, const ora_backend_t *be, long task_begin,
                     long task_end, double *energy1, double *energy2, double *task_e) {
   long noab = t->noab, nvab = t->nvab;
   dispatch_t D;
-  memset(&D, 0, sizeof(D));
-  D.t = t;
-  D.t1 = t1;
-  D.t2 = t2;
-  D.v2 = v2;
-  D.t1_hash = t1_hash;
-  D.t2_hash = t2_hash;
-  D.v2_hash = v2_hash;
-  D.be = be ? be : &builtin_backend;
-  long maxr = 0;
-  for (long b = 1; b <= noab + nvab; b++)
-    if (RNG(t, b) > maxr) maxr = RNG(t, b);
-  long maxo = 0;
-  for (long b = 1; b <= noab; b++)
-    if (RNG(t, b) > maxo) maxo = RNG(t, b);
-  /* scratch: the largest block any fetch can return (4 tile ranges) */
-  D.scratch_len = maxr * maxr * maxr * maxr;
-  for (int i = 0; i < 3; i++) {
-    D.scratch[i] = (double *)malloc((size_t)D.scratch_len * sizeof(double));
-    if (!D.scratch[i]) return -100;
-  }
+  if (dispatch_init(&D, t, t1, t1_hash, t2, t2_hash, v2, v2_hash, be)) return -100;
   double e1 = 0.0, e2 = 0.0;
   long count = 0, done = 0;
   double *singles_buf = NULL, *doubles_buf = NULL;

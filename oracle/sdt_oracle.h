@@ -32,6 +32,9 @@
  *                              antisymmetry, (b) tile-size invariance,
  *                              (c) t1 = 0  =>  E(T) == E[T]
  *   ora_ccsd_t_dot ........... bit-exact against ccsd_t_dot_ from oracle/_ref
+ *   ora_task_tiles ........... the dense textbook t3d / t3s (P(9)
+ *                              antisymmetrisers, numpy) restricted to the
+ *                              task's tiles, on spin/irrep-labelled systems
  *   ora_brute_force .......... closed-form special cases (single nonzero
  *                              amplitude/integral products)
  *
@@ -148,6 +151,19 @@ long ora_ccsd_t_gpu(const ora_tiles_t *t, const double *t1, const long *t1_hash,
  *   S[nv][nv][no][no]   <p p' || h h'>       (s1 integral block layout)
  *   eo[no], ev[nv] orbital energies.
  * e4 = E[T] = (1/36) sum t3d^2/D ; e5 = E(T) = (1/36) sum t3d (t3d+t3s)/D. */
+/* The two triples tiles of one task (tile numbers p4b,p5b,p6b,h1b,h2b,h3b,
+ * canonical order, allowed by the spin/irrep/restricted rules), as the
+ * reference's drivers leave them for the energy and for the other consumers
+ * of ccsd_t_singles_l / ccsd_t_doubles_l (cr_ccsd_t.F:139-144, emb/emb2.F:293,
+ * mrcc/ccsd_t/tce_mrcc_ccsdpt_subg.F:202): t3s from the 9 s1 rows
+ * (ccsd_t_singles_gpu.F:101-576), t3d from the d1 and d2 rows
+ * (ccsd_t_doubles_gpu.F:132-1345), both triplesx(h3,h2,h1,p6,p5,p4) in
+ * Fortran order.  Overwrites t3s/t3d; returns the tile size, -1 for a
+ * sextuple that is not a task, or a fetch error (< -1). */
+long ora_task_tiles(const ora_tiles_t *t, const double *t1, const long *t1_hash,
+                    const double *t2, const long *t2_hash, const double *v2,
+                    const long *v2_hash, const long tiles[6], double *t3s, double *t3d);
+
 void ora_brute_force(long no, long nv, const double *T1, const double *T2,
                      const double *W, const double *V, const double *S,
                      const double *eo, const double *ev, double *e4,

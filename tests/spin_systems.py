@@ -173,6 +173,40 @@ class SpinSystem:
         T = self.tiles
         return oracle.Tiles(T.range, T.noab, T.evl, T.spin, T.sym, T.alpha, T.restricted)
 
+    def dense_t3(self, d1_sign=1.0):
+        """Textbook (t3s, t3d)[a, b, c, i, j, k] over all spin-orbitals:
+            t3d = P(a/bc) P(k/ij) [-sum_e t(a e i j) <b c||k e>]
+                + P(c/ab) P(i/jk) [-sum_m t(a b i m) <m c||j k>]
+            t3s = P(a/bc) P(i/jk) t(a i) <b c||j k>
+        with P(x/yz) f = f - f(x<->y) - f(x<->z).  d1_sign flips the second
+        term (a planted mistake for the tests)."""
+        def tr(x, pa, ph):
+            return np.transpose(x, tuple(pa) + tuple(3 + q for q in ph))
+
+        id3 = ((0, 1, 2), 1.0)
+        p_a = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]   # P(a/bc) on (a,b,c)
+        p_c = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]   # P(c/ab)
+        h_k = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]   # P(k/ij) on (i,j,k)
+        h_i = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]   # P(i/jk)
+        X = -np.einsum("aeij,bcke->abcijk", self.T2, self.V)
+        Y = -np.einsum("abim,mcjk->abcijk", self.T2, self.W)
+        Z = np.einsum("ai,bcjk->abcijk", self.T1, self.S)
+        d = sum(sa * sh * tr(X, pa, ph) for pa, sa in p_a for ph, sh in h_k)
+        d = d + d1_sign * sum(sa * sh * tr(Y, pa, ph) for pa, sa in p_c for ph, sh in h_i)
+        s = sum(sa * sh * tr(Z, pa, ph) for pa, sa in p_a for ph, sh in h_i)
+        return s, d
+
+    def tile_slices(self, task):
+        """Orbital slices (a, b, c, i, j, k) of task = (p4b, p5b, p6b, h1b,
+        h2b, h3b) (1-based tiles) in the dense tensors."""
+        T = self.tiles
+
+        def sl(b):
+            o = int(T.offset[b - 1]) - (self.no if b > T.noab else 0)
+            return slice(o, o + int(T.range[b - 1]))
+
+        return tuple(sl(b) for b in task)
+
     def brute_force(self):
         import oracle
 

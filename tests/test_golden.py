@@ -150,3 +150,52 @@ def test_beta_block_missing_is_an_error():
     t2h[1 + L - 1] = 10**12  # the last key no longer matches its block
     with pytest.raises(RuntimeError):
         oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, t2h, b.v2, b.v2h)
+
+
+# ---- triples tiles of one task (the drivers' output to other consumers)
+
+TILE_CASES = SPIN_CASES + [dict(sizes_o=[3, 2], sizes_v=[4, 3], seed=7, flip_symmetric=True,
+                                restricted=True)]
+
+
+@pytest.mark.parametrize("prm", TILE_CASES, ids=["irreps", "irreps_3v", "c1_ragged", "restricted"])
+def test_oracle_task_tiles_equal_textbook_t3(prm):
+    """ora_task_tiles (the singles and doubles dispatch of one task) against
+    the dense textbook t3s / t3d restricted to the task's tiles, for every
+    task: spin/irrep labels, ragged tiles, restricted runs (beta blocks read
+    from their alpha images)."""
+    s = spin_systems.SpinSystem(**prm)
+    b = s.blocks
+    ds, dd = s.dense_t3()
+    ot = s.oracle_tiles()
+    tasks = s.tiles.tasks()
+    assert len(tasks) > 0
+    scale = np.abs(dd).max()
+    for task in tasks:
+        ts, td = oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, task)
+        sl = s.tile_slices(task)
+        assert np.abs(td - dd[sl]).max() <= 1e-14 * scale, task
+        assert np.abs(ts - ds[sl]).max() <= 1e-14 * scale, task
+
+
+def test_task_tiles_textbook_check_has_teeth():
+    """Planted mistake: the textbook t3d with the d1 term's sign flipped
+    differs from the oracle's tiles, so the check above can fail."""
+    s = spin_systems.SpinSystem(**SPIN_CASES[0])
+    b = s.blocks
+    _, bad = s.dense_t3(d1_sign=-1.0)
+    worst = 0.0
+    for task in s.tiles.tasks():
+        _, td = oracle.task_tiles(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, task)
+        worst = max(worst, np.abs(td - bad[s.tile_slices(task)]).max())
+    assert worst > 1e-6
+
+
+def test_oracle_task_tiles_rejects_non_tasks():
+    s = spin_systems.SpinSystem(**SPIN_CASES[0])
+    b = s.blocks
+    T = s.tiles
+    n = T.noab + T.nvab
+    with pytest.raises(ValueError):  # particle tiles out of order
+        oracle.task_tiles(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                          (n, T.noab + 1, n, 1, 1, 1))

@@ -199,6 +199,10 @@ def test_fortran_caller_matches_oracle():
     r = re.search(r"restart blocks=\s*(\d+)\s+E\(T\)=\s*(\S+)", out)
     assert r, out
     assert int(r.group(1)) == tm.nvab and abs(float(r.group(2)) - e2) <= tol(e2)
+    # tiles from tce_t_job_triples_tiles, the energy summed by the Fortran caller
+    t = re.search(r"tiles E\[T\]=\s*(\S+)\s+E\(T\)=\s*(\S+)", out)
+    assert t, out
+    assert abs(float(t.group(1)) - e1) <= tol(e1) and abs(float(t.group(2)) - e2) <= tol(e2)
 
 
 # Reduced-precision modes: the d1/d2 products in FP32, or split-BF16 with
@@ -549,3 +553,92 @@ def test_empty_system_job_batch_and_server():
         r = [srv.wait(t) for t in ts]
     assert (r[0]["E_bt"], r[0]["E_pt"]) == (0.0, 0.0) and (r[2]["E_bt"], r[2]["E_pt"]) == (0.0, 0.0)
     assert abs(r[1]["E_bt"] - e1) <= tol(e1) and abs(r[1]["E_pt"] - e2) <= tol(e2)
+
+
+# ---- triples tiles of one task (tce_t_job_triples_tiles)
+
+TILE_SYSTEMS = {
+    "spin_irreps": ("spin", SPIN_GPU[0]),
+    "restricted": ("spin", SPIN_GPU[2]),
+    "ragged": ("synthetic", (9, 41, 0, 0, 15, [6, 3], [16, 16, 9])),
+    "no7_nv13": ("synthetic", (7, 13, 3, 5, 11, None, None)),
+}
+TILE_REL = {0: 1e-12, 1: 1e-5, 2: 2e-4}
+
+
+def _tile_system(name):
+    kind, prm = TILE_SYSTEMS[name]
+    if kind == "spin":
+        s = spin_systems.SpinSystem(**prm)
+        return s.tiles, s.blocks, s.oracle_tiles()
+    return system(*prm)
+
+
+def _check_tiles(job, tm, b, ot, precision, tasks):
+    """Element-wise bound relative to the largest |t3d| (|t3s|) over the
+    checked tasks: some tiles are zero up to cancellation (equal tiles whose
+    elements vanish by antisymmetry), so a tile's own maximum is no scale."""
+    rel = TILE_REL[precision]
+    ref = {t: oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, t) for t in tasks}
+    ss = max(np.abs(r[0]).max() for r in ref.values())
+    sd = max(np.abs(r[1]).max() for r in ref.values())
+    assert ss > 0 and sd > 0
+    for task in tasks:
+        p4, p5, p6, h1, h2, h3 = task
+        gs, gd = job.triples_tiles(h1, h2, h3, p4, p5, p6, precision)
+        rs, rd = ref[task]
+        assert np.abs(gd - rd).max() <= rel * sd, (task, np.abs(gd - rd).max(), sd)
+        assert np.abs(gs - rs).max() <= rel * ss, (task, np.abs(gs - rs).max(), ss)
+
+
+@pytest.mark.parametrize("name", list(TILE_SYSTEMS))
+def test_triples_tiles_match_oracle(name):
+    """t3s and t3d of every task, element by element, against the oracle's
+    singles and doubles dispatch (FP64)."""
+    tm, b, ot = _tile_system(name)
+    tasks = tm.tasks()
+    print(f"tiles {name}: {len(tasks)} tasks", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        _check_tiles(job, tm, b, ot, 0, tasks)
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
+def test_triples_tiles_reduced_precision_and_orbital_jobs(precision):
+    """Reduced-precision products, on a job whose kernel tasks are the
+    orbital tiling: the export always works on the reference tiles."""
+    from nwchem_amd import TILING_ORBITAL
+    tm, b, ot = _tile_system("ragged")
+    with TriplesJob(tm, tiling=TILING_ORBITAL) as job:
+        job.upload(b)
+        _check_tiles(job, tm, b, ot, precision, tm.tasks()[::3])
+        _check_tiles(job, tm, b, ot, 0, tm.tasks()[:4])
+
+
+def test_triples_tiles_accumulate_and_reject_non_tasks():
+    """The C entry point adds into the caller's buffers (as the drivers do);
+    a sextuple that is not a task, or a job without upload, is an error."""
+    import ctypes as C
+    tm, b, ot = _tile_system("no7_nv13")
+    L = _lib.lib()
+    DP = C.POINTER(C.c_double)
+    p4, p5, p6, h1, h2, h3 = tm.tasks()[5]
+    shape = tuple(int(tm.range[t - 1]) for t in (p4, p5, p6, h1, h2, h3))
+    with TriplesJob(tm) as job:
+        with pytest.raises(RuntimeError):
+            job.triples_tiles(h1, h2, h3, p4, p5, p6)
+        job.upload(b)
+        s, d = np.zeros(shape), np.zeros(shape)
+        for _ in range(2):
+            assert L.tce_t_job_triples_tiles(job._h, 0, h1, h2, h3, p4, p5, p6,
+                                             s.ctypes.data_as(DP), d.ctypes.data_as(DP)) == 0
+        gs, gd = job.triples_tiles(h1, h2, h3, p4, p5, p6)
+        assert np.array_equal(s, 2 * gs) and np.array_equal(d, 2 * gd)
+        assert L.tce_t_job_triples_tiles(job._h, 0, h1, h2, h3, p4, p5, p6, None,
+                                         d.ctypes.data_as(DP)) == 0
+        assert np.array_equal(d, 3 * gd)
+        n = tm.noab + tm.nvab
+        for bad in [(h2, h1, h3, p4, p5, p6) if h1 != h2 else (h1, h2, h3, p5 + 1, p5, p6),
+                    (0, h2, h3, p4, p5, p6), (h1, h2, h3, p4, p5, n + 1), (h1, h2, h3, 1, p5, p6)]:
+            with pytest.raises(RuntimeError):
+                job.triples_tiles(*bad)
