@@ -86,6 +86,10 @@ int prepare_mixed(Job &J, int precision);
 // false) or the 8-wide one (wide = true); reduced precision.
 FusedArgs job_args(const Job &J, bool wide);
 MixedArgs job_margs(const Job &J, int precision);
+// Dense tensors of an upload built on the device from the canonical blocks
+// (pack.hip); the dense buffers must be allocated.  Returns a status.
+int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, const long *t2h,
+                   const double *v2, const long *v2h);
 // Free every device buffer of the job (it can be uploaded again).
 void job_release(Job &J);
 // Per-item cost weight (contraction chunks of an item) for scheduling.

@@ -649,11 +649,6 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   if (!job || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
   Job &J = *reinterpret_cast<Job *>(job);
   const Tiles &T = J.T;
-  DenseHost D;
-  int rc = pack_dense(T, J.Ep, J.Mp, t1, t1h, t2, t2h, v2, v2h, D);
-  if (rc != TCE_T_OK) return rc;
-  std::vector<double> &hT1 = D.T1, &hT2e = D.T2e, &hT2m = D.T2m, &hV = D.Vd2, &hW = D.Wd1, &hS = D.S;
-  // ---- device buffers
   auto up = [&](double *&d, const std::vector<double> &h, const char *what) -> bool {
     if (d) (void)hipFree(d);
     d = nullptr;
@@ -663,10 +658,32 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   if (!J.stream) HIPCHK(hipStreamCreateWithFlags(&J.stream, hipStreamNonBlocking), "stream");
   if (!J.ev0) HIPCHK(hipEventCreate(&J.ev0), "event");
   if (!J.ev1) HIPCHK(hipEventCreate(&J.ev1), "event");
-  if (!up(J.dT1, hT1, "T1") || !up(J.dT2e, hT2e, "T2e") || !up(J.dT2m, hT2m, "T2m") ||
-      !up(J.dVd2, hV, "Vd2") || !up(J.dWd1, hW, "Wd1") || !up(J.dS, hS, "S") ||
-      !up(J.devl, T.evl, "evl"))
-    return TCE_T_ERR_HIP;
+  // Dense tensors: fetched and sorted on the device (pack.hip) from the
+  // referenced ranges of the three files; TCE_T_PACK=host builds them with
+  // the host packer instead (same values bit for bit) and copies them over.
+  const char *pk = getenv("TCE_T_PACK");
+  if (pk && !strcmp(pk, "host")) {
+    DenseHost D;
+    const int rc = pack_dense(T, J.Ep, J.Mp, t1, t1h, t2, t2h, v2, v2h, D);
+    if (rc != TCE_T_OK) return rc;
+    if (!up(J.dT1, D.T1, "T1") || !up(J.dT2e, D.T2e, "T2e") || !up(J.dT2m, D.T2m, "T2m") ||
+        !up(J.dVd2, D.Vd2, "Vd2") || !up(J.dWd1, D.Wd1, "Wd1") || !up(J.dS, D.S, "S"))
+      return TCE_T_ERR_HIP;
+  } else {
+    const long no = T.no, nv = T.nv;
+    auto alloc = [&](double *&d, long n, const char *what) -> bool {
+      if (d) (void)hipFree(d);
+      d = nullptr;
+      return hip_ok(hipMalloc(&d, std::max(1L, n) * sizeof(double)), what);
+    };
+    if (!alloc(J.dT1, nv * no, "T1") || !alloc(J.dT2e, no * no * nv * J.Ep, "T2e") ||
+        !alloc(J.dT2m, no * nv * nv * J.Mp, "T2m") || !alloc(J.dVd2, no * nv * nv * J.Ep, "Vd2") ||
+        !alloc(J.dWd1, no * no * nv * J.Mp, "Wd1") || !alloc(J.dS, no * no * nv * nv, "S"))
+      return TCE_T_ERR_HIP;
+    const int rc = pack_on_device(J, t1, t1h, t2, t2h, v2, v2h);
+    if (rc != TCE_T_OK) return rc;
+  }
+  if (!up(J.devl, T.evl, "evl")) return TCE_T_ERR_HIP;
   if (J.dtasks) (void)hipFree(J.dtasks);
   HIPCHK(hipMalloc(&J.dtasks, std::max<size_t>(1, J.tasks.size()) * sizeof(TaskDev)), "tasks");
   HIPCHK(hipMemcpy(J.dtasks, J.tasks.data(), J.tasks.size() * sizeof(TaskDev), hipMemcpyHostToDevice),

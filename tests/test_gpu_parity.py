@@ -642,3 +642,82 @@ def test_triples_tiles_accumulate_and_reject_non_tasks():
                     (0, h2, h3, p4, p5, p6), (h1, h2, h3, p4, p5, n + 1), (h1, h2, h3, 1, p5, p6)]:
             with pytest.raises(RuntimeError):
                 job.triples_tiles(*bad)
+
+
+# ---- upload: block fetch and sort on the device (pack.hip)
+
+@pytest.mark.parametrize("name", ["ragged", "spin_irreps", "restricted", "no7_nv13"])
+def test_device_pack_equals_host_pack(name, monkeypatch):
+    """The dense tensors built on the device from the canonical blocks equal
+    the host packer's bit for bit: identical energies, task energies and
+    task tiles from uploads through either path."""
+    tm, b, ot = _tile_system(name)
+    tasks = tm.tasks()[::7]
+    out = {}
+    for mode in ("host", "device"):
+        if mode == "host":
+            monkeypatch.setenv("TCE_T_PACK", "host")
+        else:
+            monkeypatch.delenv("TCE_T_PACK", raising=False)
+        with TriplesJob(tm) as job:
+            job.upload(b)
+            e = job.run()
+            te = job.task_energies()
+            tiles = [job.triples_tiles(t[3], t[4], t[5], t[0], t[1], t[2]) for t in tasks]
+        out[mode] = (e, te, tiles)
+    h, d = out["host"], out["device"]
+    assert h[0] == d[0] and np.array_equal(h[1], d[1])
+    for (hs, hd), (ds, dd) in zip(h[2], d[2]):
+        assert np.array_equal(hs, ds) and np.array_equal(hd, dd)
+    assert_parity(d[0], d[1], oracle_energies(ot, b) if name in ("ragged", "no7_nv13")
+                  else oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h))
+
+
+def _spread(data, hash_, gap):
+    """The same blocks in a file with `gap` NaNs before every block (a file
+    that holds other data between the blocks the path reads, as the full
+    tce_mo2e_offset file holds other integral classes)."""
+    L = int(hash_[0])
+    offs = hash_[1 + L:1 + 2 * L]
+    order = np.argsort(offs)
+    ends = np.append(np.sort(offs)[1:], data.size)
+    out, new = [], hash_.copy()
+    pos = 0
+    for k, o_end in zip(order, ends):
+        o = offs[k]
+        out.append(np.full(gap, np.nan))
+        pos += gap
+        new[1 + L + k] = pos
+        out.append(data[o:o_end])
+        pos += o_end - o
+    out.append(np.full(gap, np.nan))
+    return np.concatenate(out), new
+
+
+def test_upload_reads_only_the_referenced_blocks():
+    """NaNs between the blocks of all three files change nothing: the upload
+    copies and reads only the blocks the path needs."""
+    tm, b, ot = system(7, 13, 3, 5, 11)
+    ref = oracle_energies(ot, b)
+    t1, t1h = _spread(b.t1, b.t1h, 3)
+    t2, t2h = _spread(b.t2, b.t2h, 5)
+    v2, v2h = _spread(b.v2, b.v2h, 7)
+    assert np.isnan(v2).sum() > 0
+    with TriplesJob(tm) as job:
+        job.upload(tl.Blocks(t1, t1h, t2, t2h, v2, v2h))
+        e = job.run()
+        te = job.task_energies()
+    assert np.isfinite(e).all()
+    assert_parity(e, te, ref)
+
+
+def test_device_pack_reports_missing_block():
+    """A block the spin/irrep rules require but the table lacks is an error
+    of the upload (tce_hash's errquit), before anything reaches the device."""
+    tm, b, ot = system(5, 9, 2, 4, 1)
+    h = b.t2h.copy()
+    L = int(h[0])
+    h[1 + L // 2] += 10**9
+    with TriplesJob(tm) as job:
+        with pytest.raises(RuntimeError, match="missing"):
+            job.upload(tl.Blocks(b.t1, b.t1h, b.t2, h, b.v2, b.v2h))
