@@ -165,7 +165,10 @@ long tce_t_job_precision_bytes(const tce_t_job *job, int precision);
  * canonical TCE blocks and their hash tables [L, keys(1..L), offsets(1..L)].
  * Every block the (T) path reads is fetched through the hash table exactly as
  * get_hash_block (+ tce_sort_2/4 and their signs) would, and re-laid out on
- * the device.  TCE_T_ERR_KEY if a needed key is absent. */
+ * the device: only the file ranges of those blocks are copied (the v2 file
+ * may hold every integral class), the sort runs on the GPU.  TCE_T_PACK=host
+ * in the environment sorts on the host instead (same values).
+ * TCE_T_ERR_KEY if a needed key is absent.  Synchronous. */
 int tce_t_job_upload(tce_t_job *job, const double *t1, const long *k_t1_offset,
                      const double *t2, const long *k_t2_offset,
                      const double *v2, const long *k_v2_offset);
@@ -203,7 +206,7 @@ int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b, l
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);
-/* Host half of tce_t_job_upload, for inspection and tests: the dense tensors
+/* Host packer of tce_t_job_upload, for inspection and tests: the dense tensors
  * the job places in HBM, built from the same blocks and hash tables.  With
  * Ep = no. of virtuals and Mp = no. of occupieds rounded up to a multiple of 8:
  *   T1[nv][no], T2e[no][no][nv][Ep] = t(p e h h'), T2m[no][nv][nv][Mp] =
