@@ -160,7 +160,8 @@ long tce_t_job_device_bytes(const tce_t_job *job);
  * (converted copies of the four contraction tensors); 0 for TCE_T_FP64. */
 long tce_t_job_precision_bytes(const tce_t_job *job, int precision);
 
-/* Upload T1, T2 and V2 once.  Arguments mirror d_t1/k_t1_offset,
+/* Upload T1, T2 and V2 (again: replaces every device buffer of the job,
+ * reduced-precision copies included).  Arguments mirror d_t1/k_t1_offset,
  * d_t2/k_t2_offset, d_v2/k_v2_offset of ccsd_t_gpu.F:2-3: host copies of the
  * canonical TCE blocks and their hash tables [L, keys(1..L), offsets(1..L)].
  * Every block the (T) path reads is fetched through the hash table exactly as
@@ -242,7 +243,9 @@ int tce_t_job_destroy(tce_t_job *job);
 
 /* ---- batches: many uploaded jobs in one persistent launch -------------- */
 /* No counterpart in the reference (its task farm runs one job per call).
- * The jobs stay owned by the caller and must outlive the batch. */
+ * The jobs stay owned by the caller and must outlive the batch; a job
+ * uploaded again after tce_t_batch_create makes tce_t_batch_run return
+ * TCE_T_ERR_STATE (the batch's plans hold the old device buffers). */
 typedef struct tce_t_batch tce_t_batch;
 int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch **batch);
 /* Run every job of the batch in `precision`: energies[2j] = E[T],

@@ -65,6 +65,7 @@ struct Batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double *hE = nullptr;            // pinned: task energies of all jobs
   std::vector<long> eoff;          // per job: first task slot in hE
+  std::vector<long> gens;          // per job: upload generation the plans were made for
   long ntasks = 0;
   Plan plan[3];
   float last_ms = 0.f;
@@ -258,6 +259,7 @@ extern "C" int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch
       return TCE_T_ERR_STATE;
     }
     B->jobs.push_back(J);
+    B->gens.push_back(J->gen);
     B->eoff.push_back(B->ntasks);
     B->ntasks += J->ngroups();
     dev = J->device;
@@ -288,6 +290,13 @@ extern "C" int tce_t_batch_run(tce_t_batch *batch, int precision, double *energi
     return TCE_T_ERR_ARG;
   }
   Batch &B = *reinterpret_cast<Batch *>(batch);
+  // the plans hold the jobs' device pointers: a job uploaded again (or
+  // destroyed) since the batch was made would leave them dangling
+  for (size_t j = 0; j < B.jobs.size(); ++j)
+    if (!B.jobs[j]->uploaded || B.jobs[j]->gen != B.gens[j]) {
+      set_error("tce_t_batch_run: a job was uploaded again or released after tce_t_batch_create");
+      return TCE_T_ERR_STATE;
+    }
   Plan &P = B.plan[precision];
   if (!P.ready) {
     const int rc = plan_batch(B, precision);

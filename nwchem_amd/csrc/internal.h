@@ -78,4 +78,21 @@ inline bool task_allowed(const Tiles &T, long p4b, long p5b, long p6b, long h1b,
   return true;
 }
 
+// Spin and irrep conservation of the block with tiles (a, b, c, d)
+// (tce_t2_offset.F, tce_mo2e_offset.F): stored blocks satisfy it.
+inline bool allowed4(const Tiles &T, long a, long b, long c, long d) {
+  return T.spn(a) + T.spn(b) == T.spn(c) + T.spn(d) && (T.sy(a) ^ T.sy(b) ^ T.sy(c) ^ T.sy(d)) == 0;
+}
+
+// tce_restricted_4 (tce_restricted.F:36-71): in a restricted run an all-beta
+// block is read from its alpha image.
+inline void restrict4(const Tiles &T, long &a, long &b, long &c, long &d) {
+  if (T.restricted && T.spn(a) + T.spn(b) + T.spn(c) + T.spn(d) == 8) {
+    a = T.alp(a);
+    b = T.alp(b);
+    c = T.alp(c);
+    d = T.alp(d);
+  }
+}
+
 }  // namespace tce_t

@@ -39,6 +39,7 @@ struct Job {
          *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
   TaskDev *dtasks = nullptr;
   bool uploaded = false;
+  long gen = 0;  // completed uploads: every upload replaces all device buffers
   float last_ms = 0.f;
   std::vector<double> taskE;
   long Ep = 0, Mp = 0;

@@ -434,20 +434,6 @@ extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
 
 namespace {
 
-// Map a tile tuple through tce_restricted_4 / _2 (identity when the job is
-// unrestricted or the block is not all-beta).
-void restrict4(const Tiles &T, long &a, long &b, long &c, long &d) {
-  if (T.restricted && T.spn(a) + T.spn(b) + T.spn(c) + T.spn(d) == 8) {
-    a = T.alp(a);
-    b = T.alp(b);
-    c = T.alp(c);
-    d = T.alp(d);
-  }
-}
-bool allowed4(const Tiles &T, long a, long b, long c, long d) {
-  return T.spn(a) + T.spn(b) == T.spn(c) + T.spn(d) && (T.sy(a) ^ T.sy(b) ^ T.sy(c) ^ T.sy(d)) == 0;
-}
-
 struct Packer {
   const Tiles &T;
   int err = 0;
@@ -649,6 +635,9 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   if (!job || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
   Job &J = *reinterpret_cast<Job *>(job);
   const Tiles &T = J.T;
+  // a new upload replaces the tensors and everything derived from them
+  // (reduced-precision copies, 8-wide tables, tile buffer)
+  if (J.uploaded) job_release(J);
   auto up = [&](double *&d, const std::vector<double> &h, const char *what) -> bool {
     if (d) (void)hipFree(d);
     d = nullptr;
@@ -701,6 +690,7 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   // persistent grid: 8 waves per CU (2 per SIMD)
   J.grid = std::max(1, cus) * (J.pbox == 8 ? 1 : 8);
   J.uploaded = true;
+  ++J.gen;
   if (J.pbox == 8) {
     J.dtasks8 = J.dtasks;
     J.dpart8 = J.dpart;

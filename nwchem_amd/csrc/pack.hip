@@ -200,18 +200,6 @@ __global__ void __launch_bounds__(PACK_THREADS)
   }
 }
 
-bool allowed4(const Tiles &T, long a, long b, long c, long d) {
-  return T.spn(a) + T.spn(b) == T.spn(c) + T.spn(d) && (T.sy(a) ^ T.sy(b) ^ T.sy(c) ^ T.sy(d)) == 0;
-}
-void restrict4(const Tiles &T, long &a, long &b, long &c, long &d) {
-  if (T.restricted && T.spn(a) + T.spn(b) + T.spn(c) + T.spn(d) == 8) {
-    a = T.alp(a);
-    b = T.alp(b);
-    c = T.alp(c);
-    d = T.alp(d);
-  }
-}
-
 // Block tables of one upload: per tensor, the file offset of the canonical
 // block behind each tile tuple (-1: none), and the extent (doubles) that the
 // gather reads from it.

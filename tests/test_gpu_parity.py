@@ -721,3 +721,41 @@ def test_device_pack_reports_missing_block():
     with TriplesJob(tm) as job:
         with pytest.raises(RuntimeError, match="missing"):
             job.upload(tl.Blocks(b.t1, b.t1h, b.t2, h, b.v2, b.v2h))
+
+
+def test_reupload_replaces_every_copy():
+    """A second upload into the same job replaces the tensors and every copy
+    derived from them (reduced-precision operands): each precision then gives
+    what a fresh job on the new blocks gives, bit for bit."""
+    tm = tl.TileModel.synthetic(8, 32, 8, 16, 1234)
+    b1, b2 = tl.synth_blocks(tm, 1234), tl.synth_blocks(tm, 4321)
+    fresh = {}
+    with TriplesJob(tm) as job:
+        job.upload(b2)
+        for p in (0, 1, 2):
+            fresh[p] = job.run(p)
+    with TriplesJob(tm) as job:
+        job.upload(b1)
+        first = {p: job.run(p) for p in (0, 1, 2)}
+        job.upload(b2)
+        for p in (0, 1, 2):
+            assert job.run(p) == fresh[p], p
+    assert first[0] != fresh[0]
+
+
+def test_batch_refuses_a_reuploaded_job():
+    """A batch's plans hold its jobs' device buffers: after one of its jobs
+    is uploaded again the batch reports an error instead of running on freed
+    memory."""
+    from nwchem_amd import TILING_ORBITAL, TriplesBatch
+    tm = tl.TileModel.synthetic(6, 14, 3, 5, 21)
+    b = tl.synth_blocks(tm, 21)
+    with TriplesJob(tm, tiling=TILING_ORBITAL) as j1, TriplesJob(tm, tiling=TILING_ORBITAL) as j2:
+        j1.upload(b)
+        j2.upload(b)
+        with TriplesBatch([j1, j2]) as batch:
+            e = batch.run(0)
+            assert tuple(e[0]) == tuple(e[1])
+            j2.upload(b)
+            with pytest.raises(RuntimeError):
+                batch.run(0)
