@@ -174,10 +174,17 @@ def config1(threads: int) -> dict | None:
         e = j.run()
         out["gpu"] = {"seconds": time.perf_counter() - t0, "kernel_ms": j.last_kernel_ms,
                       "E_bt": e[0], "E_pt": e[1]}
+        # reduced-precision modes at a physical energy scale (|E| ~ 10 Eh):
+        # |dE| against the FP64 result, in micro-Hartree
+        out["precision_modes_dE_uEh"] = {}
+        for name, p in (("fp32", 1), ("bf16x3", 2)):
+            em = j.run(p)
+            out["precision_modes_dE_uEh"][name] = [(em[0] - e[0]) * 1e6, (em[1] - e[1]) * 1e6]
     out["dE_gpu_vs_cpu"] = [e[0] - out["cpu_1core"]["E_bt"], e[1] - out["cpu_1core"]["E_pt"]]
     print(f"[bench] config 1: reference CPU {out['cpu_1core']['seconds']:.2f} s on 1 core, "
           f"{out['cpu_all_cores']['seconds']:.2f} s on {out['cpu_all_cores']['threads']} threads; "
-          f"GPU {out['gpu']['seconds'] * 1e3:.2f} ms; dE {out['dE_gpu_vs_cpu']}", file=sys.stderr, flush=True)
+          f"GPU {out['gpu']['seconds'] * 1e3:.2f} ms; dE {out['dE_gpu_vs_cpu']}; reduced precision "
+          f"dE (uEh) {out['precision_modes_dE_uEh']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -327,6 +334,8 @@ def main() -> int:
     ap.add_argument("--slice-ms", type=float, default=50.0)
     ap.add_argument("--reference-tiling", type=int, default=1,
                     help="also time the job with the reference tiles as kernel tasks (0: skip)")
+    ap.add_argument("--task-tiles", type=int, default=0,
+                    help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
     ap.add_argument("--modes", default="fp32,bf16x3",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
@@ -398,6 +407,27 @@ def main() -> int:
             "kernel": f"fused_triples_mixed_{name}",
             "extra_device_bytes": job.precision_bytes(prec),
         }
+    tiles_out = None
+    if args.task_tiles > 0:
+        # tce_t_job_triples_tiles: the first tasks' t3s/t3d tiles back on the
+        # host, as a CPU consumer of ccsd_t_singles_l / ccsd_t_doubles_l gets them
+        tk = tm.tasks()[:args.task_tiles]
+        r = tm.range
+        elems = [int(np.prod([r[b - 1] for b in t])) for t in tk]
+        job.triples_tiles(tk[0][3], tk[0][4], tk[0][5], tk[0][0], tk[0][1], tk[0][2])  # warm
+        t1 = time.perf_counter()
+        for t in tk:
+            job.triples_tiles(t[3], t[4], t[5], t[0], t[1], t[2])
+        tw = time.perf_counter() - t1
+        ft = sum(n * 18.0 * (args.no + args.nv + 1) for n in elems)
+        tiles_out = {"tasks": len(tk), "seconds_per_task": tw / len(tk),
+                     "elements_per_task": float(np.mean(elems)),
+                     "task_flops_per_s": ft / tw,
+                     "host_bytes_per_s": 16.0 * sum(elems) / tw}
+        print(f"[bench] task tiles: {tw / len(tk) * 1e3:.1f} ms per task "
+              f"({np.mean(elems):.3g} elements, {ft / tw / 1e12:.1f} TF/s, "
+              f"{16.0 * sum(elems) / tw / 1e9:.1f} GB/s of tiles to the host)",
+              file=sys.stderr, flush=True)
     job.close()
     reftile = None
     if args.reference_tiling:
@@ -472,6 +502,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "config1": cfg1,
         "precision_modes": modes or None,
+        "task_tiles": tiles_out,
         "batch": batch,
         "serving": serve,
     }

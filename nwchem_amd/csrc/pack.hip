@@ -323,9 +323,8 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
   r1.build({&t1tab});
   r2.build({&t2tab});
   rv.build({&wtab, &vtab, &stab});
-  for (Table *t : {&t1tab})
-    for (long &o : t->off)
-      if (o >= 0) o = r1.remap(o);
+  for (long &o : t1tab.off)
+    if (o >= 0) o = r1.remap(o);
   for (long &o : t2tab.off)
     if (o >= 0) o = r2.remap(o);
   for (Table *t : {&wtab, &vtab, &stab})
