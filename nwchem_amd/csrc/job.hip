@@ -1127,16 +1127,62 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
     set_error("tile kernel launch failed");
     return TCE_T_ERR_HIP;
   }
-  std::vector<double> h(2 * (size_t)n);
-  HIPCHK(hipMemcpyAsync(h.data(), M.f.partials, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        J.stream),
-         "D2H");
-  HIPCHK(hipStreamSynchronize(J.stream), "fused_triples_tiles");
-  if (doubles)
-    for (long i = 0; i < n; ++i) doubles[i] += h[i];
-  if (singles)
-    for (long i = 0; i < n; ++i) singles[i] += h[n + i];
-  return TCE_T_OK;
+  // Back to the host in chunks through two pinned buffers: the copy of
+  // chunk k+1 runs while host threads add chunk k into the caller's arrays.
+  // Device layout: doubles [0, n), singles [n, 2n); a NULL array's half is
+  // not copied.
+  const long first = doubles ? 0 : n, last = singles ? 2 * n : n;
+  if (first >= last) {
+    HIPCHK(hipStreamSynchronize(J.stream), "fused_triples_tiles");
+    return TCE_T_OK;
+  }
+  constexpr long CH = 8L << 20;  // doubles per chunk (64 MiB)
+  double *stage = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = TCE_T_OK;
+  if (!hip_ok(hipHostMalloc(&stage, 2 * CH * sizeof(double)), "tile staging") ||
+      !hip_ok(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), "event") ||
+      !hip_ok(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), "event"))
+    rc = TCE_T_ERR_HIP;
+  const long nch = (last - first + CH - 1) / CH;
+  auto issue = [&](long k) {
+    const long off = first + k * CH, len = std::min(CH, last - off);
+    return hip_ok(hipMemcpyAsync(stage + (k & 1) * CH, M.f.partials + off, len * sizeof(double),
+                                 hipMemcpyDeviceToHost, J.stream),
+                  "D2H") &&
+           hip_ok(hipEventRecord(ev[k & 1], J.stream), "event");
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (rc == TCE_T_OK && !issue(0)) rc = TCE_T_ERR_HIP;
+  for (long k = 0; k < nch && rc == TCE_T_OK; ++k) {
+    if (!hip_ok(hipEventSynchronize(ev[k & 1]), "D2H")) {
+      rc = TCE_T_ERR_HIP;
+      break;
+    }
+    if (k + 1 < nch && !issue(k + 1)) rc = TCE_T_ERR_HIP;  // into the buffer of chunk k-1
+    const long off = first + k * CH, len = std::min(CH, last - off);
+    const double *src = stage + (k & 1) * CH;
+    auto add = [&](long a, long b) {  // chunk positions [a, b)
+      for (long i = a; i < b; ++i) {
+        const long g = off + i;
+        if (g < n)
+          doubles[g] += src[i];
+        else
+          singles[g - n] += src[i];
+      }
+    };
+    std::vector<std::thread> th;
+    const long part = (len + nt - 1) / nt;
+    for (unsigned t = 1; t < nt && (long)t * part < len; ++t)
+      th.emplace_back(add, (long)t * part, std::min(len, (long)(t + 1) * part));
+    add(0, std::min(len, part));
+    for (auto &x : th) x.join();
+  }
+  if (rc != TCE_T_OK) (void)hipStreamSynchronize(J.stream);  // no copy left in flight
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stage) (void)hipHostFree(stage);
+  return rc;
 }
 
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {

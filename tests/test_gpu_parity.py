@@ -637,6 +637,9 @@ def test_triples_tiles_accumulate_and_reject_non_tasks():
         assert L.tce_t_job_triples_tiles(job._h, 0, h1, h2, h3, p4, p5, p6, None,
                                          d.ctypes.data_as(DP)) == 0
         assert np.array_equal(d, 3 * gd)
+        assert L.tce_t_job_triples_tiles(job._h, 0, h1, h2, h3, p4, p5, p6,
+                                         s.ctypes.data_as(DP), None) == 0
+        assert np.array_equal(s, 3 * gs) and np.array_equal(d, 3 * gd)
         n = tm.noab + tm.nvab
         for bad in [(h2, h1, h3, p4, p5, p6) if h1 != h2 else (h1, h2, h3, p5 + 1, p5, p6),
                     (0, h2, h3, p4, p5, p6), (h1, h2, h3, p4, p5, n + 1), (h1, h2, h3, 1, p5, p6)]:
@@ -759,3 +762,21 @@ def test_batch_refuses_a_reuploaded_job():
             j2.upload(b)
             with pytest.raises(RuntimeError):
                 batch.run(0)
+
+
+def test_triples_tiles_larger_than_the_host_staging_chunk():
+    """One task of 4.7e6 elements (one tile of 8 holes, one of 21
+    particles): doubles and singles (9.5e6 doubles) come back through the
+    pinned staging in two chunks of 8 Mi doubles, the boundary inside the
+    singles half; both halves against the oracle."""
+    tm, b, ot = system(8, 21, 8, 21, 31)
+    tasks = tm.tasks()
+    assert len(tasks) == 1
+    p4, p5, p6, h1, h2, h3 = tasks[0]
+    rs, rd = oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, tasks[0])
+    assert 2 * rd.size > 8 << 20
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        gs, gd = job.triples_tiles(h1, h2, h3, p4, p5, p6)
+    assert np.abs(gd - rd).max() <= 1e-12 * np.abs(rd).max()
+    assert np.abs(gs - rs).max() <= 1e-12 * np.abs(rs).max()
