@@ -334,7 +334,7 @@ def main() -> int:
     ap.add_argument("--slice-ms", type=float, default=50.0)
     ap.add_argument("--reference-tiling", type=int, default=1,
                     help="also time the job with the reference tiles as kernel tasks (0: skip)")
-    ap.add_argument("--task-tiles", type=int, default=0,
+    ap.add_argument("--task-tiles", type=int, default=1,
                     help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
     ap.add_argument("--modes", default="fp32,bf16x3",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
