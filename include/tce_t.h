@@ -154,7 +154,8 @@ int tce_t_job_create(const tce_t_tiles *tiles, tce_t_job **job);
 int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce_t_job **job);
 int tce_t_job_tiling(const tce_t_job *job);
 long tce_t_job_num_tasks(const tce_t_job *job);
-/* Device bytes the job holds once uploaded (admission control). */
+/* Device bytes the job holds once uploaded (admission control), the tile
+ * export buffer included. */
 long tce_t_job_device_bytes(const tce_t_job *job);
 /* Additional device bytes the first run in a reduced precision allocates
  * (converted copies of the four contraction tensors); 0 for TCE_T_FP64. */
@@ -201,7 +202,9 @@ int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin, double *ta
  * 1-based (holes 1..noab, particles noab+1..), h1b <= h2b <= h3b,
  * p4b <= p5b <= p6b, and the sextuple must pass the task loop's spin, irrep
  * and restricted tests (else TCE_T_ERR_ARG).  Any tiling; d1/d2 products in
- * `precision`.  Synchronous. */
+ * `precision`.  Synchronous.  The job keeps a device buffer of the largest
+ * tile pair exported (16 bytes per element) until it is released or
+ * uploaded again; tce_t_job_device_bytes counts it. */
 int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b, long h3b,
                             long p4b, long p5b, long p6b, double *singles, double *doubles);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from

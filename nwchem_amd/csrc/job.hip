@@ -429,7 +429,7 @@ extern "C" long tce_t_job_device_bytes(const tce_t_job *job) {
   if (!job) return -1;
   const Job &J = *reinterpret_cast<const Job *>(job);
   return dense_bytes(J) + J.nitems * J.ppi * 16 + (long)J.tasks.size() * sizeof(TaskDev) +
-         (J.ngroups() + 1) * 8 + J.ngroups() * 16 + 64;
+         (J.ngroups() + 1) * 8 + J.ngroups() * 16 + 64 + (long)J.dtile_bytes;
 }
 
 namespace {
