@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
+#include <thread>
 #include <vector>
 
 #include "job.h"
@@ -359,19 +361,60 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
   char *d = nullptr;
   HIPCHK(hipMalloc(&d, std::max<size_t>(bytes, 256)), "upload staging");
   int rc = TCE_T_OK;
-  auto copy_runs = [&](const Runs &R, const double *file, size_t at) {
-    for (size_t k = 0; k < R.start.size() && rc == TCE_T_OK; ++k)
-      if (!hip_ok(hipMemcpyAsync(d + at + R.base[k] * 8, file + R.start[k],
-                                 (R.end[k] - R.start[k]) * 8, hipMemcpyHostToDevice, J.stream),
-                  "upload blocks"))
-        rc = TCE_T_ERR_HIP;
+  // The referenced ranges of a file form one stream of R.total doubles (the
+  // runs in order, at device offsets base[k]); it goes to the device through
+  // two pinned buffers: host threads fill one while the DMA of the other runs.
+  constexpr long CH = 8L << 20;  // doubles per staging buffer (64 MiB)
+  double *stage = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (!hip_ok(hipHostMalloc(&stage, 2 * CH * sizeof(double)), "upload staging") ||
+      !hip_ok(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), "event") ||
+      !hip_ok(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), "event"))
+    rc = TCE_T_ERR_HIP;
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  long piece = 0;  // staging buffers used so far (alternating)
+  // copy stream positions [p0, p0 + n) of file runs R into dst
+  auto gather = [](const Runs &R, const double *file, long p0, long n, double *dst) {
+    size_t k = std::upper_bound(R.base.begin(), R.base.end(), p0) - R.base.begin() - 1;
+    while (n > 0) {
+      const long in = p0 - R.base[k], len = std::min(n, R.end[k] - R.start[k] - in);
+      std::memcpy(dst, file + R.start[k] + in, len * sizeof(double));
+      dst += len;
+      p0 += len;
+      n -= len;
+      ++k;
+    }
   };
-  if (!hip_ok(hipMemcpyAsync(d, head.data(), head.size(), hipMemcpyHostToDevice, J.stream),
+  auto stream_file = [&](const Runs &R, const double *file, size_t at) {
+    for (long p0 = 0; p0 < R.total && rc == TCE_T_OK; p0 += CH, ++piece) {
+      const long n = std::min(CH, R.total - p0);
+      double *buf = stage + (piece & 1) * CH;
+      // the DMA that last read this buffer must be done
+      if (piece >= 2 && !hip_ok(hipEventSynchronize(ev[piece & 1]), "upload blocks")) {
+        rc = TCE_T_ERR_HIP;
+        break;
+      }
+      std::vector<std::thread> th;
+      const long part = (n + nt - 1) / nt;
+      for (unsigned t = 1; t < nt && (long)t * part < n; ++t)
+        th.emplace_back(gather, std::cref(R), file, p0 + (long)t * part,
+                        std::min(n, (long)(t + 1) * part) - (long)t * part, buf + (long)t * part);
+      gather(R, file, p0, std::min(n, part), buf);
+      for (auto &x : th) x.join();
+      if (!hip_ok(hipMemcpyAsync(d + at + p0 * 8, buf, n * sizeof(double), hipMemcpyHostToDevice,
+                                 J.stream),
+                  "upload blocks") ||
+          !hip_ok(hipEventRecord(ev[piece & 1], J.stream), "event"))
+        rc = TCE_T_ERR_HIP;
+    }
+  };
+  if (rc == TCE_T_OK &&
+      !hip_ok(hipMemcpyAsync(d, head.data(), head.size(), hipMemcpyHostToDevice, J.stream),
               "upload tables"))
     rc = TCE_T_ERR_HIP;
-  copy_runs(r1, t1, f1_at);
-  copy_runs(r2, t2, f2_at);
-  copy_runs(rv, v2, fv_at);
+  stream_file(r1, t1, f1_at);
+  stream_file(r2, t2, f2_at);
+  stream_file(rv, v2, fv_at);
   if (rc == TCE_T_OK) {
     PackGeo G;
     G.tile = (const int *)d;
@@ -394,8 +437,11 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
     hipLaunchKernelGGL(pack_s, grid, blk, 0, J.stream, G, tb[4], fv, J.dS);
     if (!hip_ok(hipGetLastError(), "pack launch")) rc = TCE_T_ERR_HIP;
   }
-  // the staging copies read host memory until they complete
+  // the copies read the staging buffers (and `head`) until they complete
   if (!hip_ok(hipStreamSynchronize(J.stream), "pack")) rc = TCE_T_ERR_HIP;
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stage) (void)hipHostFree(stage);
   (void)hipFree(d);
   return rc;
 }
