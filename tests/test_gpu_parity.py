@@ -780,3 +780,37 @@ def test_triples_tiles_larger_than_the_host_staging_chunk():
         gs, gd = job.triples_tiles(h1, h2, h3, p4, p5, p6)
     assert np.abs(gd - rd).max() <= 1e-12 * np.abs(rd).max()
     assert np.abs(gs - rs).max() <= 1e-12 * np.abs(rs).max()
+
+
+def test_full_size_tilings_agree():
+    """no=40, nv=320 in tiles of 20 / 40 (480 tasks), the benchmark's size,
+    where the oracle cannot run: size-independent properties instead.  The
+    default sub-tiles, the reference tiles and the orbital tiling decompose
+    the same sum differently (different items, factors and padding): E[T]
+    and E(T) agree to 1e-12 relative, every reference task's energy agrees
+    between the sub-tiled and the reference-tiled job, and split-BF16 stays
+    within its bound of the FP64 result."""
+    from nwchem_amd import TILING_ORBITAL, TILING_REFERENCE
+    tm = tl.TileModel.synthetic(40, 320, 20, 40, 7)
+    b = tl.synth_blocks(tm, 7)
+    print("full size: 480 tasks, sub-tiles", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        e = job.run()
+        te = job.task_energies()
+        eb = job.run(2)
+    print("full size: reference tiles", flush=True)
+    with TriplesJob(tm, tiling=TILING_REFERENCE) as job:
+        job.upload(b)
+        er = job.run()
+        ter = job.task_energies()
+    print("full size: orbital tiling", flush=True)
+    with TriplesJob(tm, tiling=TILING_ORBITAL) as job:
+        job.upload(b)
+        eo = job.run()
+    assert te.shape == ter.shape == (480, 2)
+    for k in (0, 1):
+        assert abs(e[k] - er[k]) <= 1e-12 * abs(er[k]), (e, er)
+        assert abs(e[k] - eo[k]) <= 1e-12 * abs(er[k]), (e, eo)
+        assert abs(eb[k] - e[k]) <= MIXED_REL[2] * abs(e[k]), (eb, e)
+    assert np.all(np.abs(te - ter) <= 1e-12 * np.abs(ter) + 1e-9)
