@@ -68,21 +68,67 @@ def traffic_from_profiles(workload: str, kernel: str):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+    if "hbm_bytes_per_launch" in d:
+        return d["hbm_bytes_per_launch"]
+    # {kernel: figures} per kernel of the run, keyed by the template name
+    # (fused_triples_mixed_fp64 is fused_triples_mixed<0, false>)
+    base, _, prec = kernel.rpartition("_")
+    key = f"{base}<{ {'fp64': 0, 'fp32': 1, 'bf16x3': 2}.get(prec, -1)}, false>"
+    v = d.get(key)
+    return v.get("hbm_bytes_per_launch") if isinstance(v, dict) else None
 
 
-def cpu_baseline(tm, threads: int, calls=(("d2", 96), ("d1", 24), ("s1", 12))):
+def host_cpu() -> dict:
+    """The host's CPU model and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cpus = len(os.sched_getaffinity(0))
+    # physical cores among them: distinct (package, core id) pairs
+    cores = set()
+    for c in os.sched_getaffinity(0):
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", c))
+    return {"model": model, "cpus_in_affinity": cpus, "physical_cores_in_affinity": len(cores)}
+
+
+def cpu_threads_default() -> int:
+    """All physical cores this process may use, capped by OMP_NUM_THREADS when
+    the environment sets it (a shared host's share of its CPUs)."""
+    n = host_cpu()["physical_cores_in_affinity"]
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(tm, threads: int):
     """The reference's own Fortran kernels (oracle/_ref/libsdt_ref_omp.so,
-    ccsd_t_kernels_omp.F built -O3 -fopenmp) timed on the host at the exact
-    per-call shapes of the job's first task: `calls` lists how many calls of
-    each family run (d2 with K = one p7 tile, d1 with K = one h7 tile), the
-    mix of the task's 72 d2 / 18 d1 / 9 s1 calls in miniature.  A full task
-    streams its t3 tile 99 times and takes minutes on the host, so the sample
-    is bounded to a few calls; the rate is F_ref flops of the sampled calls
-    over their wall time."""
+    ccsd_t_kernels_omp.F built -O3 -fopenmp) timed on the host over one full
+    task of the job at its exact per-call shapes: all 27 variants, each d2
+    variant once per p7 tile (K = that tile), each d1 variant once per h7
+    tile, each s1 variant once -- the 9 nvab + 9 noab + 9 calls the
+    ccsd_t_gpu.F task loop makes for the job's first task (ccsd_t_doubles_gpu.F,
+    ccsd_t_singles_gpu.F).  The rate is that task's F_ref flops over the wall
+    time of its calls; OMP_PROC_BIND=close, OMP_PLACES=cores."""
     import ctypes as C
 
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    hc = host_cpu()  # before the OpenMP runtime binds this thread to one core
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
 
     try:
@@ -93,47 +139,51 @@ def cpu_baseline(tm, threads: int, calls=(("d2", 96), ("d1", 24), ("s1", 12))):
     p4, p5, p6, h1, h2, h3 = tm.tasks()[0]
     r = tm.range
     dims = [int(r[h3 - 1]), int(r[h2 - 1]), int(r[h1 - 1]), int(r[p6 - 1]), int(r[p5 - 1]), int(r[p4 - 1])]
+    h3d, h2d, h1d, p6d, p5d, p4d = dims
     n3 = int(np.prod(dims))
-    k_p = int(r[tm.noab])  # one particle tile
-    k_h = int(r[0])        # one hole tile
+    ks_p = [int(r[k]) for k in range(tm.noab, tm.noab + tm.nvab)]  # one d2 call per p7 tile
+    ks_h = [int(r[k]) for k in range(tm.noab)]                     # one d1 call per h7 tile
     rng = np.random.default_rng(0)
     t3 = np.zeros(n3)
     t3 += 1e-3  # first touch outside the timed region
     DP = C.POINTER(C.c_double)
-    flops = 0.0
-    t_tot = 0.0
-    for fam, ncalls in calls:
-        K = {"d2": k_p, "d1": k_h, "s1": 1}[fam]
-        h3d, h2d, h1d, p6d, p5d, p4d = dims
-        if fam == "s1":
-            a = rng.standard_normal(p4d * h1d)
-            b = rng.standard_normal(h3d * h2d * p6d * p5d)
-            args = [C.c_long(v) for v in dims]
-        elif fam == "d1":
-            a = rng.standard_normal(K * p4d * p5d * h1d)
-            b = rng.standard_normal(h3d * h2d * p6d * K)
-            args = [C.c_long(v) for v in dims + [K]]
-        else:
-            a = rng.standard_normal(K * p4d * h1d * h2d)
-            b = rng.standard_normal(K * h3d * p6d * p5d)
-            args = [C.c_long(v) for v in dims + [K]]
-        fn = getattr(R, f"sd_t_{fam}_1_")
-        for _ in range(ncalls):
-            t0 = time.perf_counter()
-            fn(*[C.byref(v) for v in args], t3.ctypes.data_as(DP), a.ctypes.data_as(DP),
-               b.ctypes.data_as(DP))
-            t_tot += time.perf_counter() - t0
-            flops += 2.0 * n3 * K
+    kp, kh = max(ks_p), max(ks_h)
+    ops = {"s1": (rng.standard_normal(p4d * h1d), rng.standard_normal(h3d * h2d * p6d * p5d)),
+           "d1": (rng.standard_normal(kh * p4d * p5d * h1d), rng.standard_normal(h3d * h2d * p6d * kh)),
+           "d2": (rng.standard_normal(kp * p4d * h1d * h2d), rng.standard_normal(kp * h3d * p6d * p5d))}
+    calls = {"s1": [1], "d1": ks_h, "d2": ks_p}
+    flops, n_calls = 0.0, 0
+    t0 = time.perf_counter()
+    for fam in ("d2", "d1", "s1"):
+        a, b = ops[fam]
+        for v in range(1, 10):
+            fn = getattr(R, f"sd_t_{fam}_{v}_")
+            for K in calls[fam]:
+                args = [C.c_long(x) for x in (dims if fam == "s1" else dims + [K])]
+                fn(*[C.byref(x) for x in args], t3.ctypes.data_as(DP), a.ctypes.data_as(DP),
+                   b.ctypes.data_as(DP))
+                flops += 2.0 * n3 * K
+                n_calls += 1
+    t_tot = time.perf_counter() - t0
     rate = flops / t_tot
     return {
         "value": rate / 1e12,
         "unit": "TFLOP/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "cores": threads,
         "kind": kind,
-        "sample": "reference Fortran sd_t_d2_1 x16 (K=%d), sd_t_d1_1 x4 (K=%d), sd_t_s1_1 x2 at the "
-                  "t3 tile of task 0 (%d elements), %.3e flops in %.1f s; the full job at this "
-                  "rate would take %.0f s" % (k_p, k_h, n3, flops, t_tot, f_ref(tm) / rate),
+        "sample": "one full task (the first) of this job on the reference Fortran kernels: "
+                  "sd_t_d2_1..9 x %d p7 tiles (K=%s), sd_t_d1_1..9 x %d h7 tiles (K=%s), "
+                  "sd_t_s1_1..9 once, %d calls on a t3 tile of %d elements, %.4e flops in %.1f s "
+                  "on %d OpenMP threads; the full job (%d tasks of this shape) at this rate "
+                  "would take %.0f s" % (len(ks_p), "/".join(map(str, sorted(set(ks_p)))), len(ks_h),
+                                         "/".join(map(str, sorted(set(ks_h)))), n_calls, n3, flops,
+                                         t_tot, threads, len(tm.tasks()), f_ref(tm) / rate),
         "seconds": t_tot,
+        "cpu_model": hc["model"],
+        "host_cpus_in_affinity": hc["cpus_in_affinity"],
+        "host_physical_cores_in_affinity": hc["physical_cores_in_affinity"],
+        "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                "OMP_PLACES": os.environ.get("OMP_PLACES")},
     }
 
 
@@ -325,7 +375,9 @@ def main() -> int:
     ap.add_argument("--tile-v", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the reference CPU kernels (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0: all physical cores of this "
+                         "process's affinity, capped by OMP_NUM_THREADS when set)")
     ap.add_argument("--batch-jobs", type=int, default=16,
                     help="batch of independent jobs (config 4 has 64; 0: skip)")
     ap.add_argument("--serve", type=int, default=24, help="serving requests (0: skip)")
@@ -468,10 +520,11 @@ def main() -> int:
     cpu = None
     cfg1 = None
     if args.cpu_baseline:
-        print(f"[bench] cpu baseline on {args.cpu_threads} threads", file=sys.stderr, flush=True)
+        threads = args.cpu_threads or cpu_threads_default()
+        print(f"[bench] cpu baseline on {threads} threads ({host_cpu()})", file=sys.stderr, flush=True)
         del blocks
-        cpu = cpu_baseline(tm, args.cpu_threads)
-        cfg1 = config1(args.cpu_threads)
+        cpu = cpu_baseline(tm, threads)
+        cfg1 = config1(threads)
         if cpu:
             print(f"[bench] cpu baseline {cpu['value']*1e3:.2f} GFLOP/s ({cpu['seconds']:.1f}s)",
                   file=sys.stderr, flush=True)

@@ -331,10 +331,6 @@ int tce_t_synth_dense(unsigned long long seed, long no, long nv, double *T1,
  * codes: TCE_T_ERR_HIP if any call failed since the last query, then reset. */
 int tce_t_legacy_status(void);
 
-/* Counter calibration: `repeats` launches of a plain grid-stride copy of n
- * doubles (8n bytes read, 8n written per launch) on device 0. */
-int tce_t_copy_probe(long n, int repeats);
-
 /* Philox4x32-10 block function (known-answer tests). */
 void tce_t_philox4x32(const unsigned int ctr[4], const unsigned int key[2],
                       unsigned int out[4]);

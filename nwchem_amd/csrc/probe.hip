@@ -1,7 +1,9 @@
-// probe.hip -- calibration kernel for HBM-byte counters.  A plain grid-stride
-// copy of n doubles moves exactly 8n bytes in and 8n bytes out; profiling it
-// with the same FETCH_SIZE / WRITE_SIZE counters as the (T) kernel gives the
-// factor that turns counter readings into bytes.
+// probe.hip -- calibration kernel for HBM-byte counters, built as its own
+// measurement library (libtce_t_probe.so, loaded by profiling.py), not part
+// of libtce_t.so.  A plain grid-stride copy of n doubles moves exactly 8n
+// bytes in and 8n bytes out; profiling it with the same FETCH_SIZE /
+// WRITE_SIZE counters as the (T) kernel gives the factor that turns counter
+// readings into bytes.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -15,7 +17,7 @@ __global__ void __launch_bounds__(256) stream_copy(const double *__restrict__ sr
 }
 }  // namespace
 
-extern "C" int tce_t_copy_probe(long n, int repeats) {
+extern "C" int probe_stream_copy(long n, int repeats) {
   if (n <= 0 || repeats <= 0) return TCE_T_ERR_ARG;
   double *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, n * sizeof(double)) != hipSuccess) return TCE_T_ERR_NOMEM;

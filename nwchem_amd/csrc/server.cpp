@@ -33,6 +33,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -55,10 +56,12 @@ struct System {
 struct Request {
   long id = 0;
   int sys = 0, precision = 0;
+  System *sy = nullptr;  // set at submit under the lock; Systems never move
   double t_submit = 0, t_admit = -1, t_start = -1, t_finish = -1;
   long cursor = 0, nitems = 0;
   double *dpart = nullptr, *dtaskE = nullptr;
-  double e4 = 0, e5 = 0;
+  double e4 = 0, e5 = 0;          // published with status (under the lock)
+  double e4_out = 0, e5_out = 0;  // the worker's reduction (worker only)
   int status = 1;  // 1 pending, 0 done, < 0 error
 };
 
@@ -72,7 +75,10 @@ struct Server {
   Clock::time_point t0 = Clock::now();
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::vector<System> systems;
+  // heap-allocated so a System never moves when another is added: the
+  // worker uses a request's System through Request::sy without the lock,
+  // while tce_t_server_add_system may grow this vector (under the lock)
+  std::vector<std::unique_ptr<System>> systems;
   std::deque<long> queue;
   std::vector<long> active;
   std::map<long, Request> reqs;
@@ -90,13 +96,13 @@ struct Server {
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
-Job &job_of(Server &S, int sys) { return *reinterpret_cast<Job *>(S.systems[sys].job); }
+Job &job_of(const Request &r) { return *reinterpret_cast<Job *>(r.sy->job); }
 
 long request_bytes(const Job &J) { return std::max(1L, J.nitems8) * 8 * 16 + J.ngroups() * 16; }
 
 // Bytes request r would add to the device (caller holds the lock).
-long admission_bytes(Server &S, const Request &r) {
-  Job &J = job_of(S, r.sys);
+long admission_bytes(const Request &r) {
+  Job &J = job_of(r);
   long need = request_bytes(J);
   if (!J.uploaded) need += tce_t_job_device_bytes(reinterpret_cast<tce_t_job *>(&J));
   if (r.precision != TCE_T_FP64 && !(J.uploaded && J.mx[r.precision].ready))
@@ -110,9 +116,9 @@ void fail_request(Server &S, Request &r, int rc) {
 }
 
 // Make request r active: upload / convert / allocate (called without the lock).
-int activate(Server &S, Request &r) {
-  System &sy = S.systems[r.sys];
-  Job &J = job_of(S, r.sys);
+int activate(Request &r) {
+  System &sy = *r.sy;
+  Job &J = job_of(r);
   if (!J.uploaded) {
     int rc = tce_t_job_upload(sy.job, sy.t1, sy.t1h, sy.t2, sy.t2h, sy.v2, sy.v2h);
     if (rc != TCE_T_OK) return rc;
@@ -128,19 +134,19 @@ int activate(Server &S, Request &r) {
 
 // Evict idle resident systems (LRU) until `need` bytes fit; caller holds the
 // lock, no kernel is in flight (the worker is between slices).
-bool make_room(Server &S, long need, int keep) {
+bool make_room(Server &S, long need, const System *keep) {
   for (;;) {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
     if ((long)fr - need >= S.reserve) return true;
-    int victim = -1;
-    for (int i = 0; i < (int)S.systems.size(); ++i) {
-      const System &sy = S.systems[i];
-      if (i == keep || sy.active > 0 || !reinterpret_cast<Job *>(sy.job)->uploaded) continue;
-      if (victim < 0 || sy.last_use < S.systems[victim].last_use) victim = i;
+    System *victim = nullptr;
+    for (const auto &p : S.systems) {
+      System &sy = *p;
+      if (&sy == keep || sy.active > 0 || !reinterpret_cast<Job *>(sy.job)->uploaded) continue;
+      if (!victim || sy.last_use < victim->last_use) victim = &sy;
     }
-    if (victim < 0) return false;
-    job_release(job_of(S, victim));
+    if (!victim) return false;
+    job_release(*reinterpret_cast<Job *>(victim->job));
   }
 }
 
@@ -161,7 +167,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
                   // allowed tasks has none and finishes without a launch)
   for (size_t k = 0; k < n; ++k) {
     Request &r = *rs[k];
-    const Job &J = job_of(S, r.sys);
+    const Job &J = job_of(r);
     const long left = r.nitems - r.cursor;
     if (left <= 0) continue;
     const double c = item_cost(J, precision);
@@ -204,7 +210,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
 
 // Reduce a finished request's partials and read its energies.
 int finish(Server &S, Request &r) {
-  Job &J = job_of(S, r.sys);
+  Job &J = job_of(r);
   const long nt = J.ngroups();
   std::vector<double> te(2 * std::max(1L, nt));
   if (nt > 0) {
@@ -222,8 +228,8 @@ int finish(Server &S, Request &r) {
     s4 += te[2 * t];
     s5 += te[2 * t + 1];
   }
-  r.e4 = s4;
-  r.e5 = s5;
+  r.e4_out = s4;
+  r.e5_out = s5;
   return TCE_T_OK;
 }
 
@@ -242,17 +248,17 @@ void worker_loop(Server &S) {
     // ---- admission (first come, first served)
     while (!S.queue.empty() && (int)S.active.size() < S.max_active) {
       Request &r = S.reqs[S.queue.front()];
-      if (!make_room(S, admission_bytes(S, r), r.sys)) break;
+      if (!make_room(S, admission_bytes(r), r.sy)) break;
       S.queue.pop_front();
-      S.systems[r.sys].active++;
-      S.systems[r.sys].last_use = ++S.uses;
+      r.sy->active++;
+      r.sy->last_use = ++S.uses;
       r.t_admit = S.now();
       lk.unlock();
-      const int rc = activate(S, r);
+      const int rc = activate(r);
       lk.lock();
       if (rc != TCE_T_OK) {
         release_request(r);
-        S.systems[r.sys].active--;
+        r.sy->active--;
         fail_request(S, r, rc);
         S.cv_done.notify_all();
         continue;
@@ -282,23 +288,25 @@ void worker_loop(Server &S) {
       float ms = 0;
       rc = run_slice(S, p, group[p], ms);
     }
-    // ---- completions
+    // ---- completions: reduce without the lock, publish under it (a waiter
+    // reads status, energies and times under the lock)
     std::vector<Request *> done;
+    std::vector<int> frc;
     for (int p = 0; p < 3; ++p)
       for (Request *r : group[p])
         if (rc != TCE_T_OK || r->cursor >= r->nitems) done.push_back(r);
     for (Request *r : done) {
-      const int frc = rc != TCE_T_OK ? rc : finish(S, *r);
+      frc.push_back(rc != TCE_T_OK ? rc : finish(S, *r));
       release_request(*r);
-      if (frc != TCE_T_OK)
-        r->status = frc;
-      else
-        r->status = 0;
-      r->t_finish = S.now();
     }
     lk.lock();
-    for (Request *r : done) {
-      S.systems[r->sys].active--;
+    for (size_t k = 0; k < done.size(); ++k) {
+      Request *r = done[k];
+      r->e4 = r->e4_out;
+      r->e5 = r->e5_out;
+      r->t_finish = S.now();
+      r->status = frc[k] != TCE_T_OK ? frc[k] : 0;
+      r->sy->active--;
       S.active.erase(std::find(S.active.begin(), S.active.end(), r->id));
     }
     if (!done.empty()) S.cv_done.notify_all();
@@ -343,16 +351,16 @@ extern "C" int tce_t_server_add_system(tce_t_server *server, const tce_t_tiles *
                                        const long *t2h, const double *v2, const long *v2h) {
   if (!server || !tiles || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
   Server &S = *reinterpret_cast<Server *>(server);
-  System sy;
+  auto sy = std::make_unique<System>();
   // requests return E[T] and E(T) only: the orbital tiling (least padding)
   // unless TCE_T_SUBTILE names another
   const int rc = getenv("TCE_T_SUBTILE")
-                     ? tce_t_job_create(tiles, &sy.job)
-                     : tce_t_job_create_tiling(tiles, TCE_T_TILING_ORBITAL, &sy.job);
+                     ? tce_t_job_create(tiles, &sy->job)
+                     : tce_t_job_create_tiling(tiles, TCE_T_TILING_ORBITAL, &sy->job);
   if (rc != TCE_T_OK) return rc;
-  sy.t1 = t1, sy.t1h = t1h, sy.t2 = t2, sy.t2h = t2h, sy.v2 = v2, sy.v2h = v2h;
+  sy->t1 = t1, sy->t1h = t1h, sy->t2 = t2, sy->t2h = t2h, sy->v2 = v2, sy->v2h = v2h;
   std::lock_guard<std::mutex> lk(S.mu);
-  S.systems.push_back(sy);
+  S.systems.push_back(std::move(sy));
   return (int)S.systems.size() - 1;
 }
 
@@ -366,6 +374,7 @@ extern "C" long tce_t_server_submit(tce_t_server *server, int system, int precis
   Request r;
   r.id = S.next_id++;
   r.sys = system;
+  r.sy = S.systems[system].get();
   r.precision = precision;
   r.t_submit = S.now();
   S.reqs[r.id] = r;
@@ -405,7 +414,7 @@ extern "C" int tce_t_server_stats(tce_t_server *server, long *slices, int *resid
   if (slices) *slices = S.slices;
   if (resident) {
     int n = 0;
-    for (const System &sy : S.systems) n += reinterpret_cast<Job *>(sy.job)->uploaded ? 1 : 0;
+    for (const auto &sy : S.systems) n += reinterpret_cast<Job *>(sy->job)->uploaded ? 1 : 0;
     *resident = n;
   }
   return TCE_T_OK;
@@ -420,7 +429,7 @@ extern "C" int tce_t_server_destroy(tce_t_server *server) {
   }
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
-  for (System &sy : S->systems) tce_t_job_destroy(sy.job);
+  for (auto &sy : S->systems) tce_t_job_destroy(sy->job);
   if (S->tab_dev) (void)hipFree(S->tab_dev);
   if (S->tab_host) (void)hipHostFree(S->tab_host);
   if (S->ev0) (void)hipEventDestroy(S->ev0);

@@ -95,8 +95,12 @@ def lib() -> C.CDLL:
     return L
 
 
-def ref_lib(omp: bool = False) -> C.CDLL:
-    """The reference's Fortran kernels (oracle/_ref)."""
+def ref_lib(omp: bool | str = False) -> C.CDLL:
+    """The reference's Fortran kernels (oracle/_ref): omp=False the serial
+    strict build, True the -O3 -fopenmp build, "par" the strict-FP OpenMP
+    build."""
+    if omp == "par":
+        return _load("_ref/libsdt_ref_par.so")
     return _load("_ref/libsdt_ref_omp.so" if omp else "_ref/libsdt_ref.so")
 
 
@@ -142,9 +146,9 @@ class Tiles:
         return int(lib().ora_count_tasks(C.byref(self.c)))
 
 
-def ref_backend(omp: bool = False) -> OraBackend:
+def ref_backend(omp: bool | str = False) -> OraBackend:
     """Kernel back-end made of the reference's compiled Fortran kernels
-    (CPU argument order, usedevice = 0); omp: the -O3 -fopenmp build."""
+    (CPU argument order, usedevice = 0); omp: as ref_lib."""
     R = ref_lib(omp)
     be = OraBackend()
     be.usedevice = 0
