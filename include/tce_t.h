@@ -86,6 +86,25 @@ void sd_t_d2_9_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long
 void compute_en_(double *factor, double *energy, double *eval1, double *eval2, double *eval3, double *eval4, double *eval5, double *eval6,
                  long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, double *host1, double *host2);
 
+/* No counterpart in the reference: compute_en_ for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * other harnesses that forbid hipMalloc).  Same energy as compute_en_ (same
+ * kernels and reduction order): t3d / t3s are the task's doubles / singles
+ * tiles in DEVICE memory (h3 fastest, h2, h1, p6, p5, p4 slowest), eval1..6
+ * the orbital energies of tiles h1, h2, h3, p4, p5, p6 in DEVICE memory, and
+ * energy a DEVICE array of 2 doubles receiving {E[T], E(T)}.  The kernels are
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) and nothing
+ * is synchronised or allocated: the 1024 block partials go to a fixed
+ * 16 KiB device array of the library (one per device, held until the
+ * process ends), which serves tiles of any size.  Calls that overlap on one
+ * device share that array and race: order them on one stream.  Returns
+ * TCE_T_ERR_ARG for NULL pointers or extents <= 0. */
+int tce_t_compute_en_noalloc(double factor, const double *t3d, const double *t3s,
+                             const double *eval1, const double *eval2, const double *eval3,
+                             const double *eval4, const double *eval5, const double *eval6,
+                             long h1d, long h2d, long h3d, long p4d, long p5d, long p6d,
+                             double *energy, void *stream);
+
 /* sd_t_total.cu:5-16, 5382-5396 -- allocate and zero the task's doubles /
  * singles device tiles (h3 fastest ... p4 slowest). */
 void dev_mem_d_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d);

@@ -248,12 +248,18 @@ void launch_call(int family, int n, const long dims[6], long K, const double *ha
 }
 
 // Energy of one task tile, layout (h3 fastest, h2, h1, p6, p5, p4 slowest)
-// as in ccsd_t_dot.F:89-110; first pass: one partial pair per block.
-__global__ void __launch_bounds__(256) energy_partial(
-    long h1d, long h2d, long h3d, long p4d, long p5d, long p6d, const double *__restrict__ ev,
-    double factor, const double *__restrict__ t3d, const double *__restrict__ t3s,
-    double *__restrict__ part) {
-  const double *e1 = ev, *e2 = e1 + h1d, *e3 = e2 + h2d, *e4 = e3 + h3d, *e5 = e4 + p4d, *e6 = e5 + p5d;
+// as in ccsd_t_dot.F:89-110; first pass: one partial pair per block.  The
+// orbital energies of the six tiles are e1..e6 (h1, h2, h3, p4, p5, p6).
+__device__ __forceinline__ void energy_block(long h1d, long h2d, long h3d, long p4d, long p5d,
+                                             long p6d, const double *__restrict__ e1,
+                                             const double *__restrict__ e2,
+                                             const double *__restrict__ e3,
+                                             const double *__restrict__ e4,
+                                             const double *__restrict__ e5,
+                                             const double *__restrict__ e6, double factor,
+                                             const double *__restrict__ t3d,
+                                             const double *__restrict__ t3s,
+                                             double *__restrict__ part) {
   long total = h1d * h2d * h3d * p4d * p5d * p6d;
   double s1 = 0.0, s2 = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -287,6 +293,38 @@ __global__ void __launch_bounds__(256) energy_partial(
     part[2 * blockIdx.x + 1] = r2[0];
   }
 }
+
+// compute_en_: the six energy vectors packed in one device buffer.
+__global__ void __launch_bounds__(256) energy_partial(
+    long h1d, long h2d, long h3d, long p4d, long p5d, long p6d, const double *__restrict__ ev,
+    double factor, const double *__restrict__ t3d, const double *__restrict__ t3s,
+    double *__restrict__ part) {
+  const double *e1 = ev, *e2 = e1 + h1d, *e3 = e2 + h2d, *e4 = e3 + h3d, *e5 = e4 + p4d, *e6 = e5 + p5d;
+  energy_block(h1d, h2d, h3d, p4d, p5d, p6d, e1, e2, e3, e4, e5, e6, factor, t3d, t3s, part);
+}
+
+// tce_t_compute_en_noalloc: six separate device vectors.
+__global__ void __launch_bounds__(256) energy_partial_split(
+    long h1d, long h2d, long h3d, long p4d, long p5d, long p6d, const double *__restrict__ e1,
+    const double *__restrict__ e2, const double *__restrict__ e3, const double *__restrict__ e4,
+    const double *__restrict__ e5, const double *__restrict__ e6, double factor,
+    const double *__restrict__ t3d, const double *__restrict__ t3s, double *__restrict__ part) {
+  energy_block(h1d, h2d, h3d, p4d, p5d, p6d, e1, e2, e3, e4, e5, e6, factor, t3d, t3s, part);
+}
+
+// Grid of the energy reduction: compute_en_ and tce_t_compute_en_noalloc
+// both run energy_partial on this many blocks, so their sums are the same.
+constexpr int kEnBlocks = 1024;
+constexpr int kEnThreads = 256;
+
+// The block partials of tce_t_compute_en_noalloc: 2 doubles per block,
+// 16 KiB, one copy per device for the life of the process.  Every call
+// writes all of them before reading them, so nothing needs clearing between
+// calls.  Host code never passes this name to a kernel (in host code it is
+// a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress gives on each device.
+constexpr size_t kScratchDoubles = 2 * kEnBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 // Second pass: fixed-order sum of the block partials.
 __global__ void energy_final(const double *__restrict__ part, int nblk, double *__restrict__ out) {
@@ -417,7 +455,7 @@ extern "C" void compute_en_(double *factor, double *energy, double *eval1, doubl
   }
   long d[6] = {*h1d, *h2d, *h3d, *p4d, *p5d, *p6d};
   long ne = d[0] + d[1] + d[2] + d[3] + d[4] + d[5];
-  const int nblk = 1024, threads = 256;
+  const int nblk = kEnBlocks, threads = kEnThreads;
   size_t bytes = (ne + 2 * nblk + 2) * sizeof(double);
   double *dbuf = (double *)S.pool.get(bytes);
   if (!dbuf) {
@@ -452,6 +490,77 @@ extern "C" void compute_en_(double *factor, double *energy, double *eval1, doubl
   energy[0] = S.pinned[0];
   energy[1] = S.pinned[1];
   S.pool.put(dbuf);
+}
+
+// ------------------------------------------------ no-allocation energy -----
+namespace tce_t {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {nullptr};  // device address of `scratch` per device
+
+// Address of this device's `scratch` (looked up once per device, with the
+// device current); nullptr on failure.
+double *scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch[dev] = (double *)p;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+}  // namespace tce_t
+
+extern "C" int tce_t_compute_en_noalloc(double factor, const double *t3d, const double *t3s,
+                                        const double *eval1, const double *eval2,
+                                        const double *eval3, const double *eval4,
+                                        const double *eval5, const double *eval6, long h1d,
+                                        long h2d, long h3d, long p4d, long p5d, long p6d,
+                                        double *energy, void *stream) {
+  if (!t3d || !t3s || !eval1 || !eval2 || !eval3 || !eval4 || !eval5 || !eval6 || !energy) {
+    set_error("tce_t_compute_en_noalloc: NULL pointer");
+    return TCE_T_ERR_ARG;
+  }
+  const long d[6] = {h1d, h2d, h3d, p4d, p5d, p6d};
+  long total = 1;
+  for (long x : d) {
+    if (x <= 0 || total > (1L << 62) / x) {
+      set_error("tce_t_compute_en_noalloc: tile extents must be positive and their product "
+                "below 2^62");
+      return TCE_T_ERR_ARG;
+    }
+    total *= x;
+  }
+  // the partials of kEnBlocks blocks fill the fixed scratch array exactly
+  static_assert(2 * kEnBlocks <= (int)kScratchDoubles, "scratch holds every block partial");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("tce_t_compute_en_noalloc: hipGetDevice failed");
+    return TCE_T_ERR_HIP;
+  }
+  double *part = scratch_on(dev);
+  if (!part) {
+    set_error("tce_t_compute_en_noalloc: hipGetSymbolAddress(scratch) failed");
+    return TCE_T_ERR_HIP;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(energy_partial_split, dim3(kEnBlocks), dim3(kEnThreads), 0, st, h1d, h2d, h3d,
+                     p4d, p5d, p6d, eval1, eval2, eval3, eval4, eval5, eval6, factor, t3d, t3s,
+                     part);
+  hipLaunchKernelGGL(energy_final, dim3(1), dim3(64), 0, st, part, kEnBlocks, energy);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("tce_t_compute_en_noalloc launch: ") + hipGetErrorString(e));
+    return TCE_T_ERR_HIP;
+  }
+  return TCE_T_OK;
 }
 
 extern "C" int check_device_(long *icuda) { return local_rank() < *icuda ? 1 : 0; }
