@@ -304,7 +304,8 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
     return out
 
 
-def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -> dict:
+def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
+                rates: dict | None = None) -> dict:
     """Serving: a Poisson stream (rate lam) of requests for a pool of seeded
     systems (no in [6,40], nv in [48,320], log-uniform; tiles of 20/40 by the
     tce_tile rule), each FP64 or split-BF16 at random; systems are uploaded
@@ -348,19 +349,36 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float) -
                           file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
         st = srv.stats()
+    wait = []
     for i, r in enumerate(res):
         lat.append(r["finish"] - r["submit"])
+        wait.append(r["admit"] - r["submit"])
         F += pool[picks[i]][2][int(precs[i])]
-    lat = np.array(lat)
+    lat, wait = np.array(lat), np.array(wait)
+    # offered load: the device time the arrivals ask for per second, from
+    # each request's work at the rate of its precision measured alone
+    # (rates[p] in flops/s, the main run's FP64 and bf16x3 job rates)
+    demand = None
+    if rates:
+        per_req = np.array([pool[picks[i]][2][int(precs[i])] / rates[int(precs[i])] for i in range(nreq)])
+        demand = float(per_req.mean() * lam)
+    q = max(1, nreq // 4)
+    arrival_rate = nreq / float(arrivals[-1]) if arrivals[-1] > 0 else None
     out = {"requests": nreq, "lambda_per_s": lam, "pool_systems": npool,
            "precision_mix": {"fp64": int((precs == 0).sum()), "bf16x3": int((precs == 2).sum())},
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(lat.mean()), "wall_s": wall, "energies_per_s": nreq / wall,
+           "arrival_rate_per_s": arrival_rate, "completion_rate_per_s": nreq / wall,
+           "offered_load_at_config2_rates": demand,
+           "queue_wait_first_quarter_p50_s": float(np.percentile(wait[:q], 50)),
+           "queue_wait_last_quarter_p50_s": float(np.percentile(wait[-q:], 50)),
            "fp64_equiv_tflops": F / wall / 1e12, "slices": st["slices"], "prep_s": t_prep,
            "sizes": [[int(tm.no), int(tm.nv)] for tm, b, f in pool]}
     print(f"[bench] serve: {nreq} requests at {lam}/s over {npool} systems: p50 {out['p50_s']:.3f} s, "
-          f"p99 {out['p99_s']:.3f} s, {out['energies_per_s']:.2f} energies/s, {st['slices']} slices",
-          file=sys.stderr, flush=True)
+          f"p99 {out['p99_s']:.3f} s, {out['energies_per_s']:.2f} energies/s (arrivals "
+          f"{arrival_rate:.2f}/s, offered load {demand}), queue wait p50 first/last quarter "
+          f"{out['queue_wait_first_quarter_p50_s']:.3f}/{out['queue_wait_last_quarter_p50_s']:.3f} s, "
+          f"{st['slices']} slices", file=sys.stderr, flush=True)
     return out
 
 
@@ -378,9 +396,13 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="OpenMP threads of the CPU baseline (0: all physical cores of this "
                          "process's affinity, capped by OMP_NUM_THREADS when set)")
-    ap.add_argument("--batch-jobs", type=int, default=16,
-                    help="batch of independent jobs (config 4 has 64; 0: skip)")
-    ap.add_argument("--serve", type=int, default=24, help="serving requests (0: skip)")
+    ap.add_argument("--batch-jobs", type=int, default=64,
+                    help="batch of independent jobs (config 4: 64; 0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=3,
+                    help="timed batch runs per precision (graph replays)")
+    ap.add_argument("--mode-steps", type=int, default=5,
+                    help="timed runs per reduced-precision mode (at most --steps)")
+    ap.add_argument("--serve", type=int, default=200, help="serving requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--slice-ms", type=float, default=50.0)
@@ -438,8 +460,9 @@ def main() -> int:
         job.run(prec)  # converts the contraction tensors on first use (untimed)
         t_prep = time.perf_counter() - t1
         mk = []
+        msteps = max(1, min(args.steps, args.mode_steps))
         t1 = time.perf_counter()
-        for s in range(args.steps):
+        for s in range(msteps):
             m_bt, m_pt = job.run(prec)
             mk.append(job.last_kernel_ms)
         mwall = time.perf_counter() - t1
@@ -448,11 +471,12 @@ def main() -> int:
               f"{t_prep:.1f} s) E[T]={m_bt:.12e} dE[T]={d_bt:.3f} uEh, E(T)={m_pt:.12e} "
               f"dE(T)={d_pt:.3f} uEh", file=sys.stderr, flush=True)
         modes[name] = {
-            "ms_per_step": mwall / args.steps * 1e3,
+            "steps": msteps,
+            "ms_per_step": mwall / msteps * 1e3,
             "kernel_ms": float(np.mean(mk)),
-            "energies_per_s": args.steps / mwall,
-            "speedup_vs_fp64": wall / mwall,
-            "fp64_equiv_tflops": job.work(prec)[0] * args.steps / mwall / 1e12,
+            "energies_per_s": msteps / mwall,
+            "speedup_vs_fp64": (wall / args.steps) / (mwall / msteps),
+            "fp64_equiv_tflops": job.work(prec)[0] * msteps / mwall / 1e12,
             "E_bt": m_bt, "E_pt": m_pt,
             "dE_bt_uEh": d_bt, "dE_pt_uEh": d_pt,
             "dE_rel": max(abs(m_bt - e_bt) / abs(e_bt), abs(m_pt - e_pt) / abs(e_pt)) if e_bt and e_pt else None,
@@ -514,9 +538,12 @@ def main() -> int:
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profiles(workload, kernel),
                 "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F_run,
                 "items_per_launch": items}
-    batch = bench_batch(args.batch_jobs, args.seed, max(1, args.steps)) if args.batch_jobs else None
-    serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms) \
-        if args.serve else None
+    batch = bench_batch(args.batch_jobs, args.seed, max(1, args.batch_steps)) if args.batch_jobs else None
+    rates = {0: F_run * args.steps / wall}
+    if "bf16x3" in modes:
+        rates[2] = modes["bf16x3"]["fp64_equiv_tflops"] * 1e12
+    serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms,
+                        rates if len(rates) == 2 else None) if args.serve else None
     cpu = None
     cfg1 = None
     if args.cpu_baseline:

@@ -240,6 +240,18 @@ int tce_t_pack_dense(const tce_t_tiles *tiles, const double *t1, const long *k_t
                      const long *k_v2_offset, double *T1, double *T2e, double *T2m,
                      double *Wd1, double *Vd2, double *S);
 
+/* Inspection (no reference counterpart): the job's reduced-precision copy
+ * of one contraction tensor, as the kernels read it.  tensor: 0 = T2e,
+ * 1 = Vd2 (d2 operands), 2 = Wd1, 3 = T2m (d1 operands), laid out as
+ * tce_t_pack_dense's tensors with the contraction extent zero-padded to
+ * *kout (a multiple of the kernel's chunk).  FP32: out0 gets *rows x *kout
+ * floats (out1 must be NULL); BF16X3: out0 the bf16 hi and out1 the bf16 lo
+ * parts (*rows x *kout 16-bit patterns each; either may be NULL).  Both
+ * outputs NULL: only *rows and *kout are set (no upload needed).  Converts
+ * the tensors first if no run in `precision` has yet.  Synchronous. */
+int tce_t_job_operands(tce_t_job *job, int precision, int tensor, void *out0, void *out1,
+                       long *rows, long *kout);
+
 /* Device time (ms) of the last run's contraction kernel(s), HIP events. */
 double tce_t_job_last_kernel_ms(const tce_t_job *job);
 /* Which FP64 contraction kernel the job runs: TCE_T_KERNEL_WAVE (one

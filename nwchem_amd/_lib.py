@@ -72,6 +72,8 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_last_kernel_ms.restype = C.c_double
     L.tce_t_job_precision_bytes.argtypes = [C.c_void_p, C.c_int]
     L.tce_t_job_precision_bytes.restype = C.c_long
+    L.tce_t_job_operands.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.tce_t_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.tce_t_batch_run.argtypes = [C.c_void_p, C.c_int, DP]
     L.tce_t_batch_last_ms.argtypes = [C.c_void_p]

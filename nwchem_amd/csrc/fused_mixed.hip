@@ -42,6 +42,8 @@
 // Fragment reads (16 lanes of one kq: 4 row/col slots x 4 inner indices)
 // touch 16 distinct 4-bank groups; each load instruction writes 64
 // consecutive quads and 4 consecutive lanes read one 64-byte row segment.
+#include <type_traits>
+
 #include "fused.h"
 
 namespace tce_t {
@@ -305,38 +307,94 @@ __device__ __forceinline__ void read_y(Frag<PREC> &F, const f4 *cb, int yb) {
   }
 }
 
-// The MFMAs of X quads S0..S1-1 with every Y quad (lone hole H).
-template <int PREC, int H, int S0, int S1>
-__device__ __forceinline__ void mma_x(typename Prec<PREC>::acc (&a)[64], const Frag<PREC> &F) {
-  constexpr int NT = PREC == TCE_T_FP64 ? 2 : (PREC == TCE_T_FP32 ? 4 : 1);
+// FP64: the MFMAs of X quads S0..S1-1 with every Y quad (lone hole H) on
+// the scalar accumulators, each quad gathered for its MFMA (the compiler
+// keeps the FP64 chunk loops free of moves and spills this way; the quad
+// form below spills there).
+template <int H, int S0, int S1>
+__device__ __forceinline__ void mma_x64(double (&a)[64], const Frag<TCE_T_FP64> &F) {
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
+  for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int s = S0; s < S1; ++s)
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        if constexpr (PREC == TCE_T_FP64) {
-          d4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                  a[acc_at<H>(s, h, 3)]};
-          c = __builtin_amdgcn_mfma_f64_16x16x4f64(F.x[s][t], F.y[h][t], c, 0, 0, 0);
+        d4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
+                a[acc_at<H>(s, h, 3)]};
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(F.x[s][t], F.y[h][t], c, 0, 0, 0);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
-        } else if constexpr (PREC == TCE_T_FP32) {
-          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                  a[acc_at<H>(s, h, 3)]};
-          c = __builtin_amdgcn_mfma_f32_16x16x4f32(F.x[s][t], F.y[h][t], c, 0, 0, 0);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
-        } else {
-          f4 c = {a[acc_at<H>(s, h, 0)], a[acc_at<H>(s, h, 1)], a[acc_at<H>(s, h, 2)],
-                  a[acc_at<H>(s, h, 3)]};
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.xl[s], F.y[h], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.x[s], F.yl[h], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F.x[s], F.y[h], c, 0, 0, 0);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
-        }
+        for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = c[r];
       }
+}
+
+// FP32 and split-BF16: accumulator quads of a pass, q[4 s + h] holding
+// components r = 0..3 of MFMA quad (s, h), i.e. acc[acc_at<H>(s, h, r)].
+// Inside a pass's chunk loop the 16 quads are the MFMA C/D operands
+// themselves (4 consecutive registers each); the scattered acc[] order of
+// the other passes is restored only at the pass boundary, so the chunk loop
+// needs no register moves (before: 65-136 moves per bf16 chunk).
+template <int PREC>
+struct AccQ;
+template <>
+struct AccQ<TCE_T_FP64> {
+  typedef double type[4][4];  // unused: FP64 keeps the scalar accumulators
+};
+template <>
+struct AccQ<TCE_T_FP32> {
+  typedef f4 type;
+};
+template <>
+struct AccQ<TCE_T_BF16X3> {
+  typedef f4 type;
+};
+
+template <int PREC, int H>
+__device__ __forceinline__ void to_quads(typename AccQ<PREC>::type (&q)[16],
+                                         const typename Prec<PREC>::acc (&a)[64]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[4 * s + h][r] = a[acc_at<H>(s, h, r)];
+}
+template <int PREC, int H>
+__device__ __forceinline__ void from_quads(typename Prec<PREC>::acc (&a)[64],
+                                           const typename AccQ<PREC>::type (&q)[16]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[acc_at<H>(s, h, r)] = q[4 * s + h][r];
+}
+
+// The MFMAs of X quads S0..S1-1 with every Y quad.  Split-BF16 issues the
+// three terms of all 16 quads term by term (lo*hi', hi*lo', hi*hi' for each
+// quad, in that order as before), so dependent MFMAs on one quad are 16
+// instructions apart.
+template <int PREC, int S0, int S1>
+__device__ __forceinline__ void mma_q(typename AccQ<PREC>::type (&q)[16], const Frag<PREC> &F) {
+  if constexpr (PREC == TCE_T_BF16X3) {
+#pragma unroll
+    for (int term = 0; term < 3; ++term)
+#pragma unroll
+      for (int s = S0; s < S1; ++s)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const bf8 xa = term == 0 ? F.xl[s] : F.x[s];
+          const bf8 yb = term == 1 ? F.yl[h] : F.y[h];
+          q[4 * s + h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, yb, q[4 * s + h], 0, 0, 0);
+        }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = S0; s < S1; ++s)
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+          q[4 * s + h] = __builtin_amdgcn_mfma_f32_16x16x4f32(F.x[s][t], F.y[h][t], q[4 * s + h], 0, 0, 0);
+  }
 }
 
 // This lane's fragment offsets in pass P: X quad (s = 0), Y quad (h = 0).
@@ -431,6 +489,19 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
     permute_lanes(acc, 4 * hi + kq + 16 * lo);
   }
   if constexpr (pi.minus) negate(acc);
+  typename AccQ<PREC>::type q[16];
+  if constexpr (PREC != TCE_T_FP64) to_quads<PREC, H>(q, acc);
+  // the MFMAs of X quads S0..S1-1 of fragments f on this pass's accumulators
+  auto mma = [&](auto s0_, auto s1_, const Frag<PREC> &f) {
+    constexpr int S0 = decltype(s0_)::value, S1 = decltype(s1_)::value;
+    if constexpr (PREC == TCE_T_FP64)
+      mma_x64<H, S0, S1>(acc, f);
+    else
+      mma_q<PREC, S0, S1>(q, f);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I2 = std::integral_constant<int, 2>;
+  using I4 = std::integral_constant<int, 4>;
   const int nch0 = chunks_of<PREC>(M, P);
   // load sources of this pass; those of the next passes are formed only in
   // the branches that need them (the pass's last chunks), which keeps the
@@ -439,7 +510,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
   if constexpr (Pr::SPLIT) {
     for (int c = 0; c < nch0; ++c, ++j) {
       const bool next = j + 1 < total;
-      mma_x<PREC, H, 0, 2>(acc, F);
+      mma(I0(), I2(), F);
       if (j + 2 < total)
         wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
       else
@@ -453,7 +524,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       const int2 o = c + 1 < nch0 ? frag_offsets<PREC, P>(wave, lane)
                                   : frag_offsets<PREC, (P + 1 < 18 ? P + 1 : 17)>(wave, lane);
       if (next) read_x<PREC, 0, 2>(F, cbn, o.x);
-      mma_x<PREC, H, 2, 4>(acc, F);
+      mma(I2(), I4(), F);
       if (next) {
         read_x<PREC, 2, 4>(F, cbn, o.x);
         read_y<PREC>(F, cbn, o.y);
@@ -472,9 +543,9 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       // barrier) go between the two halves of the MFMAs, so their address
       // arithmetic issues while the matrix core works (measured 0.7% faster
       // than before the MFMAs, 3% faster than after them)
-      mma_x<PREC, H, 0, 2>(acc, f);
+      mma(I0(), I2(), f);
       if (more) load_ahead<PREC, P, 2>(s0, M, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
-      mma_x<PREC, H, 2, 4>(acc, f);
+      mma(I2(), I4(), f);
       if (more)
         wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
       else
@@ -483,6 +554,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       buf = buf == 2 ? 0 : buf + 1;
     }
   }
+  if constexpr (PREC != TCE_T_FP64) from_quads<PREC, H>(acc, q);
   if constexpr (pi.minus) negate(acc);
 }
 

@@ -859,6 +859,35 @@ extern "C" long tce_t_job_precision_bytes(const tce_t_job *job, int precision) {
   return extra;
 }
 
+extern "C" int tce_t_job_operands(tce_t_job *job, int precision, int tensor, void *out0,
+                                  void *out1, long *rows, long *kout) {
+  if (!job || tensor < 0 || tensor > 3) return TCE_T_ERR_ARG;
+  if (precision != TCE_T_FP32 && precision != TCE_T_BF16X3) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  const long no = J.T.no, nv = J.T.nv;
+  const long nrows[4] = {no * no * nv, no * nv * nv, no * no * nv, no * nv * nv};
+  const long k = tensor < 2 ? mixed_extent(nv, precision) : mixed_extent(no, precision);
+  if (rows) *rows = nrows[tensor];
+  if (kout) *kout = k;
+  if (!out0 && !out1) return TCE_T_OK;
+  if (!J.uploaded) {
+    set_error("tce_t_job_operands: job not uploaded");
+    return TCE_T_ERR_STATE;
+  }
+  if (precision == TCE_T_FP32 && out1) return TCE_T_ERR_ARG;
+  const int rc = prepare_mixed(J, precision);
+  if (rc != TCE_T_OK) return rc;
+  Job::Mixed &m = J.mx[precision];
+  void *const *src = tensor == 0 ? m.x2 : tensor == 1 ? m.y2 : tensor == 2 ? m.x1 : m.y1;
+  const size_t bytes = (size_t)nrows[tensor] * k * (precision == TCE_T_FP32 ? 4 : 2);
+  void *out[2] = {out0, out1};
+  for (int p = 0; p < 2; ++p)
+    if (out[p] && bytes)
+      HIPCHK(hipMemcpyAsync(out[p], src[p], bytes, hipMemcpyDeviceToHost, J.stream), "operands D2H");
+  HIPCHK(hipStreamSynchronize(J.stream), "operands D2H");
+  return TCE_T_OK;
+}
+
 extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt) {
   if (!job) return TCE_T_ERR_ARG;
   Job &J = *reinterpret_cast<Job *>(job);

@@ -131,6 +131,25 @@ class TriplesJob:
         """Extra device bytes the first run in `precision` allocates."""
         return int(_lib.lib().tce_t_job_precision_bytes(self._h, precision))
 
+    def operands(self, precision: int, tensor: int):
+        """The reduced-precision copy of contraction tensor `tensor` (0 T2e,
+        1 Vd2, 2 Wd1, 3 T2m) as the kernels read it: FP32 -> float32 array
+        (rows, kout); BF16X3 -> (hi, lo) uint16 bf16 patterns (rows, kout)."""
+        L = _lib.lib()
+        rows, kout = C.c_long(0), C.c_long(0)
+        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, None, None, C.byref(rows),
+                                        C.byref(kout)), "tce_t_job_operands")
+        shape = (rows.value, kout.value)
+        if precision == 1:
+            a = np.zeros(shape, np.float32)
+            _lib.check(L.tce_t_job_operands(self._h, precision, tensor, a.ctypes.data, None, None,
+                                            None), "tce_t_job_operands")
+            return a
+        hi, lo = np.zeros(shape, np.uint16), np.zeros(shape, np.uint16)
+        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, hi.ctypes.data, lo.ctypes.data,
+                                        None, None), "tce_t_job_operands")
+        return hi, lo
+
     @property
     def last_kernel_ms(self) -> float:
         return float(_lib.lib().tce_t_job_last_kernel_ms(self._h))

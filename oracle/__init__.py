@@ -88,6 +88,10 @@ def lib() -> C.CDLL:
         L.ora_ccsd_t_gpu.restype = C.c_long
         L.ora_set_mutation.argtypes = [C.c_int, C.c_int]
         L.ora_set_mutation.restype = None
+        L.ora_set_arith.argtypes = [C.c_int]
+        L.ora_set_arith.restype = None
+        L.ora_split_bf16.argtypes = [DP, C.c_long, C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort)]
+        L.ora_split_bf16.restype = None
         L.ora_brute_force.argtypes = [C.c_long, C.c_long] + [DP] * 7 + [DP, DP]
         L.ora_task_tiles.argtypes = [C.POINTER(OraTiles), DP, LP, DP, LP, DP, LP, LP, DP, DP]
         L.ora_task_tiles.restype = C.c_long
@@ -218,6 +222,36 @@ def brute_force(no, nv, T1, T2, W, V, S, eo, ev):
     arrs = [np.ascontiguousarray(a, np.float64) for a in (T1, T2, W, V, S, eo, ev)]
     L.ora_brute_force(no, nv, *[_dp(a) for a in arrs], C.byref(e4), C.byref(e5))
     return e4.value, e5.value
+
+
+ARITH = {"fp64": 0, "fp32": 1, "bf16x3": 2, "abs": 3, "bf16x2_planted": 4, "bf16x3_trunc_planted": 5}
+
+
+class arith:
+    """Context manager: the built-in d1/d2 kernels in another arithmetic
+    (see sdt_oracle.h, ora_set_arith); FP64 again on exit."""
+
+    def __init__(self, mode: str):
+        self.mode = ARITH[mode]
+
+    def __enter__(self):
+        lib().ora_set_arith(self.mode)
+        return self
+
+    def __exit__(self, *a):
+        lib().ora_set_arith(0)
+
+
+def split_bf16(x, truncate_lo: bool = False):
+    """The library's operand split of FP64 values: (hi, lo) as uint16 bf16
+    bit patterns (hi = bf16(fl32(x)), lo = bf16(fl32(x - hi)), round to
+    nearest even; truncate_lo: the planted truncation)."""
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    hi = np.zeros(x.size, np.uint16)
+    lo = np.zeros(x.size, np.uint16)
+    lib().ora_split_bf16(_dp(x), x.size, int(truncate_lo), hi.ctypes.data_as(C.POINTER(C.c_ushort)),
+                         lo.ctypes.data_as(C.POINTER(C.c_ushort)))
+    return hi, lo
 
 
 def set_mutation(family: int, n: int) -> None:

@@ -279,6 +279,75 @@ static int mut_sign(int family, int n, int sg) {
   return (family == g_mut_family && n == g_mut_n) ? -sg : sg;
 }
 
+/* ------------------------------------------ reduced-precision arithmetic -- */
+
+/* Operand arithmetic of the built-in d1/d2 kernels (the s1 kernels and the
+ * energy stay FP64 in every mode).  The reduced-precision modes restate the
+ * library's operand conversion and represent every product EXACTLY in FP64:
+ * what they leave out of the GPU arithmetic is only the FP32 rounding of
+ * its accumulation, so the gap to the GPU is what an accumulation error
+ * model has to cover.  ORA_ARITH_ABS adds |a|*|b| whatever the variant's
+ * sign: the sum of product magnitudes such a model needs.  The two planted
+ * mistakes are for tests of that model's sensitivity. */
+static int g_arith = ORA_ARITH_FP64;
+void ora_set_arith(int mode) { g_arith = mode; }
+
+static unsigned f32_bits(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+static float bits_f32(unsigned u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+/* bf16 of a float, round to nearest even on the upper 16 bits (finite x). */
+static unsigned short bf16_rne(float f) {
+  unsigned u = f32_bits(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+/* bf16 of a float by truncation toward zero (the planted mistake). */
+static unsigned short bf16_trunc(float f) { return (unsigned short)(f32_bits(f) >> 16); }
+static double bf16_val(unsigned short h) { return (double)bits_f32((unsigned)h << 16); }
+
+void ora_split_bf16(const double *x, long n, int truncate_lo, unsigned short *hi,
+                    unsigned short *lo) {
+  for (long i = 0; i < n; i++) {
+    /* hi = bf16(fl32(x)); lo = bf16(fl32(x - hi)), x - hi exact in FP64 */
+    unsigned short h = bf16_rne((float)x[i]);
+    float r = (float)(x[i] - bf16_val(h));
+    hi[i] = h;
+    lo[i] = truncate_lo ? bf16_trunc(r) : bf16_rne(r);
+  }
+}
+
+/* Split operand: value = h + l (FP32: h = fl32(x), l = 0; ABS: h = |x|). */
+static void split_operand(const double *x, long n, double *h, double *l) {
+  for (long i = 0; i < n; i++) {
+    if (g_arith == ORA_ARITH_FP32) {
+      h[i] = (double)(float)x[i];
+      l[i] = 0.0;
+    } else if (g_arith == ORA_ARITH_ABS) {
+      h[i] = fabs(x[i]);
+      l[i] = 0.0;
+    } else {
+      unsigned short hi, lo;
+      ora_split_bf16(x + i, 1, g_arith == ORA_ARITH_BF16X3_TRUNC, &hi, &lo);
+      h[i] = bf16_val(hi);
+      l[i] = bf16_val(lo);
+    }
+  }
+}
+
+/* The product a*b of the current mode, exact in FP64 (8- or 24-bit
+ * significands): hi*hi' + hi*lo' + lo*hi' (the 2-term mistake drops lo*hi'). */
+static inline double emul_prod(double ha, double la, double hb, double lb) {
+  if (g_arith == ORA_ARITH_BF16X2) return ha * hb + ha * lb;
+  return ha * hb + ha * lb + la * hb;
+}
+
 /* Strides of the six loop variables inside triplesx for layout L. */
 static void out_strides(const int L[6], const long dims[6], long st[6]) {
   long s = 1;
@@ -313,9 +382,18 @@ void ora_sd_t_s1(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
 }
 
 /* t3(L) (+/-)= sum_h7 t2sub(h7,p4,p5,h1) * v2sub(h3,h2,p6,h7) */
+static void emul_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d, long p4d,
+                    long h7d, double *t3, const double *t2, const double *v2);
+static void emul_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d, long p4d,
+                    long p7d, double *t3, const double *t2, const double *v2);
+
 void ora_sd_t_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
                  long p4d, long h7d, double *t3, const double *t2,
                  const double *v2) {
+  if (g_arith != ORA_ARITH_FP64) {
+    emul_d1(n, h3d, h2d, h1d, p6d, p5d, p4d, h7d, t3, t2, v2);
+    return;
+  }
   long dims[6] = {p4d, p5d, p6d, h1d, h2d, h3d}, st[6];
   out_strides(D1_L[n - 1], dims, st);
   int sg = mut_sign(1, n, D1_SIGN[n - 1]);
@@ -345,6 +423,10 @@ void ora_sd_t_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
 void ora_sd_t_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
                  long p4d, long p7d, double *t3, const double *t2,
                  const double *v2) {
+  if (g_arith != ORA_ARITH_FP64) {
+    emul_d2(n, h3d, h2d, h1d, p6d, p5d, p4d, p7d, t3, t2, v2);
+    return;
+  }
   long dims[6] = {p4d, p5d, p6d, h1d, h2d, h3d}, st[6];
   out_strides(D2_L[n - 1], dims, st);
   int sg = mut_sign(2, n, D2_SIGN[n - 1]);
@@ -368,6 +450,70 @@ void ora_sd_t_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
               }
               t3[o] = acc;
             }
+}
+
+/* The d1 and d2 loops of the reduced-precision modes: the same element
+ * order, operands split once per call, each product exact. */
+static void emul_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d, long p4d,
+                    long h7d, double *t3, const double *t2, const double *v2) {
+  long dims[6] = {p4d, p5d, p6d, h1d, h2d, h3d}, st[6];
+  out_strides(D1_L[n - 1], dims, st);
+  int sg = g_arith == ORA_ARITH_ABS ? 1 : mut_sign(1, n, D1_SIGN[n - 1]);
+  long na = h7d * p4d * p5d * h1d, nb = h3d * h2d * p6d * h7d;
+  double *ha = malloc(sizeof(double) * (2 * na + 2 * nb)), *la = ha + na, *hb = la + na,
+         *lb = hb + nb;
+  split_operand(t2, na, ha, la);
+  split_operand(v2, nb, hb, lb);
+#pragma omp parallel for collapse(3) schedule(static)
+  for (long p4 = 0; p4 < p4d; p4++)
+    for (long p5 = 0; p5 < p5d; p5++)
+      for (long p6 = 0; p6 < p6d; p6++)
+        for (long h1 = 0; h1 < h1d; h1++)
+          for (long h2 = 0; h2 < h2d; h2++)
+            for (long h3 = 0; h3 < h3d; h3++) {
+              long o = p4 * st[P4] + p5 * st[P5] + p6 * st[P6] + h1 * st[H1] +
+                       h2 * st[H2] + h3 * st[H3];
+              long ia = h7d * (p4 + p4d * (p5 + p5d * h1));
+              double acc = t3[o];
+              for (long h7 = 0; h7 < h7d; h7++) {
+                long ib = h3 + h3d * (h2 + h2d * (p6 + p6d * h7));
+                double pr = emul_prod(ha[ia + h7], la[ia + h7], hb[ib], lb[ib]);
+                acc = sg > 0 ? acc + pr : acc - pr;
+              }
+              t3[o] = acc;
+            }
+  free(ha);
+}
+
+static void emul_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d, long p4d,
+                    long p7d, double *t3, const double *t2, const double *v2) {
+  long dims[6] = {p4d, p5d, p6d, h1d, h2d, h3d}, st[6];
+  out_strides(D2_L[n - 1], dims, st);
+  int sg = g_arith == ORA_ARITH_ABS ? 1 : mut_sign(2, n, D2_SIGN[n - 1]);
+  long na = p7d * p4d * h1d * h2d, nb = p7d * h3d * p6d * p5d;
+  double *ha = malloc(sizeof(double) * (2 * na + 2 * nb)), *la = ha + na, *hb = la + na,
+         *lb = hb + nb;
+  split_operand(t2, na, ha, la);
+  split_operand(v2, nb, hb, lb);
+#pragma omp parallel for collapse(3) schedule(static)
+  for (long p4 = 0; p4 < p4d; p4++)
+    for (long p5 = 0; p5 < p5d; p5++)
+      for (long p6 = 0; p6 < p6d; p6++)
+        for (long h1 = 0; h1 < h1d; h1++)
+          for (long h2 = 0; h2 < h2d; h2++)
+            for (long h3 = 0; h3 < h3d; h3++) {
+              long o = p4 * st[P4] + p5 * st[P5] + p6 * st[P6] + h1 * st[H1] +
+                       h2 * st[H2] + h3 * st[H3];
+              long ia = p7d * (p4 + p4d * (h1 + h1d * h2));
+              long ib = p7d * (h3 + h3d * (p6 + p6d * p5));
+              double acc = t3[o];
+              for (long p7 = 0; p7 < p7d; p7++) {
+                double pr = emul_prod(ha[ia + p7], la[ia + p7], hb[ib + p7], lb[ib + p7]);
+                acc = sg > 0 ? acc + pr : acc - pr;
+              }
+              t3[o] = acc;
+            }
+  free(ha);
 }
 
 /* Adapters so the built-in kernels share the by-reference back-end type. */

@@ -94,6 +94,24 @@ void ora_sd_t_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d,
  * (family 0 = s1, 1 = d1, 2 = d2, n = 1..9); family = -1 removes it. */
 void ora_set_mutation(int family, int n);
 
+/* Arithmetic of the built-in d1/d2 kernels (sdt_oracle.c, reduced-precision
+ * arithmetic): FP64 (default); FP32 = operands rounded to FP32; BF16X3 =
+ * operands split hi = bf16(fl32(x)), lo = bf16(fl32(x - hi)) (round to
+ * nearest even) and products hi*hi' + hi*lo' + lo*hi'; all products exact,
+ * accumulation FP64.  ABS = sum of |a||b| with every sign +.  BF16X2 (lo*hi'
+ * dropped) and BF16X3_TRUNC (lo truncated) are planted mistakes. */
+#define ORA_ARITH_FP64 0
+#define ORA_ARITH_FP32 1
+#define ORA_ARITH_BF16X3 2
+#define ORA_ARITH_ABS 3
+#define ORA_ARITH_BF16X2 4
+#define ORA_ARITH_BF16X3_TRUNC 5
+void ora_set_arith(int mode);
+/* The library's operand split (bf16 hi, lo as 16-bit patterns);
+ * truncate_lo = the planted mistake. */
+void ora_split_bf16(const double *x, long n, int truncate_lo, unsigned short *hi,
+                    unsigned short *lo);
+
 /* ccsd_t_dot.F:20-125 (energy of one task tile). */
 void ora_ccsd_t_dot(const double *singles, const double *doubles,
                     int restricted, long h1b, long h2b, long h3b, long p4b,

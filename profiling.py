@@ -74,7 +74,7 @@ def pmc_sums(db: str, like: str):
 
 def short_name(kernel: str) -> str:
     """fused_triples_mixed<2, false> from the demangled signature."""
-    k = kernel.split("(")[0]
+    k = kernel.replace("(anonymous namespace)::", "").split("(")[0]
     return k.split("::")[-1].replace("void ", "")
 
 

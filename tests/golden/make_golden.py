@@ -2,6 +2,13 @@
 
     python3 tests/golden/make_golden.py        (needs oracle/_ref built from
                                                 the reference tree)
+    python3 tests/golden/make_golden.py --config2
+                                               (the same at the full benchmark
+                                                size; ~2 min per task on 8 cores)
+    python3 tests/golden/make_golden.py --precision [--with-config2]
+                                               (reduced-precision references;
+                                                the benchmark-job tasks take
+                                                about an hour on 8 cores)
 
 kernels.npz   the 27 kernels of ccsd_t_kernels_omp.F (sd_t_s1_1..9,
               sd_t_d1_1..9, sd_t_d2_1..9), compiled from the reference tree
@@ -13,6 +20,26 @@ energies.npz  E[T], E(T) and every task's pair for the systems in SYSTEMS,
               with the reference's compiled kernels and ccsd_t_dot as the
               arithmetic, plus SHA-256 digests of the input blocks and hash
               tables, so a test can tell an input change from a result change.
+
+config2_tasks.json  (--config2) {E[T], E(T)} of eight tasks of the
+              benchmark job (no=40 in tiles of 20, nv=320 in tiles of 40,
+              seed 20261202, 480 tasks): the first and last task, tasks whose
+              three particle (and hole) tiles are all equal, two equal, all
+              distinct.  Each is one task of the ccsd_t_gpu.F task loop run
+              with the reference's compiled kernels in their strict-FP OpenMP
+              build (oracle/_ref/libsdt_ref_par.so; the `!$omp simd` of the
+              contraction loops may reorder each element's sum, ~1e-16
+              relative against the serial build), with the SHA-256 digest of
+              the input blocks and tables.
+
+precision.npz / precision.json  (--precision) for the systems of
+              PRECISION_SYSTEMS and the modes fp32 and bf16x3: the oracle's
+              energies in the same operand representation as the GPU
+              (oracle.arith, tests/precision_model.py: products exact, FP64
+              accumulation), totals and per task, with the accumulation
+              bounds of tests/precision_model.py (per task and total); and
+              for two tasks of the benchmark job (CONFIG2_PRECISION_TASKS)
+              the same per-task values.
 
 The inputs of the synthetic systems come from the library's Philox generator
 (tce_t_synth_blocks); the spin-labelled ones from tests/spin_systems.py.
@@ -99,8 +126,138 @@ def digest(*arrays) -> str:
     return h.hexdigest()
 
 
+CONFIG2 = dict(no=40, nv=320, to=20, tv=40, seed=20261202)
+# (p4b, p5b, p6b, h1b, h2b, h3b) of the fixture tasks (particles are tiles
+# 3..10, holes 1..2)
+CONFIG2_TASKS = [
+    (3, 3, 3, 1, 1, 1),     # first task: all tiles equal in both spaces
+    (10, 10, 10, 2, 2, 2),  # last task: all equal in both spaces
+    (7, 7, 7, 1, 2, 2),     # particles all equal, two holes equal
+    (3, 3, 5, 1, 1, 2),     # two particles equal, two holes equal
+    (4, 6, 6, 1, 2, 2),     # two particles equal (the outer pair)
+    (3, 5, 9, 1, 1, 2),     # particles all distinct
+    (5, 8, 10, 1, 1, 1),    # particles all distinct, holes all equal
+    (3, 4, 5, 1, 2, 2),     # particles all distinct, consecutive
+]
+
+
+PRECISION_SYSTEMS = ["cfg1_no8_nv32", "no7_nv13_to3_tv5", "no5_nv9_to2_tv4", "no9_nv41_ragged",
+                     "spin_sym_unrestricted", "spin_flip_restricted"]
+CONFIG2_PRECISION_TASKS = [(3, 3, 3, 1, 1, 1), (3, 5, 9, 1, 1, 2)]
+
+
+def precision() -> int:
+    """Reduced-precision references (see the module docstring)."""
+    import time
+
+    import oracle
+    import precision_model as pm
+
+    arrays, meta = {}, {}
+    for name in PRECISION_SYSTEMS:
+        kind, prm = SYSTEMS[name]
+        ot, b, tm = build_system(kind, prm)
+        meta[name] = {"inputs_sha256": digest(b.t1, b.t2, b.v2),
+                      "tables_sha256": digest(b.t1h, b.t2h, b.v2h, tm.range, tm.evl)}
+        for mode in ("fp32", "bf16x3"):
+            t0 = time.time()
+            e1, e2, te = pm.emulated(tm, b, ot, mode)
+            bt, btot = pm.accumulation_bounds(tm, b, ot, mode)
+            arrays[f"{name}_{mode}_tasks"] = te
+            arrays[f"{name}_{mode}_total"] = np.array([e1, e2])
+            arrays[f"{name}_{mode}_task_bounds"] = bt
+            arrays[f"{name}_{mode}_total_bounds"] = btot
+            meta[name][mode] = {"E_bt": float(e1).hex(), "E_pt": float(e2).hex(),
+                                "bound_bt": float(btot[0]), "bound_pt": float(btot[1])}
+            print(f"{name} {mode}: E[T]={e1:.15e} E(T)={e2:.15e} bound {btot} "
+                  f"({time.time() - t0:.0f} s)", flush=True)
+    np.savez_compressed(os.path.join(HERE, "precision.npz"), **arrays)
+    path = os.path.join(HERE, "precision.json")
+    if os.path.exists(path):  # keep the benchmark-job entries of an earlier run
+        with open(path) as f:
+            old = json.load(f)
+        if "config2" in old:
+            meta["config2"] = old["config2"]
+    with open(path, "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+        f.write("\n")
+    if "--with-config2" not in sys.argv:
+        return 0
+    # two tasks of the benchmark job
+    ot, b, tm = build_system("synthetic", CONFIG2)
+    tasks = [tuple(int(x) for x in t) for t in tm.tasks()]
+    meta["config2"] = {"params": CONFIG2, "inputs_sha256": digest(b.t1, b.t2, b.v2), "tasks": []}
+    for t in CONFIG2_PRECISION_TASKS:
+        k = tasks.index(t)
+        entry = {"index": k, "tiles": list(t)}
+        t0 = time.time()
+        s_, d_ = oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, t)
+        with oracle.arith("abs"):
+            _, a_ = oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, t)
+        for mode in ("fp32", "bf16x3"):
+            sums = pm.task_sums(tm, t, s_, d_, a_, mode)
+            bnd = pm.bound_from_sums(*sums)
+            with oracle.arith(mode):
+                _, _, te = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                                         task_begin=k, task_end=k + 1)
+            entry[mode] = {"E_bt": float(te[0, 0]).hex(), "E_pt": float(te[0, 1]).hex(),
+                           "E_bt_dec": float(te[0, 0]), "E_pt_dec": float(te[0, 1]),
+                           "bound_bt": float(bnd[0]), "bound_pt": float(bnd[1])}
+            print(f"config2 task {k} {t} {mode}: {te[0]} bound {bnd} ({time.time() - t0:.0f} s)",
+                  flush=True)
+        del s_, d_, a_
+        meta["config2"]["tasks"].append(entry)
+        with open(path, "w") as f:  # after every task: partial results survive
+            json.dump(meta, f, indent=1, sort_keys=True)
+            f.write("\n")
+    return 0
+
+
+def config2(threads: int | None = None) -> int:
+    """Golden task energies of the full-size benchmark job (see above)."""
+    import time
+
+    import oracle
+
+    if threads:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    ot, b, tm = build_system("synthetic", CONFIG2)
+    tasks = [tuple(int(x) for x in t) for t in tm.tasks()]
+    be = oracle.ref_backend("par")
+    path = os.path.join(HERE, "config2_tasks.json")
+    meta = {"params": CONFIG2, "tasks_total": len(tasks),
+            "inputs_sha256": digest(b.t1, b.t2, b.v2),
+            "tables_sha256": digest(b.t1h, b.t2h, b.v2h, tm.range, tm.evl),
+            "kernels": "oracle/_ref/libsdt_ref_par.so (ccsd_t_kernels_omp.F, -O2 "
+                       "-ffp-contract=off -fopenmp)",
+            "task_order": "ccsd_t_gpu.F:87-112 (index into the job's task list)",
+            "tasks": []}
+    for t in CONFIG2_TASKS:
+        k = tasks.index(t)
+        t0 = time.time()
+        e1, e2, te = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be,
+                                   task_begin=k, task_end=k + 1)
+        r = tm.range
+        n = int(np.prod([r[x - 1] for x in t]))
+        meta["tasks"].append({"index": k, "tiles": list(t), "elements": n,
+                              "E_bt": float(te[0, 0]).hex(), "E_pt": float(te[0, 1]).hex(),
+                              "E_bt_dec": float(te[0, 0]), "E_pt_dec": float(te[0, 1]),
+                              "cpu_seconds": round(time.time() - t0, 1)})
+        print(f"task {k} {t}: E[T]={te[0, 0]:.15e} E(T)={te[0, 1]:.15e} ({time.time() - t0:.0f} s)",
+              flush=True)
+        with open(path, "w") as f:  # after every task: partial results survive
+            json.dump(meta, f, indent=1)
+            f.write("\n")
+    return 0
+
+
 def main() -> int:
     import oracle
+
+    if "--config2" in sys.argv:
+        return config2()
+    if "--precision" in sys.argv:
+        return precision()
 
     R = oracle.ref_lib()
     out = {}

@@ -6,6 +6,7 @@ floor only matters when |E| > 100 Eh (the random amplitudes of the synthetic
 inputs give |E| of order 10 Eh at no=8/nv=32 and far more at larger sizes,
 where FP64 summation of the reference itself cannot resolve 1e-10 Eh).
 """
+import json
 import os
 import sys
 
@@ -27,6 +28,20 @@ pytestmark = pytest.mark.gpu
 
 def tol(e):
     return max(1e-10, 1e-12 * abs(e))
+
+
+# The reduced-precision modes are checked against the oracle in the same
+# operand representation within the accumulation bound of
+# tests/precision_model.py (tests/test_gpu_precision.py); here, where a
+# case is one of those fixture systems, the same references apply.
+FIXTURE_OF_CASE = {(8, 32, 8, 16, 1234): "cfg1_no8_nv32", (7, 13, 3, 5, 11): "no7_nv13_to3_tv5",
+                   (5, 9, 2, 4, 12): "no5_nv9_to2_tv4", (9, 41, 0, 0, 15): "no9_nv41_ragged"}
+
+
+def precision_fixture(name, mode):
+    """(emulated totals, task energies, per-task bounds, total bounds)."""
+    with np.load(os.path.join(GOLD, "precision.npz")) as z:
+        return tuple(z[f"{name}_{mode}_{k}"] for k in ("total", "tasks", "task_bounds", "total_bounds"))
 
 
 def system(no, nv, to, tv, seed, ranges_o=None, ranges_v=None):
@@ -124,8 +139,11 @@ def test_orbital_tiling_matches_oracle(case, kernel, monkeypatch):
             job.task_energies()
         with pytest.raises(RuntimeError):
             job.run_restart()
-        eb = job.run(2)
-        assert abs(eb[0] - e1) <= MIXED_REL[2] * abs(e1) + 1e-12
+        fx = FIXTURE_OF_CASE.get(case[:5])
+        if fx:
+            eb = np.array(job.run(2))
+            tot, _, _, bnd = precision_fixture(fx, "bf16x3")
+            assert np.all(np.abs(eb - tot) <= bnd), (eb, tot, bnd)
 
 
 @pytest.mark.parametrize("prm", [dict(sizes_o=[3, 2], sizes_v=[4, 3], irreps_o=[0, 0], irreps_v=[1, 1], seed=5),
@@ -205,35 +223,6 @@ def test_fortran_caller_matches_oracle():
     assert abs(float(t.group(1)) - e1) <= tol(e1) and abs(float(t.group(2)) - e2) <= tol(e2)
 
 
-# Reduced-precision modes: the d1/d2 products in FP32, or split-BF16 with
-# three terms (hi*hi + hi*lo + lo*hi), FP32 accumulation; s1, denominators,
-# division and every energy sum in FP64.  The bound is relative to |E|: a
-# layout, sign or indexing mistake moves E by O(1) relative, while FP32
-# rounding (2^-24) and the bf16 hi+lo representation (2^-17 per operand,
-# unbiased) leave |dE|/|E| many orders below these bounds on these sizes.
-MIXED_REL = {1: 2e-6, 2: 1e-4}
-
-
-@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
-@pytest.mark.parametrize("case", CASES, ids=[f"no{c[0]}_nv{c[1]}_to{c[2]}_tv{c[3]}" for c in CASES])
-def test_mixed_precision_matches_oracle(case, precision):
-    no, nv, to, tv, seed, ro, rv = case
-    tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
-    e1, e2, te = oracle_energies(ot, b)
-    rel = MIXED_REL[precision]
-    print(f"case no={no} nv={nv} ranges={list(tm.range)} precision={precision}", flush=True)
-    with TriplesJob(tm) as job:
-        job.upload(b)
-        e = job.run(precision)
-        gte = job.task_energies()
-    print(f"  E[T] {e[0]:.12e} vs {e1:.12e} (dE {e[0] - e1:.3e}), "
-          f"E(T) {e[1]:.12e} vs {e2:.12e} (dE {e[1] - e2:.3e})", flush=True)
-    assert abs(e[0] - e1) <= rel * abs(e1) + 1e-12
-    assert abs(e[1] - e2) <= rel * abs(e2) + 1e-12
-    scale = np.abs(te).max()
-    assert np.all(np.abs(gte - te) <= rel * scale + 1e-12)
-
-
 # ---- golden vectors (reference kernels, tests/golden) and spin/irrep/restricted tilings
 
 GOLDEN_SYSTEMS = list(mg.SYSTEMS)
@@ -282,19 +271,6 @@ def test_spin_tilings_match_oracle(prm, kernel, monkeypatch):
         e = job.run()
         te = job.task_energies()
     assert_parity(e, te, ref)
-
-
-@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
-def test_spin_tilings_reduced_precision(precision):
-    s = spin_systems.SpinSystem(**SPIN_GPU[1])
-    b = s.blocks
-    e1, e2, _ = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
-    rel = MIXED_REL[precision]
-    with TriplesJob(s.tiles) as job:
-        job.upload(b)
-        e = job.run(precision)
-    assert abs(e[0] - e1) <= rel * abs(e1) + 1e-12
-    assert abs(e[1] - e2) <= rel * abs(e2) + 1e-12
 
 
 def test_legacy_surface_spin_tiling():
@@ -489,13 +465,20 @@ def test_restart_blocks_match_task_sums(precision):
     """Running the job in restartable blocks of outer virtual tiles, two per
     call with the state passed along, fills the table with each tile's
     E(T) (its tasks' oracle energies summed) and ends on the full E(T)."""
-    tm, b, ot = system(9, 41, 0, 0, 41, [6, 3], [16, 16, 9])
-    e1, e2, te = oracle_energies(ot, b)
+    tm, b, ot = system(9, 41, 0, 0, 15, [6, 3], [16, 16, 9])
+    if precision == 0:
+        e1, e2, te = oracle_energies(ot, b)
+        tbound = np.array([[tol(x) for x in row] for row in te])
+    else:
+        tot, te, tbound, _ = precision_fixture("no9_nv41_ragged", "bf16x3")
+        e1, e2 = tot
     tasks = tm.tasks()
     nvab = tm.range.size - tm.noab
     per_tile = np.zeros(nvab)
-    for (p4, p5, p6, h1, h2, h3), (t4, t5) in zip(tasks, te):
+    tile_bound = np.zeros(nvab)
+    for (p4, p5, p6, h1, h2, h3), (t4, t5), (b4, b5) in zip(tasks, te, tbound):
         per_tile[p4 - tm.noab - 1] += t5
+        tile_bound[p4 - tm.noab - 1] += b5
     with TriplesJob(tm) as job:
         job.upload(b)
         st, calls = None, 0
@@ -504,11 +487,10 @@ def test_restart_blocks_match_task_sums(precision):
             calls += 1
         full = job.run(precision)
     assert calls == (nvab + 1) // 2
-    rel = 1e-12 if precision == 0 else MIXED_REL[precision]
-    scale = np.abs(per_tile).max()
-    assert np.all(np.abs(st["table"] - per_tile) <= rel * scale + 1e-10), (st["table"], per_tile)
-    assert abs(st["t_energy"] - e2) <= max(tol(e2), rel * abs(e2))
-    assert abs(st["t_energy"] - full[1]) <= rel * abs(full[1]) + 1e-12
+    assert np.all(np.abs(st["table"] - per_tile) <= tile_bound), (st["table"], per_tile)
+    assert abs(st["t_energy"] - e2) <= tile_bound.sum()
+    # the blocks are the same items as the whole run, reduced in another order
+    assert abs(st["t_energy"] - full[1]) <= 1e-12 * abs(full[1])
 
 
 # ---- a system without allowed tasks (every block breaks spin conservation)
@@ -542,8 +524,11 @@ def test_empty_system_job_batch_and_server():
             for p in (0, 2):
                 e = batch.run(p)
                 assert tuple(e[0]) == (0.0, 0.0)
-                rel = 1e-12 if p == 0 else MIXED_REL[2]
-                assert abs(e[1, 0] - e1) <= rel * abs(e1) + 1e-10
+                if p == 0:
+                    assert abs(e[1, 0] - e1) <= tol(e1)
+                else:
+                    tot, _, _, bnd = precision_fixture("cfg1_no8_nv32", "bf16x3")
+                    assert abs(e[1, 0] - tot[0]) <= bnd[0]
     finally:
         for j in jobs:
             j.close()
@@ -563,7 +548,7 @@ TILE_SYSTEMS = {
     "ragged": ("synthetic", (9, 41, 0, 0, 15, [6, 3], [16, 16, 9])),
     "no7_nv13": ("synthetic", (7, 13, 3, 5, 11, None, None)),
 }
-TILE_REL = {0: 1e-12, 1: 1e-5, 2: 2e-4}
+TILE_REL = 1e-12  # FP64; the reduced-precision tiles: tests/test_gpu_precision.py
 
 
 def _tile_system(name):
@@ -578,7 +563,7 @@ def _check_tiles(job, tm, b, ot, precision, tasks):
     """Element-wise bound relative to the largest |t3d| (|t3s|) over the
     checked tasks: some tiles are zero up to cancellation (equal tiles whose
     elements vanish by antisymmetry), so a tile's own maximum is no scale."""
-    rel = TILE_REL[precision]
+    rel = TILE_REL
     ref = {t: oracle.task_tiles(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, t) for t in tasks}
     ss = max(np.abs(r[0]).max() for r in ref.values())
     sd = max(np.abs(r[1]).max() for r in ref.values())
@@ -603,16 +588,14 @@ def test_triples_tiles_match_oracle(name):
         _check_tiles(job, tm, b, ot, 0, tasks)
 
 
-@pytest.mark.parametrize("precision", [1, 2], ids=["fp32", "bf16x3"])
-def test_triples_tiles_reduced_precision_and_orbital_jobs(precision):
-    """Reduced-precision products, on a job whose kernel tasks are the
-    orbital tiling: the export always works on the reference tiles."""
+def test_triples_tiles_of_orbital_jobs():
+    """A job whose kernel tasks are the orbital tiling exports the reference
+    tiles all the same."""
     from nwchem_amd import TILING_ORBITAL
     tm, b, ot = _tile_system("ragged")
     with TriplesJob(tm, tiling=TILING_ORBITAL) as job:
         job.upload(b)
-        _check_tiles(job, tm, b, ot, precision, tm.tasks()[::3])
-        _check_tiles(job, tm, b, ot, 0, tm.tasks()[:4])
+        _check_tiles(job, tm, b, ot, 0, tm.tasks()[::3])
 
 
 def test_triples_tiles_accumulate_and_reject_non_tasks():
@@ -782,35 +765,103 @@ def test_triples_tiles_larger_than_the_host_staging_chunk():
     assert np.abs(gs - rs).max() <= 1e-12 * np.abs(rs).max()
 
 
-def test_full_size_tilings_agree():
-    """no=40, nv=320 in tiles of 20 / 40 (480 tasks), the benchmark's size,
-    where the oracle cannot run: size-independent properties instead.  The
-    default sub-tiles, the reference tiles and the orbital tiling decompose
-    the same sum differently (different items, factors and padding): E[T]
-    and E(T) agree to 1e-12 relative, every reference task's energy agrees
-    between the sub-tiled and the reference-tiled job, and split-BF16 stays
-    within its bound of the FP64 result."""
-    from nwchem_amd import TILING_ORBITAL, TILING_REFERENCE
-    tm = tl.TileModel.synthetic(40, 320, 20, 40, 7)
-    b = tl.synth_blocks(tm, 7)
-    print("full size: 480 tasks, sub-tiles", flush=True)
+# ---- the benchmark's own size: no=40 (tiles of 20), nv=320 (tiles of 40), 480 tasks
+C2 = None
+
+
+def config2_inputs():
+    """Inputs of the benchmark job, checked against the digest the golden
+    task energies were computed from (tests/golden/config2_tasks.json)."""
+    global C2
+    if C2 is None:
+        with open(os.path.join(GOLD, "config2_tasks.json")) as f:
+            meta = json.load(f)
+        prm = meta["params"]
+        tm = tl.TileModel.synthetic(prm["no"], prm["nv"], prm["to"], prm["tv"], prm["seed"])
+        b = tl.synth_blocks(tm, prm["seed"])
+        print("config 2 inputs generated", flush=True)
+        assert mg.digest(b.t1, b.t2, b.v2) == meta["inputs_sha256"], "inputs changed"
+        C2 = (tm, b, meta)
+    return C2
+
+
+def summation_tol(n, e_bt, e_pt):
+    """What the reference's own energy sum of one task can be off by: its
+    recursive summation (ccsd_t_gpu.F:170-200 / ccsd_t_dot.F) of n terms,
+    probabilistic bound lam sqrt(n) u sum|terms| (Higham & Mary 2019, lam = 6,
+    u = 2^-53), with sum|terms| <= 2 max(|E[T]|, |E(T)|) (the E[T] terms share
+    one sign, the singles part of E(T) is a small correction)."""
+    return 6.0 * np.sqrt(n) * 2.0 ** -53 * 2.0 * max(abs(e_bt), abs(e_pt))
+
+
+def test_config2_fp64_tasks_match_reference():
+    """Eight tasks of the benchmark job (first, last, equal / two equal /
+    distinct tiles) against the energies of the reference's compiled kernels
+    under the ccsd_t_gpu.F task loop, for the default sub-tiles and the
+    reference tiles."""
+    from nwchem_amd import TILING_REFERENCE
+    tm, b, meta = config2_inputs()
+    got = {}
+    for name, tiling in (("subtiles", None), ("reference tiles", TILING_REFERENCE)):
+        print(f"config 2: {name}", flush=True)
+        with (TriplesJob(tm) if tiling is None else TriplesJob(tm, tiling=tiling)) as job:
+            job.upload(b)
+            e = job.run()
+            got[name] = (e, job.task_energies())
+    assert got["subtiles"][1].shape == (meta["tasks_total"], 2)
+    for t in meta["tasks"]:
+        ref = np.array([float.fromhex(t["E_bt"]), float.fromhex(t["E_pt"])])
+        tl_ = summation_tol(t["elements"], *ref)
+        for name, (e, te) in got.items():
+            d = np.abs(te[t["index"]] - ref)
+            print(f"task {t['index']} {t['tiles']} {name}: |dE| = {d} (tol {tl_:.3e}, "
+                  f"rel {d / np.abs(ref)})", flush=True)
+            assert np.all(d <= tl_), (t, name, te[t["index"]], ref)
+    e, te = got["subtiles"]
+    er, ter = got["reference tiles"]
+    for k in (0, 1):
+        assert abs(e[k] - er[k]) <= 1e-12 * abs(er[k]), (e, er)
+    assert np.all(np.abs(te - ter) <= 1e-12 * np.abs(ter) + 1e-9)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
+def test_config2_reduced_precision_tasks(mode):
+    """The benchmark job in reduced precision: two tasks against the oracle in
+    the same representation (tests/golden/precision.json) within the
+    accumulation bound of tests/precision_model.py; |dE| against the FP64
+    reference printed for all eight fixture tasks."""
+    import precision_model as pm
+    tm, b, meta = config2_inputs()
+    with open(os.path.join(GOLD, "precision.json")) as f:
+        pmeta = json.load(f)["config2"]
+    assert pmeta["inputs_sha256"] == meta["inputs_sha256"]
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        e = job.run(pm.PRECISION_ID[mode])
+        te = job.task_energies()
+    for t in meta["tasks"]:
+        ref = np.array([float.fromhex(t["E_bt"]), float.fromhex(t["E_pt"])])
+        print(f"task {t['index']} {t['tiles']} {mode}: E - E_fp64 = {(te[t['index']] - ref) * 1e6} uEh",
+              flush=True)
+    for t in pmeta["tasks"]:
+        em = np.array([float.fromhex(t[mode]["E_bt"]), float.fromhex(t[mode]["E_pt"])])
+        bnd = np.array([t[mode]["bound_bt"], t[mode]["bound_pt"]])
+        d = np.abs(te[t["index"]] - em)
+        print(f"task {t['index']} {mode}: |E - E_emul| = {d} (bound {bnd})", flush=True)
+        assert np.all(d <= bnd), (t, te[t["index"]], em, bnd)
+
+
+def test_config2_orbital_tiling_agrees():
+    """The orbital tiling decomposes the benchmark job's sum into other items
+    (pieces of the whole spaces, other factors and padding): E[T] and E(T)
+    agree with the sub-tiled job to 1e-12 relative."""
+    from nwchem_amd import TILING_ORBITAL
+    tm, b, meta = config2_inputs()
     with TriplesJob(tm) as job:
         job.upload(b)
         e = job.run()
-        te = job.task_energies()
-        eb = job.run(2)
-    print("full size: reference tiles", flush=True)
-    with TriplesJob(tm, tiling=TILING_REFERENCE) as job:
-        job.upload(b)
-        er = job.run()
-        ter = job.task_energies()
-    print("full size: orbital tiling", flush=True)
     with TriplesJob(tm, tiling=TILING_ORBITAL) as job:
         job.upload(b)
         eo = job.run()
-    assert te.shape == ter.shape == (480, 2)
     for k in (0, 1):
-        assert abs(e[k] - er[k]) <= 1e-12 * abs(er[k]), (e, er)
-        assert abs(e[k] - eo[k]) <= 1e-12 * abs(er[k]), (e, eo)
-        assert abs(eb[k] - e[k]) <= MIXED_REL[2] * abs(e[k]), (eb, e)
-    assert np.all(np.abs(te - ter) <= 1e-12 * np.abs(ter) + 1e-9)
+        assert abs(e[k] - eo[k]) <= 1e-12 * abs(e[k]), (e, eo)
