@@ -227,7 +227,7 @@ def config1(threads: int) -> dict | None:
         # reduced-precision modes at a physical energy scale (|E| ~ 10 Eh):
         # |dE| against the FP64 result, in micro-Hartree
         out["precision_modes_dE_uEh"] = {}
-        for name, p in (("fp32", 1), ("bf16x3", 2)):
+        for name, p in (("fp32", 1), ("bf16x3", 2), ("bf16x6", 3)):
             em = j.run(p)
             out["precision_modes_dE_uEh"][name] = [(em[0] - e[0]) * 1e6, (em[1] - e[1]) * 1e6]
     out["dE_gpu_vs_cpu"] = [e[0] - out["cpu_1core"]["E_bt"], e[1] - out["cpu_1core"]["E_pt"]]
@@ -238,7 +238,7 @@ def config1(threads: int) -> dict | None:
     return out
 
 
-def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
+def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2, 3)) -> dict:
     """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256}, occupied tiles
     6-20, virtual tiles 16-48, ragged last tiles) in one persistent launch per
     run (autotuned, replayed as a hipGraph), against the same jobs run one
@@ -265,7 +265,7 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2)) -> dict:
               f"ready at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
     out = {"jobs": njobs, "flops_reference_tiling": F, "prep_s": t_prep}
-    names = {0: "fp64", 1: "fp32", 2: "bf16x3"}
+    names = {0: "fp64", 1: "fp32", 2: "bf16x3", 3: "bf16x6"}
     with TriplesBatch(jobs) as batch:
         for p in modes:
             print(f"[bench] batch {names[p]}: planning (one timed run per candidate, graph capture)",
@@ -308,7 +308,7 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
                 rates: dict | None = None) -> dict:
     """Serving: a Poisson stream (rate lam) of requests for a pool of seeded
     systems (no in [6,40], nv in [48,320], log-uniform; tiles of 20/40 by the
-    tce_tile rule), each FP64 or split-BF16 at random; systems are uploaded
+    tce_tile rule), each FP64 or 6-term split-BF16 at random; systems are uploaded
     on first use and stay resident.  Latency = finish - submit."""
     from nwchem_amd import TriplesServer, tiles as tl
 
@@ -319,13 +319,13 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
         no = int(round(np.exp(rng.uniform(np.log(6), np.log(40)))))
         nv = int(round(np.exp(rng.uniform(np.log(48), np.log(320)))))
         tm = tl.TileModel.synthetic(no, nv, min(no, 20), min(nv, 40), seed + 100 + k)
-        pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), {p: job_work(tm, p) for p in (0, 2)}))
+        pool.append((tm, tl.synth_blocks(tm, seed + 100 + k), {p: job_work(tm, p) for p in (0, 3)}))
         print(f"[bench] serve system {k}: no={no} nv={nv} ready at {time.perf_counter() - t0:.1f} s",
               file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
     arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
     picks = rng.integers(0, npool, nreq)
-    precs = rng.choice([0, 2], nreq)
+    precs = rng.choice([0, 3], nreq)  # FP64 or split-BF16 at <= 1 uEh (6 terms)
     lat, F = [], 0.0
     with TriplesServer(slice_ms=slice_ms, max_active=16) as srv:
         sids = [srv.add_system(tm, b) for tm, b, f in pool]
@@ -365,7 +365,7 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
     q = max(1, nreq // 4)
     arrival_rate = nreq / float(arrivals[-1]) if arrivals[-1] > 0 else None
     out = {"requests": nreq, "lambda_per_s": lam, "pool_systems": npool,
-           "precision_mix": {"fp64": int((precs == 0).sum()), "bf16x3": int((precs == 2).sum())},
+           "precision_mix": {"fp64": int((precs == 0).sum()), "bf16x6": int((precs == 3).sum())},
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(lat.mean()), "wall_s": wall, "energies_per_s": nreq / wall,
            "arrival_rate_per_s": arrival_rate, "completion_rate_per_s": nreq / wall,
@@ -410,7 +410,7 @@ def main() -> int:
                     help="also time the job with the reference tiles as kernel tasks (0: skip)")
     ap.add_argument("--task-tiles", type=int, default=1,
                     help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
-    ap.add_argument("--modes", default="fp32,bf16x3",
+    ap.add_argument("--modes", default="fp32,bf16x3,bf16x6",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -455,7 +455,7 @@ def main() -> int:
           f"({F / F_run:.2f}x less than the reference tiling's F_ref)", file=sys.stderr, flush=True)
     modes = {}
     for name in [m for m in args.modes.split(",") if m]:
-        prec = {"fp32": 1, "bf16x3": 2}[name]
+        prec = {"fp32": 1, "bf16x3": 2, "bf16x6": 3}[name]
         t1 = time.perf_counter()
         job.run(prec)  # converts the contraction tensors on first use (untimed)
         t_prep = time.perf_counter() - t1
@@ -540,8 +540,8 @@ def main() -> int:
                 "items_per_launch": items}
     batch = bench_batch(args.batch_jobs, args.seed, max(1, args.batch_steps)) if args.batch_jobs else None
     rates = {0: F_run * args.steps / wall}
-    if "bf16x3" in modes:
-        rates[2] = modes["bf16x3"]["fp64_equiv_tflops"] * 1e12
+    if "bf16x6" in modes:
+        rates[3] = modes["bf16x6"]["fp64_equiv_tflops"] * 1e12
     serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms,
                         rates if len(rates) == 2 else None) if args.serve else None
     cpu = None

@@ -80,6 +80,44 @@ void sd_t_d2_7_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long
 void sd_t_d2_8_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
 void sd_t_d2_9_cuda_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d, long *p7d, double *t3, double *t2, double *v2);
 
+/* The reference's CPU kernel surface (ccsd_t_kernels_omp.F:5-1198, argument
+ * order h3d, h2d, h1d, p6d, p5d, p4d[, h7d | p7d], triplesx, t1sub | t2sub,
+ * v2sub): the same 27 contractions accumulated into the caller's HOST
+ * triplesx (not zeroed; its layout the variant's triplesx declaration), for
+ * the CPU drivers ccsd_t_singles_l.F / offl_ccsd_t_doubles_l.F and their
+ * other callers (cr_ccsd_t.F, lambda_ccsd_t*.F, emb/emb2.F, the MRCC (T)).
+ * Each call stages triplesx and the operands through the device and back
+ * (PCIe-bound for large tiles); the whole-job and task-tile entry points
+ * below are the fast paths.  These names replace ccsd_t_kernels_omp.o in
+ * a link. */
+void sd_t_s1_1_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_2_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_3_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_4_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_5_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_6_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_7_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_8_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_s1_9_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, double *triplesx, double *t1sub, double *v2sub);
+void sd_t_d1_1_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_2_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_3_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_4_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_5_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_6_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_7_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_8_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d1_9_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *h7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_1_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_2_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_3_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_4_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_5_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_6_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_7_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_8_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+void sd_t_d2_9_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d, long *p4d, long *p7d, double *triplesx, double *t2sub, double *v2sub);
+
 /* sd_t_total.cu:5373-5379 -- energy of the current task's device tiles:
  * energy[0] = E[T] part, energy[1] = E(T) part; eval1..6 point at the
  * orbital energies of tiles h1,h2,h3,p4,p5,p6 (host); host1/host2 unused. */
@@ -143,6 +181,10 @@ typedef struct {
 #define TCE_T_BF16X3 2      /* each operand x = hi + lo (two bf16), the  */
                             /* products hi*hi + hi*lo + lo*hi on BF16    */
                             /* MFMA with FP32 accumulation               */
+#define TCE_T_BF16X6 3      /* x = hi + mid + lo (three bf16), the six   */
+                            /* products down to 2^-24 relative (hi*hi,   */
+                            /* hi*mid, mid*hi, hi*lo, mid*mid, lo*hi) on */
+                            /* BF16 MFMA with FP32 accumulation          */
 
 typedef struct tce_t_job tce_t_job;
 
@@ -195,7 +237,8 @@ int tce_t_job_upload(tce_t_job *job, const double *t1, const long *k_t1_offset,
                      const double *v2, const long *k_v2_offset);
 
 /* Run every task; *e_bt = E[T] (energy1), *e_pt = E(T) (energy2) of
- * ccsd_t_gpu.F:243-244.  precision: TCE_T_FP64, TCE_T_FP32 or TCE_T_BF16X3
+ * ccsd_t_gpu.F:243-244.  precision: TCE_T_FP64, TCE_T_FP32, TCE_T_BF16X3 or
+ * TCE_T_BF16X6
  * (the first run in a reduced precision converts the contraction tensors on
  * the device; see tce_t_job_precision_bytes).  Synchronous. */
 int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double *e_pt);
@@ -226,6 +269,18 @@ int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin, double *ta
  * uploaded again; tce_t_job_device_bytes counts it. */
 int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b, long h3b,
                             long p4b, long p5b, long p6b, double *singles, double *doubles);
+/* The memory-sliced form of the same (ccsd_t_6dts.F:238-266, the xmem path):
+ * ccsd_t_singles_l_6dts / ccsd_t_doubles_l_6dts for the slice p4 in
+ * [istart, istop], p5 in [jstart, jstop] (1-based within tiles p4b, p5b) of
+ * the task's t3s / t3d, added into singles / doubles laid out
+ * triplesx(h3,h2,h1,p6,maxp5,maxp4) as the 6DTS kernels
+ * (ccsd_t_dts_kernels_omp.F) leave it.  istop - istart + 1 <= maxp4 and
+ * jstop - jstart + 1 <= maxp5 (else TCE_T_ERR_ARG).  Same elements, bit for
+ * bit, as the corresponding part of tce_t_job_triples_tiles. */
+int tce_t_job_triples_tiles_6dts(tce_t_job *job, int precision, long h1b, long h2b, long h3b,
+                                 long p4b, long p5b, long p6b, long istart, long istop,
+                                 long jstart, long jstop, long maxp4, long maxp5,
+                                 double *singles, double *doubles);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);
@@ -245,12 +300,13 @@ int tce_t_pack_dense(const tce_t_tiles *tiles, const double *t1, const long *k_t
  * 1 = Vd2 (d2 operands), 2 = Wd1, 3 = T2m (d1 operands), laid out as
  * tce_t_pack_dense's tensors with the contraction extent zero-padded to
  * *kout (a multiple of the kernel's chunk).  FP32: out0 gets *rows x *kout
- * floats (out1 must be NULL); BF16X3: out0 the bf16 hi and out1 the bf16 lo
- * parts (*rows x *kout 16-bit patterns each; either may be NULL).  Both
- * outputs NULL: only *rows and *kout are set (no upload needed).  Converts
- * the tensors first if no run in `precision` has yet.  Synchronous. */
+ * floats (out1, out2 must be NULL); BF16X3: out0 the bf16 hi and out1 the
+ * bf16 lo parts (*rows x *kout 16-bit patterns each; out2 must be NULL);
+ * BF16X6: hi, mid, lo in out0..2.  Any output may be NULL; all NULL: only
+ * *rows and *kout are set (no upload needed).  Converts the tensors first if
+ * no run in `precision` has yet.  Synchronous. */
 int tce_t_job_operands(tce_t_job *job, int precision, int tensor, void *out0, void *out1,
-                       long *rows, long *kout);
+                       void *out2, long *rows, long *kout);
 
 /* Device time (ms) of the last run's contraction kernel(s), HIP events. */
 double tce_t_job_last_kernel_ms(const tce_t_job *job);
@@ -358,6 +414,18 @@ int tce_t_synth_blocks(const tce_t_tiles *tiles, unsigned long long seed,
  * = <m p||h h'>, V[nv][nv][no][nv] = <p p'||h e>, S[nv][nv][no][no]. */
 int tce_t_synth_dense(unsigned long long seed, long no, long nv, double *T1,
                       double *T2, double *W, double *V, double *S);
+/* Back end of the per-call d1/d2 contractions (groups 1 and the CPU
+ * surface): DIRECT, one thread per t3 element with the contraction loop
+ * (default), or TTGT, the alternative of sd_t_total_ttlg.cu: one DGEMM on
+ * the FP64 matrix cores (rocBLAS) into a scratch matrix, then a transpose
+ * kernel adding it into the tile in the variant's layout.  Same results up
+ * to FP64 summation order.  TCE_T_LEGACY=ttgt in the environment selects
+ * TTGT at the first call.  Returns the back end in force (QUERY: only that),
+ * or TCE_T_ERR_ARG. */
+#define TCE_T_LEGACY_DIRECT 0
+#define TCE_T_LEGACY_TTGT 1
+#define TCE_T_LEGACY_QUERY (-1)
+int tce_t_legacy_backend(int backend);
 /* Sticky failure flag of the per-call surface (group 1), which has no return
  * codes: TCE_T_ERR_HIP if any call failed since the last query, then reset. */
 int tce_t_legacy_status(void);

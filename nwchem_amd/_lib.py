@@ -73,7 +73,7 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_precision_bytes.argtypes = [C.c_void_p, C.c_int]
     L.tce_t_job_precision_bytes.restype = C.c_long
     L.tce_t_job_operands.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
-                                     C.POINTER(C.c_long), C.POINTER(C.c_long)]
+                                     C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.tce_t_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.tce_t_batch_run.argtypes = [C.c_void_p, C.c_int, DP]
     L.tce_t_batch_last_ms.argtypes = [C.c_void_p]
@@ -89,6 +89,7 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_server_destroy.argtypes = [C.c_void_p]
     L.tce_t_job_run_restart.argtypes = [C.c_void_p, C.c_int, LP, DP, C.c_long, DP]
     L.tce_t_job_triples_tiles.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 6 + [DP, DP]
+    L.tce_t_job_triples_tiles_6dts.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 12 + [DP, DP]
     L.tce_t_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.tce_t_job_kernel.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.restype = C.c_int
@@ -110,10 +111,17 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_philox4x32.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
     L.tce_t_philox4x32.restype = None
     L.tce_t_legacy_status.restype = C.c_int
+    L.tce_t_legacy_backend.argtypes = [C.c_int]
+    L.tce_t_legacy_backend.restype = C.c_int
     LPtr = C.POINTER(C.c_long)
     for fam, nd in (("s1", 6), ("d1", 7), ("d2", 7)):
         for n in range(1, 10):
             fn = getattr(L, f"sd_t_{fam}_{n}_cuda_")
+            fn.argtypes = [LPtr] * nd + [DP, DP, DP]
+            fn.restype = None
+    for fam, nd in (("s1", 6), ("d1", 7), ("d2", 7)):  # the CPU surface
+        for n in range(1, 10):
+            fn = getattr(L, f"sd_t_{fam}_{n}_")
             fn.argtypes = [LPtr] * nd + [DP, DP, DP]
             fn.restype = None
     L.compute_en_.argtypes = [DP, DP] + [DP] * 6 + [LPtr] * 6 + [DP, DP]

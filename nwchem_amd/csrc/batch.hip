@@ -67,7 +67,7 @@ struct Batch {
   std::vector<long> eoff;          // per job: first task slot in hE
   std::vector<long> gens;          // per job: upload generation the plans were made for
   long ntasks = 0;
-  Plan plan[3];
+  Plan plan[NPREC];
   float last_ms = 0.f;
   int cus = 1;
 };
@@ -149,9 +149,12 @@ int enqueue(Batch &B, int precision, const Plan &P) {
   const char *segs = (const char *)P.tab + n * asz;
   if (P.total > 0) {
     int rc;
-    if (mixed)
+    if (mixed) {
+      bool sparse = false;
+      for (const Job *j : B.jobs) sparse = sparse || j->sparse;
       rc = launch_fused_mixed((const MixedSegment *)segs, P.nseg, P.total, precision, P.grid,
-                              B.stream);
+                              B.stream, sparse);
+    }
     else if (P.choice.family == FAM_WG)
       rc = launch_fused_wg((const Segment *)segs, P.nseg, P.total, P.grid, B.stream);
     else
@@ -285,7 +288,7 @@ extern "C" int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch
 
 extern "C" int tce_t_batch_run(tce_t_batch *batch, int precision, double *energies) {
   if (!batch) return TCE_T_ERR_ARG;
-  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+  if (!valid_precision(precision)) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
   }
@@ -330,7 +333,7 @@ extern "C" double tce_t_batch_last_ms(const tce_t_batch *batch) {
 
 extern "C" int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family,
                                 int *cost_order, double *tuned_ms) {
-  if (!batch || precision < 0 || precision > 2) return TCE_T_ERR_ARG;
+  if (!batch || !valid_precision(precision)) return TCE_T_ERR_ARG;
   const Plan &P = reinterpret_cast<const Batch *>(batch)->plan[precision];
   if (!P.ready) return TCE_T_ERR_STATE;
   if (family) *family = P.choice.family;

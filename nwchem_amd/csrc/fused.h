@@ -35,6 +35,40 @@
 
 namespace tce_t {
 
+// The 18 d1/d2 contractions of fused_mixed.hip in its pass order: kind
+// (0 = d2, 1 = d1), lone particle L (0 a, 1 b, 2 c), lone hole h (0 i, 1 j,
+// 2 k), sign (1 = minus).  The contraction index x (e of d2, m of d1) pairs
+// with L and the hole pair (r, s) = the two holes other than h on one
+// operand (t(L e r s), <m L||r s>), so spin and irrep conservation of that
+// block fix x's spin to spin(r) + spin(s) - spin(L) and its irrep to
+// irrep(L) ^ irrep(r) ^ irrep(s).
+struct PassDef {
+  int kind, L, h, minus;
+};
+constexpr PassDef MIXED_PASS[18] = {
+    {0, 0, 2, 1},  // d2 N=1: -t(a e i j)<b c||k e>
+    {1, 0, 2, 1},  // d1 N=6: -t(b c k m)<m a||i j>
+    {0, 0, 1, 0},  // d2 N=3: +t(a e i k)<b c||j e>
+    {1, 0, 1, 0},  // d1 N=5: +t(b c j m)<m a||i k>
+    {0, 0, 0, 1},  // d2 N=2: -t(a e j k)<b c||i e>
+    {1, 0, 0, 1},  // d1 N=4: -t(b c i m)<m a||j k>
+    {0, 1, 0, 0},  // d2 N=5: +t(b e j k)<a c||i e>
+    {1, 1, 0, 0},  // d1 N=7: +t(a c i m)<m b||j k>
+    {0, 1, 2, 0},  // d2 N=4: +t(b e i j)<a c||k e>
+    {1, 1, 2, 0},  // d1 N=9: +t(a c k m)<m b||i j>
+    {0, 1, 1, 1},  // d2 N=6: -t(b e i k)<a c||j e>
+    {1, 1, 1, 1},  // d1 N=8: -t(a c j m)<m b||i k>
+    {0, 2, 2, 1},  // d2 N=7: -t(c e i j)<a b||k e>
+    {1, 2, 2, 1},  // d1 N=3: -t(a b k m)<m c||i j>
+    {0, 2, 1, 0},  // d2 N=9: +t(c e i k)<a b||j e>
+    {1, 2, 1, 0},  // d1 N=2: +t(a b j m)<m c||i k>
+    {0, 2, 0, 1},  // d2 N=8: -t(c e j k)<a b||i e>
+    {1, 2, 0, 1},  // d1 N=1: -t(a b i m)<m c||j k>
+};
+// hole pair (r, s) of lone hole h
+constexpr int pair_r(int h) { return h == 0 ? 1 : 0; }
+constexpr int pair_s(int h) { return h == 2 ? 1 : 2; }
+
 struct TaskDev {
   int op4, op5, op6;  // particle-space offsets of the task's particle tiles
   int oh1, oh2, oh3;  // hole offsets
@@ -42,6 +76,12 @@ struct TaskDev {
   int nb4, nb5, nb6, nb1, nb2, nb3;  // boxes per index (particle box edge pbox, hole 4)
   long item0;                        // first work item of the task
   double factor;
+  // Contraction range of pass p of MIXED_PASS: orbitals [x0[p], x1[p]) of
+  // the virtual (d2) or occupied (d1) space hold every orbital of the spin
+  // and irrep the pass's blocks allow (the reference skips the other
+  // h7b / p7b tiles, ccsd_t_doubles_gpu.F:260-263, 924-927); outside it
+  // every operand of the pass is zero.  x0 = x1 = 0: no orbital allowed.
+  unsigned short x0[18], x1[18];
 };
 
 struct FusedArgs {
@@ -55,12 +95,12 @@ struct FusedArgs {
 
 // Reduced-precision operands of the d1/d2 contractions (fused_mixed.hip):
 // the dense tensors of FusedArgs converted to FP32 (part 0) or split into
-// bf16 hi (part 0) and lo (part 1), contraction index padded with zeros to
-// Ek (d2) / Mk (d1) elements.
+// bf16 hi (part 0) and lo (part 1), or hi, mid, lo (parts 0-2), contraction
+// index padded with zeros to Ek (d2) / Mk (d1) elements.
 struct MixedArgs {
   FusedArgs f;
-  const void *x2[2], *y2[2];  // d2: T2e, Vd2
-  const void *x1[2], *y1[2];  // d1: Wd1, T2m
+  const void *x2[3], *y2[3];  // d2: T2e, Vd2 (parts: FP32 | hi, lo | hi, mid, lo)
+  const void *x1[3], *y1[3];  // d1: Wd1, T2m
   int Ek, Mk;
 };
 
@@ -95,8 +135,8 @@ struct FusedArgsG {
 };
 struct MixedArgsG {
   FusedArgsG f;
-  gptr<const void> x2[2], y2[2];
-  gptr<const void> x1[2], y1[2];
+  gptr<const void> x2[3], y2[3];
+  gptr<const void> x1[3], y1[3];
   int Ek, Mk;
 };
 static_assert(sizeof(FusedArgsG) == sizeof(FusedArgs), "argument mirror");
@@ -127,7 +167,7 @@ __device__ __forceinline__ MixedArgsG load_args(const MixedArgs *p) {
   const gptr<const MixedArgs> q = (gptr<const MixedArgs>)p;
   MixedArgsG M;
   M.f = load_args(&p->f);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < 3; ++i) {
     M.x2[i] = (gptr<const void>)q->x2[i];
     M.y2[i] = (gptr<const void>)q->y2[i];
     M.x1[i] = (gptr<const void>)q->x1[i];
@@ -300,11 +340,13 @@ __device__ __forceinline__ int find_task(gptr<const TaskDev> tasks, int ntasks, 
 // segments (device table), `grid` persistent workgroups.
 int launch_fused_wave(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
 int launch_fused_wg(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
+// sparse: some job of the table narrows its passes' contraction ranges
+// (TaskDev::x0/x1, spin- or irrep-labelled tiles).
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
-                       hipStream_t s);
+                       hipStream_t s, bool sparse);
 // One task's t3 tiles (fused_mixed.hip): `total` items of the single task at
 // args->f.tasks[0], tiles written to args->f.partials (doubles, then singles).
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
-                       hipStream_t s);
+                       hipStream_t s, bool sparse);
 
 }  // namespace tce_t

@@ -10,6 +10,10 @@
 //   TCE_T_BF16X3  v_mfma_f32_16x16x32_bf16, each operand x split into
 //                 hi = bf16(x), lo = bf16(x - hi); per product hi*hi + hi*lo
 //                 + lo*hi (3 terms), FP32 accumulation
+//   TCE_T_BF16X6  the same MFMA, x = hi + mid + lo (three bf16, 24 bits);
+//                 per product the 6 terms down to 2^-24 relative (hi*hi,
+//                 hi*mid, mid*hi, hi*lo, mid*mid, lo*hi), FP32 accumulation;
+//                 two LDS chunk buffers (three parts of 16 KB per chunk)
 //
 // Work item and workgroup as fused_wg.hip: one task x an 8x8x8 particle box x
 // a 4x4x4 hole box, 512 threads (8 waves, 2 per SIMD); wave w owns the 4x4x4
@@ -81,6 +85,15 @@ struct Prec<TCE_T_BF16X3> {
   static constexpr bool SPLIT = false;  // measured 20% slower split (64 fragment VGPRs)
   typedef float acc;
 };
+template <>
+struct Prec<TCE_T_BF16X6> {
+  static constexpr int KC = 32;
+  static constexpr int NPART = 3;  // hi, mid, lo
+  static constexpr int ES = 2;
+  static constexpr int NBUF = 2;   // 3 x 48 KB do not fit the 160 KB of LDS
+  static constexpr bool SPLIT = false;
+  typedef float acc;
+};
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
@@ -105,33 +118,13 @@ constexpr int lds_quads() {
   return nb * buf_quads<PREC>() > EPI_QUADS ? nb * buf_quads<PREC>() : EPI_QUADS;
 }
 
-// The 18 passes: kind (0 = d2, 1 = d1), lone particle L (a b c), lone hole
-// h (i j k), sign.  d2: X = t(L e r s) (T2e), Y = <p1 p2||h e> (Vd2); d1:
-// X = <m L||r s> (Wd1), Y = t(p1 p2 h m) (T2m); signs and pairs as the
-// variant table in fused.h (ccsd_t_kernels_omp.F layouts).
-struct PassInfo {
-  int kind, L, h, minus;
-};
-constexpr PassInfo PASS[18] = {
-    {0, 0, 2, 1},  // d2 N=1: -t(a e i j)<b c||k e>
-    {1, 0, 2, 1},  // d1 N=6: -t(b c k m)<m a||i j>
-    {0, 0, 1, 0},  // d2 N=3: +t(a e i k)<b c||j e>
-    {1, 0, 1, 0},  // d1 N=5: +t(b c j m)<m a||i k>
-    {0, 0, 0, 1},  // d2 N=2: -t(a e j k)<b c||i e>
-    {1, 0, 0, 1},  // d1 N=4: -t(b c i m)<m a||j k>
-    {0, 1, 0, 0},  // d2 N=5: +t(b e j k)<a c||i e>
-    {1, 1, 0, 0},  // d1 N=7: +t(a c i m)<m b||j k>
-    {0, 1, 2, 0},  // d2 N=4: +t(b e i j)<a c||k e>
-    {1, 1, 2, 0},  // d1 N=9: +t(a c k m)<m b||i j>
-    {0, 1, 1, 1},  // d2 N=6: -t(b e i k)<a c||j e>
-    {1, 1, 1, 1},  // d1 N=8: -t(a c j m)<m b||i k>
-    {0, 2, 2, 1},  // d2 N=7: -t(c e i j)<a b||k e>
-    {1, 2, 2, 1},  // d1 N=3: -t(a b k m)<m c||i j>
-    {0, 2, 1, 0},  // d2 N=9: +t(c e i k)<a b||j e>
-    {1, 2, 1, 0},  // d1 N=2: +t(a b j m)<m c||i k>
-    {0, 2, 0, 1},  // d2 N=8: -t(c e j k)<a b||i e>
-    {1, 2, 0, 1},  // d1 N=1: -t(a b i m)<m c||j k>
-};
+// The 18 passes (fused.h, MIXED_PASS): kind (0 = d2, 1 = d1), lone particle
+// L (a b c), lone hole h (i j k), sign.  d2: X = t(L e r s) (T2e), Y =
+// <p1 p2||h e> (Vd2); d1: X = <m L||r s> (Wd1), Y = t(p1 p2 h m) (T2m);
+// signs and pairs as the variant table in fused.h (ccsd_t_kernels_omp.F
+// layouts).
+typedef PassDef PassInfo;
+#define PASS MIXED_PASS
 
 // Hole box of an item: global hole o + min(b0 + u, rm), u = 0..3.
 struct HoleBox {
@@ -206,7 +199,7 @@ __device__ __forceinline__ Src<PREC> pass_src(const MixedArgsG &M, const ItemGeo
   using Pr = Prec<PREC>;
   constexpr PassInfo pi = PASS[P];
   constexpr int L = pi.L, h = pi.h;
-  constexpr int r = h == 0 ? 1 : 0, s = h == 2 ? 1 : 2, p1 = L == 0 ? 1 : 0, p2 = L == 2 ? 1 : 2;
+  constexpr int r = pair_r(h), s = pair_s(h), p1 = L == 0 ? 1 : 0, p2 = L == 2 ? 1 : 2;
   constexpr int QE = 16 / Pr::ES;  // elements per quad
   const int no = M.f.no, nv = M.f.nv;
   const long Kp = pi.kind ? M.Mk : M.Ek;
@@ -275,6 +268,10 @@ template <>
 struct Frag<TCE_T_BF16X3> {
   bf8 x[4], y[4], xl[4], yl[4];
 };
+template <>
+struct Frag<TCE_T_BF16X6> {
+  bf8 x[4], y[4], xm[4], ym[4], xl[4], yl[4];
+};
 
 // X quads S0..S1-1 (offset xa for s = 0, s adds 128 quads) and the Y quads
 // (offset yb for h = 0, h adds 256) of the chunk in buffer cb.
@@ -286,9 +283,13 @@ __device__ __forceinline__ void read_x(Frag<PREC> &F, const f4 *cb, int xa) {
       F.x[s] = reinterpret_cast<const dv2 *>(cb)[xa + 128 * s];
     } else if constexpr (PREC == TCE_T_FP32) {
       F.x[s] = cb[xa + 128 * s];
-    } else {
+    } else if constexpr (PREC == TCE_T_BF16X3) {
       F.x[s] = reinterpret_cast<const bf8 *>(cb)[xa + 128 * s];
       F.xl[s] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + xa + 128 * s];
+    } else {
+      F.x[s] = reinterpret_cast<const bf8 *>(cb)[xa + 128 * s];
+      F.xm[s] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + xa + 128 * s];
+      F.xl[s] = reinterpret_cast<const bf8 *>(cb)[2 * PART_QUADS + xa + 128 * s];
     }
   }
 }
@@ -300,9 +301,13 @@ __device__ __forceinline__ void read_y(Frag<PREC> &F, const f4 *cb, int yb) {
       F.y[h] = reinterpret_cast<const dv2 *>(cb)[X_QUADS + yb + 256 * h];
     } else if constexpr (PREC == TCE_T_FP32) {
       F.y[h] = cb[X_QUADS + yb + 256 * h];
-    } else {
+    } else if constexpr (PREC == TCE_T_BF16X3) {
       F.y[h] = reinterpret_cast<const bf8 *>(cb)[X_QUADS + yb + 256 * h];
       F.yl[h] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + X_QUADS + yb + 256 * h];
+    } else {
+      F.y[h] = reinterpret_cast<const bf8 *>(cb)[X_QUADS + yb + 256 * h];
+      F.ym[h] = reinterpret_cast<const bf8 *>(cb)[PART_QUADS + X_QUADS + yb + 256 * h];
+      F.yl[h] = reinterpret_cast<const bf8 *>(cb)[2 * PART_QUADS + X_QUADS + yb + 256 * h];
     }
   }
 }
@@ -347,6 +352,10 @@ template <>
 struct AccQ<TCE_T_BF16X3> {
   typedef f4 type;
 };
+template <>
+struct AccQ<TCE_T_BF16X6> {
+  typedef f4 type;
+};
 
 template <int PREC, int H>
 __device__ __forceinline__ void to_quads(typename AccQ<PREC>::type (&q)[16],
@@ -384,6 +393,18 @@ __device__ __forceinline__ void mma_q(typename AccQ<PREC>::type (&q)[16], const 
         for (int h = 0; h < 4; ++h) {
           const bf8 xa = term == 0 ? F.xl[s] : F.x[s];
           const bf8 yb = term == 1 ? F.yl[h] : F.y[h];
+          q[4 * s + h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, yb, q[4 * s + h], 0, 0, 0);
+        }
+  } else if constexpr (PREC == TCE_T_BF16X6) {
+    // smallest terms first: lo*hi', hi*lo', mid*mid', mid*hi', hi*mid', hi*hi'
+#pragma unroll
+    for (int term = 0; term < 6; ++term)
+#pragma unroll
+      for (int s = S0; s < S1; ++s)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const bf8 xa = term == 0 ? F.xl[s] : ((term == 2 || term == 3) ? F.xm[s] : F.x[s]);
+          const bf8 yb = term == 1 ? F.yl[h] : ((term == 2 || term == 4) ? F.ym[h] : F.y[h]);
           q[4 * s + h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, yb, q[4 * s + h], 0, 0, 0);
         }
   } else {
@@ -431,25 +452,41 @@ __device__ __forceinline__ void negate(T (&a)[64]) {
   for (int r = 0; r < 64; ++r) a[r] = -a[r];
 }
 
-template <int PREC>
-__device__ __forceinline__ constexpr int chunks_of(const MixedArgsG &M, int p) {
-  return p >= 18 ? 0 : (PASS[p].kind ? M.Mk : M.Ek) / Prec<PREC>::KC;
+// Chunks of pass p of the item's task: the chunks covering the task's
+// contraction range of the pass (every operand outside it is zero), first
+// chunk c0, at least one (a pass without allowed orbitals then contracts
+// a chunk of zeros).
+struct PassChunks {
+  int c0, n;
+};
+// SP = false (jobs without spin or irrep labels: every range is the whole
+// space) takes the padded extents and never reads the ranges.
+template <int PREC, bool SP>
+__device__ __forceinline__ PassChunks chunks_of(const MixedArgsG &M, gptr<const TaskDev> T, int p) {
+  constexpr int KC = Prec<PREC>::KC;
+  if (p >= 18) return PassChunks{0, 0};
+  if constexpr (!SP) {
+    return PassChunks{0, (PASS[p].kind ? M.Mk : M.Ek) / KC};
+  } else {
+    const int c0 = T->x0[p] / KC, c1 = (T->x1[p] + KC - 1) / KC;
+    return PassChunks{c0, c1 > c0 ? c1 - c0 : 1};
+  }
 }
 
 // Load global chunk `c2` of the pass sequence that starts at pass P (chunk
 // c2 of pass P, or of one of the next D passes when P has fewer chunks;
 // every pass has at least one) into buffer nb.
-template <int PREC, int P, int D>
+template <int PREC, bool SP, int P, int D>
 __device__ __forceinline__ void load_ahead(const Src<PREC> &s0, const MixedArgsG &M,
-                                           const ItemGeo &G, int wave, int lane, int c2,
-                                           unsigned lds_base, int nb) {
+                                           gptr<const TaskDev> T, const ItemGeo &G, int wave,
+                                           int lane, int c2, unsigned lds_base, int nb) {
   using Pr = Prec<PREC>;
-  const int n = chunks_of<PREC>(M, P);
-  if (c2 < n) {
-    load_chunk<PREC>(s0, (long)c2 * Pr::KC * Pr::ES, lds_base, nb, wave);
+  const PassChunks pc = chunks_of<PREC, SP>(M, T, P);
+  if (c2 < pc.n) {
+    load_chunk<PREC>(s0, (long)(pc.c0 + c2) * Pr::KC * Pr::ES, lds_base, nb, wave);
   } else if constexpr (D > 0 && P + 1 < 18) {
-    load_ahead<PREC, P + 1, D - 1>(pass_src<PREC, P + 1>(M, G, wave, lane), M, G, wave, lane,
-                                   c2 - n, lds_base, nb);
+    load_ahead<PREC, SP, P + 1, D - 1>(pass_src<PREC, P + 1>(M, G, wave, lane), M, T, G, wave,
+                                       lane, c2 - pc.n, lds_base, nb);
   }
 }
 
@@ -472,10 +509,10 @@ __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F
 // and its MFMAs run, with the loads of chunk j+2 (into the buffer freed by
 // the last barrier) issued halfway through them; then this thread's loads
 // of chunk j+1 complete and the barrier publishes it.
-template <int PREC, int P>
+template <int PREC, bool SP, int P>
 __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
-                                          unsigned lds_base,
-                                          const MixedArgsG &M, const ItemGeo &G, int wave,
+                                          unsigned lds_base, const MixedArgsG &M,
+                                          gptr<const TaskDev> T, const ItemGeo &G, int wave,
                                           int lane, int &j, int &buf, int total,
                                           Frag<PREC> &F) {
   using Pr = Prec<PREC>;
@@ -502,7 +539,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
   using I0 = std::integral_constant<int, 0>;
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
-  const int nch0 = chunks_of<PREC>(M, P);
+  const int nch0 = chunks_of<PREC, SP>(M, T, P).n;
   // load sources of this pass; those of the next passes are formed only in
   // the branches that need them (the pass's last chunks), which keeps the
   // chunk loop's live registers to the accumulators, fragments and s0
@@ -517,7 +554,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
         wait_vm<0>();
       wait_lgkm0();
       block_barrier();
-      if (j + 3 < total) load_ahead<PREC, P, 3>(s0, M, G, wave, lane, c + 3, lds_base, buf);
+      if (j + 3 < total) load_ahead<PREC, SP, P, 3>(s0, M, T, G, wave, lane, c + 3, lds_base, buf);
       const int nbuf = buf == 2 ? 0 : buf + 1;
       const f4 *cbn = lds + nbuf * buf_quads<PREC>();
       // chunk j+1: this pass or the first chunk of pass P+1
@@ -539,34 +576,46 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       const f4 *cb = lds + buf * buf_quads<PREC>();
       read_x<PREC, 0, 4>(f, cb, o.x);
       read_y<PREC>(f, cb, o.y);
-      // the loads of chunk j+2 (buffer (j + 2) % 3, free since the last
-      // barrier) go between the two halves of the MFMAs, so their address
-      // arithmetic issues while the matrix core works (measured 0.7% faster
-      // than before the MFMAs, 3% faster than after them)
-      mma(I0(), I2(), f);
-      if (more) load_ahead<PREC, P, 2>(s0, M, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
-      mma(I2(), I4(), f);
-      if (more)
-        wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
-      else
-        wait_vm<0>();
-      block_barrier();
-      buf = buf == 2 ? 0 : buf + 1;
+      if constexpr (Pr::NBUF == 3) {
+        // the loads of chunk j+2 (buffer (j + 2) % 3, free since the last
+        // barrier) go between the two halves of the MFMAs, so their address
+        // arithmetic issues while the matrix core works (measured 0.7% faster
+        // than before the MFMAs, 3% faster than after them)
+        mma(I0(), I2(), f);
+        if (more) load_ahead<PREC, SP, P, 2>(s0, M, T, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
+        mma(I2(), I4(), f);
+        if (more)
+          wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
+        else
+          wait_vm<0>();
+        block_barrier();
+        buf = buf == 2 ? 0 : buf + 1;
+      } else {
+        // two buffers: chunk j+2 goes into this chunk's buffer once every
+        // wave has read it (after the barrier), with the whole of chunk
+        // j+1 to land
+        mma(I0(), I2(), f);
+        mma(I2(), I4(), f);
+        wait_vm<0>();  // chunk j+1 landed (no later chunk is in flight)
+        block_barrier();
+        if (more) load_ahead<PREC, SP, P, 2>(s0, M, T, G, wave, lane, c + 2, lds_base, buf);
+        buf ^= 1;
+      }
     }
   }
   if constexpr (PREC != TCE_T_FP64) from_quads<PREC, H>(acc, q);
   if constexpr (pi.minus) negate(acc);
 }
 
-template <int PREC, int P>
+template <int PREC, bool SP, int P>
 __device__ __forceinline__ void all_passes(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
-                                           unsigned lds_base,
-                                           const MixedArgsG &M, const ItemGeo &G, int wave,
+                                           unsigned lds_base, const MixedArgsG &M,
+                                           gptr<const TaskDev> T, const ItemGeo &G, int wave,
                                            int lane, int &j, int &buf, int total,
                                            Frag<PREC> &F) {
-  pass_step<PREC, P>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
+  pass_step<PREC, SP, P>(acc, lds, lds_base, M, T, G, wave, lane, j, buf, total, F);
   if constexpr (P + 1 < 18)
-    all_passes<PREC, P + 1>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
+    all_passes<PREC, SP, P + 1>(acc, lds, lds_base, M, T, G, wave, lane, j, buf, total, F);
 }
 
 // Stage the item's singles operands into LDS (FP64, as fused_wg.hip).
@@ -721,7 +770,7 @@ __device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double 
 
 // The item loop of both kernels below: TILES = false writes each item's
 // energy partials, TILES = true the task's t3 tiles (store_tiles).
-template <int PREC, bool TILES>
+template <int PREC, bool TILES, bool SP>
 __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *segs, int nseg,
                                                            long nglobal) {
   __shared__ f4 lds[lds_quads<PREC>()];
@@ -740,8 +789,11 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     const long item = sg.item0 + (g - sg.g0);
     const MixedArgsG M = load_args(sg.args);
     const FusedArgsG &A = M.f;
-    const int total = 9 * (M.Ek / Pr::KC) + 9 * (M.Mk / Pr::KC);  // chunks per item
-    const TaskDev T = *(const TaskDev *)(A.tasks + find_task(A.tasks, A.ntasks, item));
+    const gptr<const TaskDev> Tp = A.tasks + find_task(A.tasks, A.ntasks, item);
+    const TaskDev T = *(const TaskDev *)Tp;
+    int total = 0;  // chunks of the item (the passes' contraction ranges)
+#pragma unroll
+    for (int p = 0; p < 18; ++p) total += chunks_of<PREC, SP>(M, Tp, p).n;
     long box = item - T.item0;
     const int bc = box % T.nb6; box /= T.nb6;
     const int bb = box % T.nb5; box /= T.nb5;
@@ -769,7 +821,7 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
       const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
 #pragma unroll
       for (int k = 0; k < PRO; ++k)
-        if (k < total) load_ahead<PREC, 0, PRO>(s0, M, G, wave, lane, k, lds_base, k);
+        if (k < total) load_ahead<PREC, SP, 0, PRO>(s0, M, Tp, G, wave, lane, k, lds_base, k);
       // chunk 0 has landed once at most the loads of the later ones are
       // pending (fewer chunks in the item: wait for all)
       if (total >= PRO)
@@ -784,7 +836,7 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
       }
     }
     int j = 0, buf = 0;
-    all_passes<PREC, 0>(acc, lds, lds_base, M, G, wave, lane, j, buf, total, F);
+    all_passes<PREC, SP, 0>(acc, lds, lds_base, M, Tp, G, wave, lane, j, buf, total, F);
 
     // Epilogue: the chunk buffers are free (the last chunk ended with a
     // barrier and no load is outstanding).
@@ -813,38 +865,41 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 
 // The contraction extents of every job in the table must be multiples of
 // the precision's chunk (16 FP32 / 32 bf16 values); the job layer pads them.
-int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
-                       hipStream_t s) {
-  if (precision == TCE_T_FP64) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP64, false>), dim3((unsigned)grid), dim3(512), 0, s, segs,
-                       nseg, total);
-  } else if (precision == TCE_T_FP32) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP32, false>), dim3((unsigned)grid), dim3(512), 0, s, segs,
-                       nseg, total);
-  } else if (precision == TCE_T_BF16X3) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_BF16X3, false>), dim3((unsigned)grid), dim3(512), 0, s,
-                       segs, nseg, total);
-  } else {
+// sparse: some job of the table has contraction ranges narrower than its
+// spaces (spin / irrep labels), which the SP = true kernels follow.
+template <int PREC, bool TILES, bool SP>
+static void launch_one(const MixedSegment *segs, int nseg, long total, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<PREC, TILES, SP>), dim3((unsigned)grid),
+                     dim3(512), 0, s, segs, nseg, total);
+}
+template <bool TILES>
+static int launch_prec(const MixedSegment *segs, int nseg, long total, int precision, int grid,
+                       bool sparse, hipStream_t s) {
+  if (precision == TCE_T_FP64)
+    sparse ? launch_one<TCE_T_FP64, TILES, true>(segs, nseg, total, grid, s)
+           : launch_one<TCE_T_FP64, TILES, false>(segs, nseg, total, grid, s);
+  else if (precision == TCE_T_FP32)
+    sparse ? launch_one<TCE_T_FP32, TILES, true>(segs, nseg, total, grid, s)
+           : launch_one<TCE_T_FP32, TILES, false>(segs, nseg, total, grid, s);
+  else if (precision == TCE_T_BF16X3)
+    sparse ? launch_one<TCE_T_BF16X3, TILES, true>(segs, nseg, total, grid, s)
+           : launch_one<TCE_T_BF16X3, TILES, false>(segs, nseg, total, grid, s);
+  else if (precision == TCE_T_BF16X6)
+    sparse ? launch_one<TCE_T_BF16X6, TILES, true>(segs, nseg, total, grid, s)
+           : launch_one<TCE_T_BF16X6, TILES, false>(segs, nseg, total, grid, s);
+  else
     return -1;
-  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
+                       hipStream_t s, bool sparse) {
+  return launch_prec<false>(segs, nseg, total, precision, grid, sparse, s);
+}
+
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
-                       hipStream_t s) {
-  if (precision == TCE_T_FP64) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP64, true>), dim3((unsigned)grid), dim3(512), 0, s, segs,
-                       1, total);
-  } else if (precision == TCE_T_FP32) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_FP32, true>), dim3((unsigned)grid), dim3(512), 0, s, segs,
-                       1, total);
-  } else if (precision == TCE_T_BF16X3) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<TCE_T_BF16X3, true>), dim3((unsigned)grid), dim3(512), 0, s,
-                       segs, 1, total);
-  } else {
-    return -1;
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+                       hipStream_t s, bool sparse) {
+  return launch_prec<true>(segs, 1, total, precision, grid, sparse, s);
 }
 
 }  // namespace tce_t

@@ -69,6 +69,12 @@ inline double task_factor(bool restricted, long p4b, long p5b, long p6b, long h1
 }
 
 // Task filter of ccsd_t_gpu.F:94-112.
+// Precision modes of the d1/d2 contractions (TCE_T_FP64 .. TCE_T_BF16X6)
+// and the number of operand parts of a reduced-precision copy.
+constexpr int NPREC = 4;
+inline bool valid_precision(int p) { return p >= TCE_T_FP64 && p <= TCE_T_BF16X6; }
+inline int mixed_parts(int p) { return p == TCE_T_FP32 ? 1 : (p == TCE_T_BF16X3 ? 2 : 3); }
+
 inline bool task_allowed(const Tiles &T, long p4b, long p5b, long p6b, long h1b, long h2b,
                          long h3b) {
   long sp = T.spn(p4b) + T.spn(p5b) + T.spn(p6b), sh = T.spn(h1b) + T.spn(h2b) + T.spn(h3b);

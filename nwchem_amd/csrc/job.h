@@ -46,6 +46,7 @@ struct Job {
   int grid = 0;
   int pbox = 4;  // particle box edge of a work item: 4 (fused_wave) or 8 (fused_wg)
   bool k16 = false;  // FP64 on the 16x16 MFMA kernel (fused_triples_mixed, FP64 policy)
+  bool sparse = false;  // some task's contraction ranges are narrower than the spaces
   int ppi = 1;   // energy partials per work item (waves per item)
   // reduced-precision path (fused_mixed.hip): 8-wide particle boxes always
   std::vector<TaskDev> tasks8;
@@ -59,11 +60,11 @@ struct Job {
   void *dtile = nullptr;  // tile export: task entry, then doubles and singles tiles
   size_t dtile_bytes = 0;
   struct Mixed {
-    void *x2[2] = {nullptr, nullptr}, *y2[2] = {nullptr, nullptr};  // T2e, Vd2
-    void *x1[2] = {nullptr, nullptr}, *y1[2] = {nullptr, nullptr};  // Wd1, T2m
+    void *x2[3] = {nullptr, nullptr, nullptr}, *y2[3] = {nullptr, nullptr, nullptr};  // T2e, Vd2
+    void *x1[3] = {nullptr, nullptr, nullptr}, *y1[3] = {nullptr, nullptr, nullptr};  // Wd1, T2m
     int Ek = 0, Mk = 0;
     bool ready = false;
-  } mx[3];  // indexed by precision (1 = FP32, 2 = BF16X3)
+  } mx[4];  // indexed by precision (1 = FP32, 2 = BF16X3, 3 = BF16X6)
 };
 
 // Records a HIP failure (tce_t_last_error) and returns false.

@@ -77,7 +77,8 @@ __global__ void __launch_bounds__(256) task_reduce(const long *__restrict__ boun
 
 // Rows of K valid FP64 values (row stride kin) -> rows of kout reduced-
 // precision values, zero-padded: FP32 (out0), or bf16 hi = bf16(x) (out0)
-// and lo = bf16(x - hi) (out1), both rounded to nearest even.
+// and lo = bf16(x - hi) (out1), or hi, mid = bf16(x - hi), lo = bf16(x -
+// hi - mid) (out0..2), all rounded to nearest even.
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
   unsigned u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
@@ -85,7 +86,7 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 }
 
 __global__ void convert_rows(const double *__restrict__ in, long rows, int K, int kin, int kout,
-                             int precision, void *out0, void *out1) {
+                             int precision, void *out0, void *out1, void *out2) {
   const long n = rows * kout;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const long r = i / kout;
@@ -93,11 +94,19 @@ __global__ void convert_rows(const double *__restrict__ in, long rows, int K, in
     const double x = k < K ? in[r * kin + k] : 0.0;
     if (precision == TCE_T_FP32) {
       ((float *)out0)[i] = (float)x;
-    } else {
+    } else if (precision == TCE_T_BF16X3) {
       const unsigned short h = bf16_rne((float)x);
       const double hv = (double)__uint_as_float((unsigned)h << 16);
       ((unsigned short *)out0)[i] = h;
       ((unsigned short *)out1)[i] = bf16_rne((float)(x - hv));
+    } else {  // BF16X6: hi, mid, lo; each remainder exact in FP64
+      const unsigned short h = bf16_rne((float)x);
+      const double r1 = x - (double)__uint_as_float((unsigned)h << 16);
+      const unsigned short m = bf16_rne((float)r1);
+      const double r2 = r1 - (double)__uint_as_float((unsigned)m << 16);
+      ((unsigned short *)out0)[i] = h;
+      ((unsigned short *)out1)[i] = m;
+      ((unsigned short *)out2)[i] = bf16_rne((float)r2);
     }
   }
 }
@@ -157,6 +166,44 @@ extern "C" int tce_t_init(int device) {
   return TCE_T_OK;
 }
 
+// Contraction ranges of the 18 passes of fused_mixed (TaskDev::x0/x1) for a
+// task with tiles (p4b, p5b, p6b, h1b, h2b, h3b): the hull of the orbitals
+// whose spin and irrep the pass's blocks allow, in the occupied (d1) or
+// virtual (d2) space.  The reference applies the same spin and irrep tests
+// to each h7b / p7b tile (ccsd_t_doubles_gpu.F:260-263, 924-927) and skips
+// the tiles that fail them; here the forbidden orbitals outside the hull are
+// skipped (those inside it hold zeros when classes interleave).
+struct ClassRanges {
+  // [lo, hi) per (space 0 = occupied / 1 = virtual, spin 0..3, irrep 0..7);
+  // hi = 0: no orbital of that class
+  long lo[2][4][8], hi[2][4][8];
+  explicit ClassRanges(const Tiles &T) {
+    for (int sp = 0; sp < 2; ++sp)
+      for (int a = 0; a < 4; ++a)
+        for (int g = 0; g < 8; ++g) lo[sp][a][g] = 0, hi[sp][a][g] = 0;
+    for (long b = 1; b <= T.noab + T.nvab; ++b) {
+      const int sp = b > T.noab ? 1 : 0;
+      const long spin = T.spn(b), sym = T.sy(b);
+      if (spin < 0 || spin > 3 || sym < 0 || sym > 7) continue;
+      const long o = T.ofs(b) - (sp ? T.no : 0), e = o + T.rng(b);
+      long &l = lo[sp][spin][sym], &h = hi[sp][spin][sym];
+      if (h == 0) l = o, h = e;
+      else l = std::min(l, o), h = std::max(h, e);
+    }
+  }
+  void set(const Tiles &T, const long tiles[6], TaskDev &d) const {
+    for (int p = 0; p < 18; ++p) {
+      const PassDef &q = MIXED_PASS[p];
+      const long L = tiles[q.L], r = tiles[3 + pair_r(q.h)], s = tiles[3 + pair_s(q.h)];
+      const long spin = T.spn(r) + T.spn(s) - T.spn(L), sym = T.sy(L) ^ T.sy(r) ^ T.sy(s);
+      const int sp = q.kind == 1 ? 0 : 1;
+      const bool ok = spin >= 0 && spin <= 3 && sym >= 0 && sym <= 7;
+      d.x0[p] = ok ? (unsigned short)lo[sp][spin][sym] : 0;
+      d.x1[p] = ok ? (unsigned short)hi[sp][spin][sym] : 0;
+    }
+  }
+};
+
 // Kernel task table in the order of the reference task loop
 // (ccsd_t_gpu.F:87-92): work items of pb-wide particle boxes and 4-wide hole
 // boxes.  With sub = true each reference task is replaced by the kernel
@@ -172,6 +219,7 @@ static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &
   tasks.clear();
   bound.clear();
   if (task_tiles) task_tiles->clear();
+  const ClassRanges cr(T);
   struct Piece {
     int o, r;
   };
@@ -193,6 +241,9 @@ static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &
             for (long h3b = h2b; h3b <= noab; ++h3b) {
               if (!task_allowed(T, p4b, p5b, p6b, h1b, h2b, h3b)) continue;
               bound.push_back(item);
+              const long tiles6[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
+              TaskDev ranges;
+              cr.set(T, tiles6, ranges);
               if (task_tiles) {
                 long tt[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
                 task_tiles->insert(task_tiles->end(), tt, tt + 6);
@@ -219,6 +270,8 @@ static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &
                 d.nb3 = (d.r3 + 3) / 4;
                 d.item0 = item;
                 d.factor = task_factor(T.restricted, p4b, p5b, p6b, h1b, h2b, h3b);
+                std::copy(ranges.x0, ranges.x0 + 18, d.x0);
+                std::copy(ranges.x1, ranges.x1 + 18, d.x1);
                 item += (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
                 tasks.push_back(d);
                 continue;
@@ -245,6 +298,8 @@ static long build_tasks(const Tiles &T, int pb, bool sub, std::vector<TaskDev> &
                           d.nb4 = d.nb5 = d.nb6 = d.nb1 = d.nb2 = d.nb3 = 1;
                           d.item0 = item++;
                           d.factor = rf * hf * sym_factor(pab && a == b, pbc && b == c);
+                          std::copy(ranges.x0, ranges.x0 + 18, d.x0);
+                          std::copy(ranges.x1, ranges.x1 + 18, d.x1);
                           tasks.push_back(d);
                         }
                   }
@@ -392,6 +447,13 @@ extern "C" int tce_t_job_create_tiling(const tce_t_tiles *tiles, int tiling, tce
     J->nitems8 = job_tasks(*J, 8, J->tasks8, J->bound8);
   J->Ep = round8(T.nv);
   J->Mp = round8(T.no);
+  for (const TaskDev &d : J->tasks8)
+    for (int p = 0; p < 18 && !J->sparse; ++p) {
+      const long full = MIXED_PASS[p].kind ? T.no : T.nv;
+      J->sparse = d.x0[p] > 0 || d.x1[p] < full;
+    }
+  if (const char *e = getenv("TCE_T_SPARSE"))  // 0: always the whole spaces
+    if (!strcmp(e, "0")) J->sparse = false;
   *out = reinterpret_cast<tce_t_job *>(J.release());
   return TCE_T_OK;
 }
@@ -401,7 +463,7 @@ extern "C" long tce_t_job_num_tasks(const tce_t_job *job) {
 }
 
 extern "C" int tce_t_job_work(const tce_t_job *job, int precision, double *flops, long *items) {
-  if (!job || precision < TCE_T_FP64 || precision > TCE_T_BF16X3) return TCE_T_ERR_ARG;
+  if (!job || !valid_precision(precision)) return TCE_T_ERR_ARG;
   const Job &J = *reinterpret_cast<const Job *>(job);
   const bool wide = precision != TCE_T_FP64 || J.pbox == 8;
   const std::vector<TaskDev> &tab = wide ? J.tasks8 : J.tasks;
@@ -741,7 +803,7 @@ static long mixed_bytes(const Job &J, int precision) {
   const long no = J.T.no, nv = J.T.nv;
   const long Ek = mixed_extent(nv, precision), Mk = mixed_extent(no, precision);
   const long elems = no * no * nv * Ek + no * nv * nv * Ek + no * no * nv * Mk + no * nv * nv * Mk;
-  return elems * 4;  // 4 bytes (FP32) or 2 x 2 bytes (bf16 hi, lo) per element
+  return elems * (precision == TCE_T_BF16X6 ? 6 : 4);  // FP32, bf16 hi + lo, or hi + mid + lo
 }
 
 // Convert the four contraction tensors once per precision (device to
@@ -774,7 +836,7 @@ int tce_t::prepare_mixed(Job &J, int precision) {
   const long no = J.T.no, nv = J.T.nv;
   m.Ek = mixed_extent(nv, precision);
   m.Mk = mixed_extent(no, precision);
-  const int nparts = precision == TCE_T_FP32 ? 1 : 2;
+  const int nparts = mixed_parts(precision);
   const size_t es = precision == TCE_T_FP32 ? 4 : 2;
   struct Conv {
     const double *src;
@@ -789,7 +851,8 @@ int tce_t::prepare_mixed(Job &J, int precision) {
     for (int p = 0; p < nparts; ++p)
       HIPCHK(hipMalloc(&t.dst[p], std::max<size_t>(16, (size_t)t.rows * t.kout * es)), "mixed tensors");
     hipLaunchKernelGGL(convert_rows, dim3(4096), dim3(256), 0, J.stream, t.src, t.rows, t.K, t.kin,
-                       t.kout, precision, t.dst[0], nparts > 1 ? t.dst[1] : nullptr);
+                       t.kout, precision, t.dst[0], nparts > 1 ? t.dst[1] : nullptr,
+                       nparts > 2 ? t.dst[2] : nullptr);
     HIPCHK(hipGetLastError(), "convert launch");
   }
   {
@@ -826,13 +889,13 @@ MixedArgs tce_t::job_margs(const Job &J, int precision) {
   M.f = job_args(J, true);
   if (precision == TCE_T_FP64) {  // the 16x16 kernel on the FP64 tensors
     M.x2[0] = J.dT2e, M.y2[0] = J.dVd2, M.x1[0] = J.dWd1, M.y1[0] = J.dT2m;
-    M.x2[1] = M.y2[1] = M.x1[1] = M.y1[1] = nullptr;
+    for (int p = 1; p < 3; ++p) M.x2[p] = M.y2[p] = M.x1[p] = M.y1[p] = nullptr;
     M.Ek = (int)J.Ep;
     M.Mk = (int)J.Mp;
     return M;
   }
   const Job::Mixed &m = J.mx[precision];
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < 3; ++p) {
     M.x2[p] = m.x2[p];
     M.y2[p] = m.y2[p];
     M.x1[p] = m.x1[p];
@@ -846,42 +909,43 @@ MixedArgs tce_t::job_margs(const Job &J, int precision) {
 double tce_t::item_cost(const Job &J, int precision) {
   if (precision == TCE_T_FP64) return (double)(J.Ep + J.Mp);
   const Job::Mixed &m = J.mx[precision];
-  return (double)(m.Ek + m.Mk) * (precision == TCE_T_FP32 ? 0.5 : 0.25);
+  return (double)(m.Ek + m.Mk) * (precision == TCE_T_FP32 ? 0.5 : (precision == TCE_T_BF16X3 ? 0.25 : 0.4));
 }
 
 extern "C" long tce_t_job_precision_bytes(const tce_t_job *job, int precision) {
   if (!job) return -1;
   const Job &J = *reinterpret_cast<const Job *>(job);
   if (precision == TCE_T_FP64) return 0;
-  if (precision != TCE_T_FP32 && precision != TCE_T_BF16X3) return -1;
+  if (!valid_precision(precision)) return -1;
   long extra = mixed_bytes(J, precision);
   if (J.pbox != 8) extra += (long)J.tasks8.size() * sizeof(TaskDev) + J.nitems8 * 8 * 16;
   return extra;
 }
 
 extern "C" int tce_t_job_operands(tce_t_job *job, int precision, int tensor, void *out0,
-                                  void *out1, long *rows, long *kout) {
+                                  void *out1, void *out2, long *rows, long *kout) {
   if (!job || tensor < 0 || tensor > 3) return TCE_T_ERR_ARG;
-  if (precision != TCE_T_FP32 && precision != TCE_T_BF16X3) return TCE_T_ERR_ARG;
+  if (precision == TCE_T_FP64 || !valid_precision(precision)) return TCE_T_ERR_ARG;
   Job &J = *reinterpret_cast<Job *>(job);
   const long no = J.T.no, nv = J.T.nv;
   const long nrows[4] = {no * no * nv, no * nv * nv, no * no * nv, no * nv * nv};
   const long k = tensor < 2 ? mixed_extent(nv, precision) : mixed_extent(no, precision);
   if (rows) *rows = nrows[tensor];
   if (kout) *kout = k;
-  if (!out0 && !out1) return TCE_T_OK;
+  if (!out0 && !out1 && !out2) return TCE_T_OK;
   if (!J.uploaded) {
     set_error("tce_t_job_operands: job not uploaded");
     return TCE_T_ERR_STATE;
   }
-  if (precision == TCE_T_FP32 && out1) return TCE_T_ERR_ARG;
+  if ((mixed_parts(precision) < 2 && out1) || (mixed_parts(precision) < 3 && out2))
+    return TCE_T_ERR_ARG;
   const int rc = prepare_mixed(J, precision);
   if (rc != TCE_T_OK) return rc;
   Job::Mixed &m = J.mx[precision];
   void *const *src = tensor == 0 ? m.x2 : tensor == 1 ? m.y2 : tensor == 2 ? m.x1 : m.y1;
   const size_t bytes = (size_t)nrows[tensor] * k * (precision == TCE_T_FP32 ? 4 : 2);
-  void *out[2] = {out0, out1};
-  for (int p = 0; p < 2; ++p)
+  void *out[3] = {out0, out1, out2};
+  for (int p = 0; p < 3; ++p)
     if (out[p] && bytes)
       HIPCHK(hipMemcpyAsync(out[p], src[p], bytes, hipMemcpyDeviceToHost, J.stream), "operands D2H");
   HIPCHK(hipStreamSynchronize(J.stream), "operands D2H");
@@ -895,7 +959,7 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     set_error("tce_t_job_run before tce_t_job_upload");
     return TCE_T_ERR_STATE;
   }
-  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+  if (!valid_precision(precision)) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
   }
@@ -929,7 +993,7 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     if (!sg) return TCE_T_ERR_HIP;
     HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
     lrc = launch_fused_mixed(sg, 1, J.nitems8, precision,
-                             (int)std::min<long>(std::max(1, cus), J.nitems8), J.stream);
+                             (int)std::min<long>(std::max(1, cus), J.nitems8), J.stream, J.sparse);
   } else {
     const long grid = std::min<long>(J.grid, J.nitems);
     const Segment *sg = stage_table(J, A, J.nitems);
@@ -978,7 +1042,7 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
     set_error("tce_t_job_run_restart before tce_t_job_upload");
     return TCE_T_ERR_STATE;
   }
-  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+  if (!valid_precision(precision)) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
   }
@@ -1022,7 +1086,8 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
         const MixedSegment *sg = stage_table(J, job_margs(J, precision), i1, i0);
         if (!sg) return TCE_T_ERR_HIP;
         lrc = launch_fused_mixed(sg, 1, i1 - i0, precision,
-                                 (int)std::min<long>(std::max(1, cus), i1 - i0), J.stream);
+                                 (int)std::min<long>(std::max(1, cus), i1 - i0), J.stream,
+                                 J.sparse);
       } else {
         const Segment *sg = stage_table(J, A, i1, i0);
         if (!sg) return TCE_T_ERR_HIP;
@@ -1067,16 +1132,19 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
 // kernel on the resident tensors (store_tiles epilogue) and the tiles come
 // back to the host, where they are added to *singles / *doubles (either may
 // be NULL).  Synchronous on the job's stream.
-extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b,
-                                       long h3b, long p4b, long p5b, long p6b, double *singles,
-                                       double *doubles) {
-  if (!job) return TCE_T_ERR_ARG;
-  Job &J = *reinterpret_cast<Job *>(job);
+// The task's t3s / t3d tiles, or the slice p4 in [i0, i0 + ni), p5 in
+// [j0, j0 + nj) of them (0-based within the tiles), added into the caller's
+// arrays; ld5 = the p5 extent of the caller's layout (>= nj): element
+// (p4', p5', p6, h1, h2, h3) of the slice at
+// ((((p4' ld5 + p5') r6 + p6) r1 + h1) r2 + h2) r3 + h3.
+static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, long p4b, long p5b,
+                         long p6b, long i0, long ni, long j0, long nj, long ld5, double *singles,
+                         double *doubles) {
   if (!J.uploaded) {
     set_error("tce_t_job_triples_tiles before tce_t_job_upload");
     return TCE_T_ERR_STATE;
   }
-  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3) {
+  if (!valid_precision(precision)) {
     set_error("unsupported precision mode");
     return TCE_T_ERR_ARG;
   }
@@ -1092,15 +1160,20 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
     const int rc = prepare_mixed(J, precision);
     if (rc != TCE_T_OK) return rc;
   }
+  if (ni <= 0 || nj <= 0 || i0 < 0 || j0 < 0 || i0 + ni > T.rng(p4b) || j0 + nj > T.rng(p5b) ||
+      ld5 < nj) {
+    set_error("tile slice outside the task's tiles");
+    return TCE_T_ERR_ARG;
+  }
   TaskDev d;
-  d.op4 = (int)(T.ofs(p4b) - T.no);
-  d.op5 = (int)(T.ofs(p5b) - T.no);
+  d.op4 = (int)(T.ofs(p4b) - T.no + i0);
+  d.op5 = (int)(T.ofs(p5b) - T.no + j0);
   d.op6 = (int)(T.ofs(p6b) - T.no);
   d.oh1 = (int)T.ofs(h1b);
   d.oh2 = (int)T.ofs(h2b);
   d.oh3 = (int)T.ofs(h3b);
-  d.r4 = (int)T.rng(p4b);
-  d.r5 = (int)T.rng(p5b);
+  d.r4 = (int)ni;
+  d.r5 = (int)nj;
   d.r6 = (int)T.rng(p6b);
   d.r1 = (int)T.rng(h1b);
   d.r2 = (int)T.rng(h2b);
@@ -1113,6 +1186,10 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
   d.nb3 = (d.r3 + 3) / 4;
   d.item0 = 0;
   d.factor = 1.0;
+  {
+    const long tiles6[6] = {p4b, p5b, p6b, h1b, h2b, h3b};
+    ClassRanges(T).set(T, tiles6, d);
+  }
   const long n = (long)d.r1 * d.r2 * d.r3 * d.r4 * d.r5 * d.r6;
   const long items = (long)d.nb4 * d.nb5 * d.nb6 * d.nb1 * d.nb2 * d.nb3;
   constexpr size_t HEAD = 256;  // the task entry, then the tiles
@@ -1152,7 +1229,7 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   if (launch_fused_tiles(sg, items, precision, (int)std::min<long>(std::max(1, cus), items),
-                         J.stream) != 0) {
+                         J.stream, J.sparse) != 0) {
     set_error("tile kernel launch failed");
     return TCE_T_ERR_HIP;
   }
@@ -1191,13 +1268,16 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
     if (k + 1 < nch && !issue(k + 1)) rc = TCE_T_ERR_HIP;  // into the buffer of chunk k-1
     const long off = first + k * CH, len = std::min(CH, last - off);
     const double *src = stage + (k & 1) * CH;
+    // element g of the device tile (p5 extent nj) at g + p4'(ld5 - nj) inner
+    // of the caller's layout (p5 extent ld5)
+    const long inner = (long)d.r6 * d.r1 * d.r2 * d.r3, per4 = nj * inner, gap = (ld5 - nj) * inner;
     auto add = [&](long a, long b) {  // chunk positions [a, b)
       for (long i = a; i < b; ++i) {
         const long g = off + i;
         if (g < n)
-          doubles[g] += src[i];
+          doubles[g + (g / per4) * gap] += src[i];
         else
-          singles[g - n] += src[i];
+          singles[(g - n) + ((g - n) / per4) * gap] += src[i];
       }
     };
     std::vector<std::thread> th;
@@ -1212,6 +1292,38 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
     if (e) (void)hipEventDestroy(e);
   if (stage) (void)hipHostFree(stage);
   return rc;
+}
+
+extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, long h2b,
+                                       long h3b, long p4b, long p5b, long p6b, double *singles,
+                                       double *doubles) {
+  if (!job) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (p4b < 1 || p5b < 1 || p4b > J.T.noab + J.T.nvab || p5b > J.T.noab + J.T.nvab) {
+    set_error("tile sextuple is not a task of the job");
+    return TCE_T_ERR_ARG;
+  }
+  const long r4 = J.T.rng(p4b), r5 = J.T.rng(p5b);
+  return triples_tiles(J, precision, h1b, h2b, h3b, p4b, p5b, p6b, 0, r4, 0, r5, r5, singles,
+                       doubles);
+}
+
+extern "C" int tce_t_job_triples_tiles_6dts(tce_t_job *job, int precision, long h1b, long h2b,
+                                            long h3b, long p4b, long p5b, long p6b, long istart,
+                                            long istop, long jstart, long jstop, long maxp4,
+                                            long maxp5, double *singles, double *doubles) {
+  if (!job) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (istop - istart + 1 > maxp4 || jstop - jstart + 1 > maxp5) {
+    set_error("6dts slice larger than maxp4 x maxp5");
+    return TCE_T_ERR_ARG;
+  }
+  if (p4b < 1 || p5b < 1 || p4b > J.T.noab + J.T.nvab || p5b > J.T.noab + J.T.nvab) {
+    set_error("tile sextuple is not a task of the job");
+    return TCE_T_ERR_ARG;
+  }
+  return triples_tiles(J, precision, h1b, h2b, h3b, p4b, p5b, p6b, istart - 1,
+                       istop - istart + 1, jstart - 1, jstop - jstart + 1, maxp5, singles, doubles);
 }
 
 extern "C" int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks) {
@@ -1257,10 +1369,11 @@ void tce_t::job_release(Job &J) {
   J.dtile = nullptr;
   J.dtile_bytes = 0;
   for (auto &m : J.mx) {
-    for (void **b : {&m.x2[0], &m.x2[1], &m.y2[0], &m.y2[1], &m.x1[0], &m.x1[1], &m.y1[0], &m.y1[1]}) {
-      if (*b) (void)hipFree(*b);
-      *b = nullptr;
-    }
+    for (int p = 0; p < 3; ++p)
+      for (void **b : {&m.x2[p], &m.y2[p], &m.x1[p], &m.y1[p]}) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+      }
     m.ready = false;
   }
   J.uploaded = false;

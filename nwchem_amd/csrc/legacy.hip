@@ -17,6 +17,7 @@
 // host-side sum.  Errors are recorded (tce_t_last_error) and reported on
 // stderr rather than calling exit(1).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <cstdio>
 #include <cstdlib>
@@ -77,6 +78,8 @@ struct Legacy {
   std::mutex mu;
   Pool pool;
   hipStream_t stream = nullptr;
+  int backend = -1;                  // TCE_T_LEGACY_DIRECT / _TTGT (-1: from the environment)
+  rocblas_handle blas = nullptr;     // TTGT: DGEMM on the library stream
   double *t3_d = nullptr, *t3_s = nullptr;  // task tiles (doubles, singles)
   long dims_d[6] = {0}, dims_s[6] = {0};    // h1,h2,h3,p4,p5,p6 of each tile
   double *pinned = nullptr;
@@ -101,6 +104,26 @@ bool ok(hipError_t e, const char *what) {
 bool ensure_stream() {
   if (L().stream) return true;
   return ok(hipStreamCreateWithFlags(&L().stream, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+int backend_of(Legacy &S) {
+  if (S.backend < 0) {
+    const char *e = getenv("TCE_T_LEGACY");
+    S.backend = (e && !strcmp(e, "ttgt")) ? TCE_T_LEGACY_TTGT : TCE_T_LEGACY_DIRECT;
+  }
+  return S.backend;
+}
+
+bool ensure_blas(Legacy &S) {
+  if (S.blas) return true;
+  if (rocblas_create_handle(&S.blas) != rocblas_status_success ||
+      rocblas_set_stream(S.blas, S.stream) != rocblas_status_success) {
+    set_error("rocBLAS handle for the TTGT back end");
+    fprintf(stderr, "tce_t legacy: %s\n", g_err.c_str());
+    S.failed = true;
+    return false;
+  }
+  return true;
 }
 
 // Row-index codes of the reference kernels' loop variables.
@@ -155,14 +178,37 @@ __global__ void __launch_bounds__(256) contract_kernel(CallArgs c, double *__res
   t3[o] += c.sign * acc;
 }
 
+// TTGT back end (the alternative of sd_t_total_ttlg.cu: GEMM plus
+// transposes): the d1/d2 contraction as one DGEMM into C[m + M n] (rocBLAS,
+// FP64 matrix cores), m and n the output indices that sit on the t2 and the
+// v2 operand; this kernel adds C into the task tile in the variant's layout
+// (the transpose).  Strides of C by row code in c.sa; c.sb, c.ka, c.kb and
+// c.K unused.
+__global__ void __launch_bounds__(256) permute_add_kernel(CallArgs c, double *__restrict__ t3,
+                                                          const double *__restrict__ C) {
+  long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= c.total) return;
+  long r = o, oc = 0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    long v = r % c.dimL[q];
+    r /= c.dimL[q];
+    oc += v * c.sa[c.codeL[q]];
+  }
+  t3[o] += c.sign * C[oc];
+}
+
 // family: 0 = s1, 1 = d1, 2 = d2.  dims by row code; K contraction length.
+// host_t3 == nullptr: accumulate into the library's device tile of the task
+// (the GPU surface); otherwise into the caller's host triplesx (the CPU
+// surface), staged through the device for the call.
 void launch_call(int family, int n, const long dims[6], long K, const double *ha, long na,
-                 const double *hb, long nb, bool singles) {
+                 const double *hb, long nb, bool singles, double *host_t3 = nullptr) {
   Legacy &S = L();
   std::lock_guard<std::mutex> lk(S.mu);
   if (!ensure_stream()) return;
-  double *t3 = singles ? S.t3_s : S.t3_d;
-  if (!t3) {
+  double *t3 = host_t3 ? nullptr : (singles ? S.t3_s : S.t3_d);
+  if (!host_t3 && !t3) {
     set_error("sd_t kernel called before dev_mem_s_/dev_mem_d_");
     fprintf(stderr, "tce_t legacy: %s\n", g_err.c_str());
     S.failed = true;
@@ -180,7 +226,7 @@ void launch_call(int family, int n, const long dims[6], long K, const double *ha
   // The task tile holds exactly these elements (same extents, permuted).
   const long *td = singles ? S.dims_s : S.dims_d;
   long tsize = td[0] * td[1] * td[2] * td[3] * td[4] * td[5];
-  if (tsize != c.total) {
+  if (!host_t3 && tsize != c.total) {
     set_error("operand dimensions do not match the task tile");
     fprintf(stderr, "tce_t legacy: %s\n", g_err.c_str());
     S.failed = true;
@@ -238,13 +284,71 @@ void launch_call(int family, int n, const long dims[6], long K, const double *ha
     return;
   }
   if (!ok(hipMemcpyAsync(dops, S.pinned, bytes, hipMemcpyHostToDevice, S.stream), "H2D")) return;
+  const size_t tbytes = (size_t)c.total * sizeof(double);
+  if (host_t3) {
+    t3 = (double *)S.pool.get(tbytes);
+    if (!t3) {
+      set_error("device pool allocation failed");
+      S.failed = true;
+      S.pool.put(dops);
+      return;
+    }
+    // pageable copies: the caller's array is read and written in place
+    if (!ok(hipMemcpyAsync(t3, host_t3, tbytes, hipMemcpyHostToDevice, S.stream), "H2D t3")) return;
+  }
   int threads = 256;
   long blocks = (c.total + threads - 1) / threads;
-  hipLaunchKernelGGL(contract_kernel, dim3((unsigned)blocks), dim3(threads), 0, S.stream, c, t3,
-                     dops, dops + na);
-  if (!ok(hipGetLastError(), "contract_kernel launch")) return;
+  double *dC = nullptr;
+  if (family != 0 && backend_of(S) == TCE_T_LEGACY_TTGT && ensure_blas(S)) {
+    // m: the indices on t2sub, n: those on v2sub (C[m + M n])
+    long M, N;
+    CallArgs p = c;
+    for (int q = 0; q < 6; ++q) p.sa[q] = 0;
+    if (family == 2) {  // t2sub(p7,p4,h1,h2), v2sub(p7,h3,p6,p5)
+      M = p4d * h1d * h2d, N = h3d * p6d * p5d;
+      p.sa[P4] = 1, p.sa[H1] = p4d, p.sa[H2] = p4d * h1d;
+      p.sa[H3] = M, p.sa[P6] = M * h3d, p.sa[P5] = M * h3d * p6d;
+    } else {  // t2sub(h7,p4,p5,h1), v2sub(h3,h2,p6,h7)
+      M = p4d * p5d * h1d, N = h3d * h2d * p6d;
+      p.sa[P4] = 1, p.sa[P5] = p4d, p.sa[H1] = p4d * p5d;
+      p.sa[H3] = M, p.sa[H2] = M * h3d, p.sa[P6] = M * h3d * h2d;
+    }
+    dC = (double *)S.pool.get((size_t)M * N * sizeof(double));
+    if (!dC) {
+      set_error("device pool allocation failed");
+      S.failed = true;
+      S.pool.put(dops);
+      if (host_t3) S.pool.put(t3);
+      return;
+    }
+    const double one = 1.0, zero = 0.0;
+    const rocblas_status st =
+        family == 2
+            ? rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, (int)M,
+                            (int)N, (int)K, &one, dops, (int)K, dops + na, (int)K, &zero, dC, (int)M)
+            : rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_transpose,
+                            (int)M, (int)N, (int)K, &one, dops, (int)K, dops + na, (int)N, &zero,
+                            dC, (int)M);
+    if (st != rocblas_status_success) {
+      set_error("rocblas_dgemm failed");
+      fprintf(stderr, "tce_t legacy: %s\n", g_err.c_str());
+      S.failed = true;
+      return;
+    }
+    hipLaunchKernelGGL(permute_add_kernel, dim3((unsigned)blocks), dim3(threads), 0, S.stream, p,
+                       t3, dC);
+    if (!ok(hipGetLastError(), "permute_add_kernel launch")) return;
+  } else {
+    hipLaunchKernelGGL(contract_kernel, dim3((unsigned)blocks), dim3(threads), 0, S.stream, c, t3,
+                       dops, dops + na);
+    if (!ok(hipGetLastError(), "contract_kernel launch")) return;
+  }
+  if (host_t3 && !ok(hipMemcpyAsync(host_t3, t3, tbytes, hipMemcpyDeviceToHost, S.stream), "D2H t3"))
+    return;
   ok(hipStreamSynchronize(S.stream), "contract_kernel");
   S.pool.put(dops);
+  if (dC) S.pool.put(dC);
+  if (host_t3) S.pool.put(t3);
 }
 
 // Energy of one task tile, layout (h3 fastest, h2, h1, p6, p5, p4 slowest)
@@ -407,6 +511,37 @@ extern "C" const char *tce_t_last_error(void) { return g_err.c_str(); }
 ALL9(S1_ENTRY)
 ALL9(D1_ENTRY)
 ALL9(D2_ENTRY)
+
+// The CPU surface (ccsd_t_kernels_omp.F argument order: h3d, h2d, h1d, p6d,
+// p5d, p4d[, h7d / p7d]): the same contractions accumulated into the
+// caller's host triplesx (not zeroed), for the CPU drivers
+// (ccsd_t_singles_l.F, offl_ccsd_t_doubles_l.F) and their other callers.
+#define S1_CPU(N)                                                                          \
+  extern "C" void sd_t_s1_##N##_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d,    \
+                                 long *p4d, double *triplesx, double *t1sub, double *v2sub) { \
+    long dims[6] = {*p4d, *p5d, *p6d, *h1d, *h2d, *h3d};                                   \
+    launch_call(0, N, dims, 1, t1sub, (*p4d) * (*h1d), v2sub,                              \
+                (*h3d) * (*h2d) * (*p6d) * (*p5d), true, triplesx);                        \
+  }
+#define D1_CPU(N)                                                                          \
+  extern "C" void sd_t_d1_##N##_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d,    \
+                                 long *p4d, long *h7d, double *triplesx, double *t2sub,   \
+                                 double *v2sub) {                                          \
+    long dims[6] = {*p4d, *p5d, *p6d, *h1d, *h2d, *h3d};                                   \
+    launch_call(1, N, dims, *h7d, t2sub, (*h7d) * (*p4d) * (*p5d) * (*h1d), v2sub,         \
+                (*h3d) * (*h2d) * (*p6d) * (*h7d), false, triplesx);                       \
+  }
+#define D2_CPU(N)                                                                          \
+  extern "C" void sd_t_d2_##N##_(long *h3d, long *h2d, long *h1d, long *p6d, long *p5d,    \
+                                 long *p4d, long *p7d, double *triplesx, double *t2sub,   \
+                                 double *v2sub) {                                          \
+    long dims[6] = {*p4d, *p5d, *p6d, *h1d, *h2d, *h3d};                                   \
+    launch_call(2, N, dims, *p7d, t2sub, (*p7d) * (*p4d) * (*h1d) * (*h2d), v2sub,         \
+                (*p7d) * (*h3d) * (*p6d) * (*p5d), false, triplesx);                       \
+  }
+ALL9(S1_CPU)
+ALL9(D1_CPU)
+ALL9(D2_CPU)
 
 extern "C" void dev_mem_d_(long *h1d, long *h2d, long *h3d, long *p4d, long *p5d, long *p6d) {
   long d[6] = {*h1d, *h2d, *h3d, *p4d, *p5d, *p6d};
@@ -576,6 +711,15 @@ extern "C" int device_init_(long *icuda, long *cuda_device_number) {
     (void)hipSetDevice(local_rank());
   }
   return 1;
+}
+
+extern "C" int tce_t_legacy_backend(int backend) {
+  Legacy &S = L();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (backend == TCE_T_LEGACY_QUERY) return backend_of(S);
+  if (backend != TCE_T_LEGACY_DIRECT && backend != TCE_T_LEGACY_TTGT) return TCE_T_ERR_ARG;
+  S.backend = backend;
+  return backend;
 }
 
 // Sticky failure flag of the per-call surface (the reference exits instead).

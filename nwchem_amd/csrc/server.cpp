@@ -90,7 +90,7 @@ struct Server {
   void *tab_host = nullptr, *tab_dev = nullptr;  // launch tables (pinned / device)
   size_t tab_bytes = 0;
   int cus = 1;
-  double ms_per_cost[3] = {0, 0, 0};  // fitted launch time per item-cost unit
+  double ms_per_cost[NPREC] = {0, 0, 0, 0};  // fitted launch time per item-cost unit
   long slices = 0;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -163,6 +163,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
   // items per CU for each request)
   double total_cost = 0;
   long g = 0;
+  bool sparse = false;
   size_t ns = 0;  // segments: requests with items left (a system without
                   // allowed tasks has none and finishes without a launch)
   for (size_t k = 0; k < n; ++k) {
@@ -179,6 +180,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
     take = std::max(1L, std::min(take, left));
     MixedArgs M = job_margs(J, precision);
     M.f.partials = r.dpart;
+    sparse = sparse || J.sparse;
     std::memcpy(h + ns * asz, &M, sizeof M);
     MixedSegment sg{(const MixedArgs *)(d + ns * asz), r.cursor, r.cursor + take, g};
     std::memcpy(h + n * asz + ns * ssz, &sg, sizeof sg);
@@ -194,7 +196,8 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
   HIPCHK(hipEventRecord(S.ev0, S.stream), "event");
   const int grid = (int)std::min<long>(S.cus, g);
   const int rc =
-      launch_fused_mixed((const MixedSegment *)(d + n * asz), (int)ns, g, precision, grid, S.stream);
+      launch_fused_mixed((const MixedSegment *)(d + n * asz), (int)ns, g, precision, grid, S.stream,
+                         sparse);
   if (rc != 0) {
     set_error("slice launch failed");
     return TCE_T_ERR_HIP;
@@ -276,11 +279,11 @@ void worker_loop(Server &S) {
       continue;
     }
     // ---- one slice per precision present
-    std::vector<Request *> group[3];
+    std::vector<Request *> group[NPREC];
     for (long id : S.active) group[S.reqs[id].precision].push_back(&S.reqs[id]);
     lk.unlock();
     int rc = TCE_T_OK;
-    for (int p = 0; p < 3 && rc == TCE_T_OK; ++p) {
+    for (int p = 0; p < NPREC && rc == TCE_T_OK; ++p) {
       if (group[p].empty()) continue;
       const double ts = S.now();
       for (Request *r : group[p])
@@ -292,7 +295,7 @@ void worker_loop(Server &S) {
     // reads status, energies and times under the lock)
     std::vector<Request *> done;
     std::vector<int> frc;
-    for (int p = 0; p < 3; ++p)
+    for (int p = 0; p < NPREC; ++p)
       for (Request *r : group[p])
         if (rc != TCE_T_OK || r->cursor >= r->nitems) done.push_back(r);
     for (Request *r : done) {
@@ -366,8 +369,7 @@ extern "C" int tce_t_server_add_system(tce_t_server *server, const tce_t_tiles *
 
 extern "C" long tce_t_server_submit(tce_t_server *server, int system, int precision) {
   if (!server) return TCE_T_ERR_ARG;
-  if (precision != TCE_T_FP64 && precision != TCE_T_FP32 && precision != TCE_T_BF16X3)
-    return TCE_T_ERR_ARG;
+  if (!valid_precision(precision)) return TCE_T_ERR_ARG;
   Server &S = *reinterpret_cast<Server *>(server);
   std::lock_guard<std::mutex> lk(S.mu);
   if (system < 0 || system >= (int)S.systems.size() || S.stop) return TCE_T_ERR_ARG;

@@ -101,6 +101,18 @@ module tce_t_mi355x
       integer(c_long), value :: h1b, h2b, h3b, p4b, p5b, p6b
       real(c_double), intent(inout) :: singles(*), doubles(*)
     end function
+    ! the memory-sliced drivers of ccsd_t_6dts.F (ccsd_t_singles_l_6dts,
+    ! ccsd_t_doubles_l_6dts): one p4 x p5 slice of the task's tiles
+    integer(c_int) function tce_t_job_triples_tiles_6dts(job, precision, h1b, h2b, h3b, p4b, &
+        p5b, p6b, istart, istop, jstart, jstop, maxp4, maxp5, singles, doubles) &
+        bind(C, name="tce_t_job_triples_tiles_6dts")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      integer(c_int), value :: precision
+      integer(c_long), value :: h1b, h2b, h3b, p4b, p5b, p6b
+      integer(c_long), value :: istart, istop, jstart, jstop, maxp4, maxp5
+      real(c_double), intent(inout) :: singles(*), doubles(*)
+    end function
 
     integer(c_int) function tce_t_job_task_energies(job, out, ntasks) &
         bind(C, name="tce_t_job_task_energies")

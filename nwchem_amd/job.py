@@ -21,6 +21,7 @@ from .tiles import Blocks, TileModel
 FP64 = 0    # d1/d2 contractions on FP64 MFMA
 FP32 = 1    # FP32 operands, FP32 MFMA and accumulation
 BF16X3 = 2  # operands split into bf16 hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulation
+BF16X6 = 3  # operands split into bf16 hi + mid + lo, the 6 products to 2^-24, FP32 accumulation
 
 # Kernel-task tilings (tce_t_job_create_tiling)
 TILING_REFERENCE = 0  # the reference tiles (the reference's work)
@@ -119,6 +120,25 @@ class TriplesJob:
               "tce_t_job_triples_tiles")
         return s, d
 
+    def triples_tiles_6dts(self, h1b: int, h2b: int, h3b: int, p4b: int, p5b: int, p6b: int,
+                           istart: int, istop: int, jstart: int, jstop: int, maxp4: int,
+                           maxp5: int, precision: int = FP64) -> tuple[np.ndarray, np.ndarray]:
+        """The slice p4 in [istart, istop], p5 in [jstart, jstop] (1-based
+        within the tiles) of the task's (t3s, t3d), as ccsd_t_singles_l_6dts /
+        ccsd_t_doubles_l_6dts leave it: arrays of shape (maxp4, maxp5, r(p6b),
+        r(h1b), r(h2b), r(h3b)) (triplesx(h3,h2,h1,p6,maxp5,maxp4), Fortran
+        order), the slice in the leading corner."""
+        r = self.tiles.range
+        shape = (maxp4, maxp5) + tuple(int(r[b - 1]) for b in (p6b, h1b, h2b, h3b))
+        s = np.zeros(shape)
+        d = np.zeros(shape)
+        check(_lib.lib().tce_t_job_triples_tiles_6dts(self._h, precision, h1b, h2b, h3b, p4b, p5b,
+                                                      p6b, istart, istop, jstart, jstop, maxp4,
+                                                      maxp5, s.ctypes.data_as(DP),
+                                                      d.ctypes.data_as(DP)),
+              "tce_t_job_triples_tiles_6dts")
+        return s, d
+
     def work(self, precision: int = FP64) -> tuple[float, int]:
         """(flops, work items) of a run in `precision`: the reference's
         per-task flop count summed over the kernel tasks actually run (the
@@ -134,21 +154,23 @@ class TriplesJob:
     def operands(self, precision: int, tensor: int):
         """The reduced-precision copy of contraction tensor `tensor` (0 T2e,
         1 Vd2, 2 Wd1, 3 T2m) as the kernels read it: FP32 -> float32 array
-        (rows, kout); BF16X3 -> (hi, lo) uint16 bf16 patterns (rows, kout)."""
+        (rows, kout); BF16X3 -> (hi, lo), BF16X6 -> (hi, mid, lo) uint16 bf16
+        patterns (rows, kout)."""
         L = _lib.lib()
         rows, kout = C.c_long(0), C.c_long(0)
-        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, None, None, C.byref(rows),
+        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, None, None, None, C.byref(rows),
                                         C.byref(kout)), "tce_t_job_operands")
         shape = (rows.value, kout.value)
-        if precision == 1:
+        if precision == FP32:
             a = np.zeros(shape, np.float32)
             _lib.check(L.tce_t_job_operands(self._h, precision, tensor, a.ctypes.data, None, None,
-                                            None), "tce_t_job_operands")
+                                            None, None), "tce_t_job_operands")
             return a
-        hi, lo = np.zeros(shape, np.uint16), np.zeros(shape, np.uint16)
-        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, hi.ctypes.data, lo.ctypes.data,
-                                        None, None), "tce_t_job_operands")
-        return hi, lo
+        parts = [np.zeros(shape, np.uint16) for _ in range(2 if precision == BF16X3 else 3)]
+        ptrs = [p.ctypes.data for p in parts] + [None] * (3 - len(parts))
+        _lib.check(L.tce_t_job_operands(self._h, precision, tensor, *ptrs, None, None),
+                   "tce_t_job_operands")
+        return tuple(parts)
 
     @property
     def last_kernel_ms(self) -> float:
@@ -164,7 +186,8 @@ class TriplesJob:
     def kernel_for(precision: int, fp64_kernel: str) -> str:
         """Kernel that runs the contractions in `precision`."""
         return fp64_kernel if precision == FP64 else \
-            {FP32: "fused_triples_mixed_fp32", BF16X3: "fused_triples_mixed_bf16x3"}[precision]
+            {FP32: "fused_triples_mixed_fp32", BF16X3: "fused_triples_mixed_bf16x3",
+             BF16X6: "fused_triples_mixed_bf16x6"}[precision]
 
     def close(self) -> None:
         if self._h:

@@ -92,6 +92,8 @@ def lib() -> C.CDLL:
         L.ora_set_arith.restype = None
         L.ora_split_bf16.argtypes = [DP, C.c_long, C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort)]
         L.ora_split_bf16.restype = None
+        L.ora_split3_bf16.argtypes = [DP, C.c_long] + [C.POINTER(C.c_ushort)] * 3
+        L.ora_split3_bf16.restype = None
         L.ora_brute_force.argtypes = [C.c_long, C.c_long] + [DP] * 7 + [DP, DP]
         L.ora_task_tiles.argtypes = [C.POINTER(OraTiles), DP, LP, DP, LP, DP, LP, LP, DP, DP]
         L.ora_task_tiles.restype = C.c_long
@@ -224,7 +226,8 @@ def brute_force(no, nv, T1, T2, W, V, S, eo, ev):
     return e4.value, e5.value
 
 
-ARITH = {"fp64": 0, "fp32": 1, "bf16x3": 2, "abs": 3, "bf16x2_planted": 4, "bf16x3_trunc_planted": 5}
+ARITH = {"fp64": 0, "fp32": 1, "bf16x3": 2, "abs": 3, "bf16x2_planted": 4, "bf16x3_trunc_planted": 5,
+         "bf16x6": 6}
 
 
 class arith:
@@ -252,6 +255,14 @@ def split_bf16(x, truncate_lo: bool = False):
     lib().ora_split_bf16(_dp(x), x.size, int(truncate_lo), hi.ctypes.data_as(C.POINTER(C.c_ushort)),
                          lo.ctypes.data_as(C.POINTER(C.c_ushort)))
     return hi, lo
+
+
+def split3_bf16(x):
+    """The library's three-part split (hi, mid, lo as uint16 bf16 patterns)."""
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    parts = [np.zeros(x.size, np.uint16) for _ in range(3)]
+    lib().ora_split3_bf16(_dp(x), x.size, *[p.ctypes.data_as(C.POINTER(C.c_ushort)) for p in parts])
+    return tuple(parts)
 
 
 def set_mutation(family: int, n: int) -> None:

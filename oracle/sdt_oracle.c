@@ -323,15 +323,38 @@ void ora_split_bf16(const double *x, long n, int truncate_lo, unsigned short *hi
   }
 }
 
-/* Split operand: value = h + l (FP32: h = fl32(x), l = 0; ABS: h = |x|). */
-static void split_operand(const double *x, long n, double *h, double *l) {
+void ora_split3_bf16(const double *x, long n, unsigned short *hi, unsigned short *mid,
+                     unsigned short *lo) {
   for (long i = 0; i < n; i++) {
+    /* hi = bf16(fl32(x)), mid = bf16(fl32(x - hi)), lo = bf16(fl32(x - hi -
+     * mid)); both remainders exact in FP64 */
+    unsigned short h = bf16_rne((float)x[i]);
+    double r1 = x[i] - bf16_val(h);
+    unsigned short m = bf16_rne((float)r1);
+    double r2 = r1 - bf16_val(m);
+    hi[i] = h;
+    mid[i] = m;
+    lo[i] = bf16_rne((float)r2);
+  }
+}
+
+/* Split operand: value = h + m + l (FP32: h = fl32(x), m = l = 0; BF16X3:
+ * m = 0; ABS: h = |x|). */
+static void split_operand(const double *x, long n, double *h, double *m, double *l) {
+  for (long i = 0; i < n; i++) {
+    m[i] = 0.0;
     if (g_arith == ORA_ARITH_FP32) {
       h[i] = (double)(float)x[i];
       l[i] = 0.0;
     } else if (g_arith == ORA_ARITH_ABS) {
       h[i] = fabs(x[i]);
       l[i] = 0.0;
+    } else if (g_arith == ORA_ARITH_BF16X6) {
+      unsigned short hi, mi, lo;
+      ora_split3_bf16(x + i, 1, &hi, &mi, &lo);
+      h[i] = bf16_val(hi);
+      m[i] = bf16_val(mi);
+      l[i] = bf16_val(lo);
     } else {
       unsigned short hi, lo;
       ora_split_bf16(x + i, 1, g_arith == ORA_ARITH_BF16X3_TRUNC, &hi, &lo);
@@ -342,9 +365,14 @@ static void split_operand(const double *x, long n, double *h, double *l) {
 }
 
 /* The product a*b of the current mode, exact in FP64 (8- or 24-bit
- * significands): hi*hi' + hi*lo' + lo*hi' (the 2-term mistake drops lo*hi'). */
-static inline double emul_prod(double ha, double la, double hb, double lb) {
+ * significands, at most 6 terms spanning < 53 bits): hi*hi' + hi*lo' +
+ * lo*hi' (the 2-term mistake drops lo*hi'); BF16X6 also hi*mid' + mid*hi' +
+ * mid*mid' (mid = 0 in the other modes). */
+static inline double emul_prod(double ha, double ma, double la, double hb, double mb,
+                               double lb) {
   if (g_arith == ORA_ARITH_BF16X2) return ha * hb + ha * lb;
+  if (g_arith == ORA_ARITH_BF16X6)
+    return ha * hb + (ha * mb + ma * hb) + (ha * lb + ma * mb + la * hb);
   return ha * hb + ha * lb + la * hb;
 }
 
@@ -460,10 +488,10 @@ static void emul_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d, lon
   out_strides(D1_L[n - 1], dims, st);
   int sg = g_arith == ORA_ARITH_ABS ? 1 : mut_sign(1, n, D1_SIGN[n - 1]);
   long na = h7d * p4d * p5d * h1d, nb = h3d * h2d * p6d * h7d;
-  double *ha = malloc(sizeof(double) * (2 * na + 2 * nb)), *la = ha + na, *hb = la + na,
-         *lb = hb + nb;
-  split_operand(t2, na, ha, la);
-  split_operand(v2, nb, hb, lb);
+  double *ha = malloc(sizeof(double) * (3 * na + 3 * nb)), *ma = ha + na, *la = ma + na,
+         *hb = la + na, *mb = hb + nb, *lb = mb + nb;
+  split_operand(t2, na, ha, ma, la);
+  split_operand(v2, nb, hb, mb, lb);
 #pragma omp parallel for collapse(3) schedule(static)
   for (long p4 = 0; p4 < p4d; p4++)
     for (long p5 = 0; p5 < p5d; p5++)
@@ -477,7 +505,7 @@ static void emul_d1(int n, long h3d, long h2d, long h1d, long p6d, long p5d, lon
               double acc = t3[o];
               for (long h7 = 0; h7 < h7d; h7++) {
                 long ib = h3 + h3d * (h2 + h2d * (p6 + p6d * h7));
-                double pr = emul_prod(ha[ia + h7], la[ia + h7], hb[ib], lb[ib]);
+                double pr = emul_prod(ha[ia + h7], ma[ia + h7], la[ia + h7], hb[ib], mb[ib], lb[ib]);
                 acc = sg > 0 ? acc + pr : acc - pr;
               }
               t3[o] = acc;
@@ -491,10 +519,10 @@ static void emul_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d, lon
   out_strides(D2_L[n - 1], dims, st);
   int sg = g_arith == ORA_ARITH_ABS ? 1 : mut_sign(2, n, D2_SIGN[n - 1]);
   long na = p7d * p4d * h1d * h2d, nb = p7d * h3d * p6d * p5d;
-  double *ha = malloc(sizeof(double) * (2 * na + 2 * nb)), *la = ha + na, *hb = la + na,
-         *lb = hb + nb;
-  split_operand(t2, na, ha, la);
-  split_operand(v2, nb, hb, lb);
+  double *ha = malloc(sizeof(double) * (3 * na + 3 * nb)), *ma = ha + na, *la = ma + na,
+         *hb = la + na, *mb = hb + nb, *lb = mb + nb;
+  split_operand(t2, na, ha, ma, la);
+  split_operand(v2, nb, hb, mb, lb);
 #pragma omp parallel for collapse(3) schedule(static)
   for (long p4 = 0; p4 < p4d; p4++)
     for (long p5 = 0; p5 < p5d; p5++)
@@ -508,7 +536,8 @@ static void emul_d2(int n, long h3d, long h2d, long h1d, long p6d, long p5d, lon
               long ib = p7d * (h3 + h3d * (p6 + p6d * p5));
               double acc = t3[o];
               for (long p7 = 0; p7 < p7d; p7++) {
-                double pr = emul_prod(ha[ia + p7], la[ia + p7], hb[ib + p7], lb[ib + p7]);
+                double pr = emul_prod(ha[ia + p7], ma[ia + p7], la[ia + p7], hb[ib + p7], mb[ib + p7],
+                                      lb[ib + p7]);
                 acc = sg > 0 ? acc + pr : acc - pr;
               }
               t3[o] = acc;

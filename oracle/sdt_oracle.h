@@ -98,15 +98,21 @@ void ora_set_mutation(int family, int n);
  * arithmetic): FP64 (default); FP32 = operands rounded to FP32; BF16X3 =
  * operands split hi = bf16(fl32(x)), lo = bf16(fl32(x - hi)) (round to
  * nearest even) and products hi*hi' + hi*lo' + lo*hi'; all products exact,
- * accumulation FP64.  ABS = sum of |a||b| with every sign +.  BF16X2 (lo*hi'
- * dropped) and BF16X3_TRUNC (lo truncated) are planted mistakes. */
+ * accumulation FP64.  BF16X6 = hi, mid, lo and the six products of the
+ * library's 6-term mode.  ABS = sum of |a||b| with every sign +.  BF16X2
+ * (lo*hi' dropped) and BF16X3_TRUNC (lo truncated) are planted mistakes. */
 #define ORA_ARITH_FP64 0
 #define ORA_ARITH_FP32 1
 #define ORA_ARITH_BF16X3 2
 #define ORA_ARITH_ABS 3
 #define ORA_ARITH_BF16X2 4
 #define ORA_ARITH_BF16X3_TRUNC 5
+#define ORA_ARITH_BF16X6 6  /* x = hi + mid + lo, 6 products */
 void ora_set_arith(int mode);
+/* The library's three-part split: hi, mid = bf16(fl32(x - hi)), lo =
+ * bf16(fl32(x - hi - mid)), round to nearest even. */
+void ora_split3_bf16(const double *x, long n, unsigned short *hi, unsigned short *mid,
+                     unsigned short *lo);
 /* The library's operand split (bf16 hi, lo as 16-bit patterns);
  * truncate_lo = the planted mistake. */
 void ora_split_bf16(const double *x, long n, int truncate_lo, unsigned short *hi,

@@ -33,7 +33,7 @@ config2_tasks.json  (--config2) {E[T], E(T)} of eight tasks of the
               the input blocks and tables.
 
 precision.npz / precision.json  (--precision) for the systems of
-              PRECISION_SYSTEMS and the modes fp32 and bf16x3: the oracle's
+              PRECISION_SYSTEMS and the modes fp32, bf16x3 and bf16x6: the oracle's
               energies in the same operand representation as the GPU
               (oracle.arith, tests/precision_model.py: products exact, FP64
               accumulation), totals and per task, with the accumulation
@@ -159,7 +159,7 @@ def precision() -> int:
         ot, b, tm = build_system(kind, prm)
         meta[name] = {"inputs_sha256": digest(b.t1, b.t2, b.v2),
                       "tables_sha256": digest(b.t1h, b.t2h, b.v2h, tm.range, tm.evl)}
-        for mode in ("fp32", "bf16x3"):
+        for mode in ("fp32", "bf16x3", "bf16x6"):
             t0 = time.time()
             e1, e2, te = pm.emulated(tm, b, ot, mode)
             bt, btot = pm.accumulation_bounds(tm, b, ot, mode)

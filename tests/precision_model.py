@@ -4,18 +4,20 @@ The GPU's FP32 and split-BF16 modes differ from FP64 in two ways:
 
   representation  every d1/d2 operand is converted (FP32: fl32(x); BF16X3:
                   hi = bf16(fl32(x)), lo = bf16(fl32(x - hi)), and the
-                  product hi*hi' + hi*lo' + lo*hi').  This part is
-                  deterministic: the oracle restates it exactly
-                  (oracle.arith("fp32" / "bf16x3"), products exact in FP64,
-                  FP64 accumulation), and the conversion itself is checked
-                  bit for bit against oracle.split_bf16.
+                  product hi*hi' + hi*lo' + lo*hi'; BF16X6: hi, mid, lo and
+                  the six products hi*hi', hi*mid', mid*hi', hi*lo',
+                  mid*mid', lo*hi').  This part is deterministic: the oracle
+                  restates it exactly (oracle.arith("fp32" / "bf16x3" /
+                  "bf16x6"), products exact in FP64, FP64 accumulation), and
+                  the conversion itself is checked bit for bit against
+                  oracle.split_bf16 / split3_bf16.
   accumulation    the products are summed in FP32 on the matrix cores.
 
 Model of the accumulation error (probabilistic rounding-error analysis with
 mean-independent rounding errors, Higham & Mary, SIAM J. Sci. Comput. 41
 (2019) A2815): every element d_i of a t3 tile receives at most N_r rounding
 errors (one per product and term: N_r = terms * (9 no + 9 nv), terms = 1 for
-FP32, 3 for BF16X3), each of magnitude <= u |partial sum| <= u A_i, with
+FP32, 3 for BF16X3, 6 for BF16X6), each of magnitude <= u |partial sum| <= u A_i, with
 u = 2^-24 and A_i = sum of |products| of the element (oracle.arith("abs")).
 The energy is E[T] = sum f d^2/D and E(T) = sum f d (d+s)/D, so to first order
 dE = sum w_i dd_i with w_i = 2 f d_i / D_i (E[T]) or f (2 d_i + s_i)/D_i
@@ -38,8 +40,8 @@ import oracle
 
 U32 = 2.0 ** -24
 LAM = 6.0
-TERMS = {"fp32": 1, "bf16x3": 3}
-PRECISION_ID = {"fp32": 1, "bf16x3": 2}
+TERMS = {"fp32": 1, "bf16x3": 3, "bf16x6": 6}
+PRECISION_ID = {"fp32": 1, "bf16x3": 2, "bf16x6": 3}
 
 
 def task_factor(restricted: bool, t) -> float:

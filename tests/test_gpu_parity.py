@@ -181,11 +181,22 @@ def test_job_zero_singles_equal_energies():
     assert e[0] == e[1] and np.array_equal(te[:, 0], te[:, 1])
 
 
+@pytest.fixture(params=[0, 1], ids=["direct", "ttgt"])
+def legacy_backend(request):
+    """The per-call d1/d2 back end (tce_t_legacy_backend): the direct kernel
+    or TTGT (DGEMM + transpose); restored to direct afterwards."""
+    L = _lib.lib()
+    assert L.tce_t_legacy_backend(request.param) == request.param
+    yield request.param
+    L.tce_t_legacy_backend(0)
+
+
 @pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[5]],
                          ids=["cfg1", "ragged_small", "ragged_tiles"])
-def test_legacy_surface_matches_oracle(case):
+def test_legacy_surface_matches_oracle(case, legacy_backend):
     """ccsd_t_gpu.F's usedevice=1 call sequence (dev_mem_*, sd_t_*_cuda_,
-    compute_en_, dev_release_, finalizememmodule_) against libtce_t.so."""
+    compute_en_, dev_release_, finalizememmodule_) against libtce_t.so, on
+    both back ends of the d1/d2 calls."""
     no, nv, to, tv, seed, ro, rv = case
     tm, b, ot = system(no, nv, to, tv, seed, ro, rv)
     ref = oracle_energies(ot, b)
@@ -194,6 +205,29 @@ def test_legacy_surface_matches_oracle(case):
     got = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be)
     assert L.tce_t_legacy_status() == 0
     assert_parity(got[:2], got[2], ref)
+
+
+def test_cpu_order_kernels_equal_the_reference_kernels(legacy_backend):
+    """The CPU surface (sd_t_s1/d1/d2_N_ in ccsd_t_kernels_omp.F's argument
+    order, accumulating into a host triplesx) against the tiles the
+    reference's own compiled kernels leave on the golden inputs
+    (tests/golden/kernels.npz; exact binary fractions, so bit for bit, in
+    any summation order), on both back ends."""
+    import ctypes as C
+    L = _lib.lib()
+    DPt = C.POINTER(C.c_double)
+    with np.load(os.path.join(GOLD, "kernels.npz")) as z:
+        gold = {k: z[k] for k in z.files}
+    for si, shape in enumerate(mg.KERNEL_SHAPES):
+        dims = [C.c_long(v) for v in shape]
+        for fam in ("s1", "d1", "d2"):
+            for n in range(1, 10):
+                t3, a, b = mg.kernel_inputs(fam, shape, n)
+                args = dims[:6] if fam == "s1" else dims
+                getattr(L, f"sd_t_{fam}_{n}_")(*[C.byref(d) for d in args], t3.ctypes.data_as(DPt),
+                                                a.ctypes.data_as(DPt), b.ctypes.data_as(DPt))
+                assert np.array_equal(t3, gold[f"shape{si}_{fam}_{n}"]), (si, fam, n)
+    assert L.tce_t_legacy_status() == 0
 
 
 def test_fortran_caller_matches_oracle():
@@ -271,6 +305,25 @@ def test_spin_tilings_match_oracle(prm, kernel, monkeypatch):
         e = job.run()
         te = job.task_energies()
     assert_parity(e, te, ref)
+
+
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "bf16x3"])
+@pytest.mark.parametrize("prm", SPIN_GPU, ids=["spin_irreps", "spin_8_16", "restricted"])
+def test_forbidden_blocks_are_skipped_exactly(prm, precision, monkeypatch):
+    """Spin- and irrep-labelled tilings: the 16x16 kernel contracts each pass
+    only over the orbitals of the spin and irrep its blocks allow (the
+    reference skips the other h7b / p7b tiles).  The skipped chunks hold
+    zeros, so the energies equal the run over the whole spaces bit for bit."""
+    s = spin_systems.SpinSystem(**prm)
+    monkeypatch.setenv("TCE_T_KERNEL", "wg16")
+    out = {}
+    for sparse in ("0", "1"):
+        monkeypatch.setenv("TCE_T_SPARSE", sparse)
+        with TriplesJob(s.tiles) as job:
+            job.upload(s.blocks)
+            out[sparse] = (job.run(precision), job.task_energies())
+    assert out["0"][0] == out["1"][0]
+    assert np.array_equal(out["0"][1], out["1"][1])
 
 
 def test_legacy_surface_spin_tiling():
@@ -598,6 +651,32 @@ def test_triples_tiles_of_orbital_jobs():
         _check_tiles(job, tm, b, ot, 0, tm.tasks()[::3])
 
 
+@pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "bf16x3"])
+def test_triples_tiles_6dts_slices_equal_the_whole_tiles(precision):
+    """The memory-sliced export (ccsd_t_6dts.F's xmem path): every p4 x p5
+    slice, ragged last slices included, equals the same part of the whole
+    tile bit for bit, in the slice layout triplesx(h3,h2,h1,p6,maxp5,maxp4)
+    with the unused part of the leading corner left untouched (zero)."""
+    tm, b, ot = _tile_system("ragged")
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        for task in (tm.tasks()[0], tm.tasks()[len(tm.tasks()) // 2], tm.tasks()[-1]):
+            p4, p5, p6, h1, h2, h3 = task
+            gs, gd = job.triples_tiles(h1, h2, h3, p4, p5, p6, precision)
+            r4, r5 = gs.shape[0], gs.shape[1]
+            m4, m5 = max(1, (r4 + 2) // 3), max(1, (r5 + 1) // 2)  # uneven slices
+            for i0 in range(0, r4, m4):
+                for j0 in range(0, r5, m5):
+                    i1, j1 = min(r4, i0 + m4), min(r5, j0 + m5)
+                    ss, sd = job.triples_tiles_6dts(h1, h2, h3, p4, p5, p6, i0 + 1, i1, j0 + 1, j1,
+                                                    m4, m5, precision)
+                    assert np.array_equal(sd[:i1 - i0, :j1 - j0], gd[i0:i1, j0:j1]), (task, i0, j0)
+                    assert np.array_equal(ss[:i1 - i0, :j1 - j0], gs[i0:i1, j0:j1]), (task, i0, j0)
+                    assert not sd[i1 - i0:].any() and not sd[:, j1 - j0:].any()
+        with pytest.raises(RuntimeError):  # a slice larger than maxp4
+            job.triples_tiles_6dts(h1, h2, h3, p4, p5, p6, 1, 3, 1, 1, 2, 1)
+
+
 def test_triples_tiles_accumulate_and_reject_non_tasks():
     """The C entry point adds into the caller's buffers (as the drivers do);
     a sextuple that is not a task, or a job without upload, is an error."""
@@ -849,6 +928,32 @@ def test_config2_reduced_precision_tasks(mode):
         d = np.abs(te[t["index"]] - em)
         print(f"task {t['index']} {mode}: |E - E_emul| = {d} (bound {bnd})", flush=True)
         assert np.all(d <= bnd), (t, te[t["index"]], em, bnd)
+
+
+def test_config2_task_through_the_per_call_surface(legacy_backend):
+    """The first task of the benchmark job through the reference's per-call
+    GPU surface in ccsd_t_gpu.F order (the drop-in of sd_t_total.cu: host
+    block fetch and sort per call, operands copied per call, t3 accumulated
+    in device tiles, compute_en_), against the reference kernels' energy of
+    that task; the time per task is printed (what the binary drop-in costs
+    next to the whole-job path)."""
+    import time
+    tm, b, meta = config2_inputs()
+    ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
+    L = _lib.lib()
+    be = oracle.device_backend(L)
+    t = meta["tasks"][0]
+    k = t["index"]
+    t0 = time.perf_counter()
+    got = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be, task_begin=k,
+                        task_end=k + 1)
+    dt = time.perf_counter() - t0
+    assert L.tce_t_legacy_status() == 0
+    ref = np.array([float.fromhex(t["E_bt"]), float.fromhex(t["E_pt"])])
+    d = np.abs(got[2][0] - ref)
+    print(f"per-call surface ({['direct', 'ttgt'][legacy_backend]}), config-2 task {k} {t['tiles']}: "
+          f"{dt:.2f} s, |dE| = {d}", flush=True)
+    assert np.all(d <= summation_tol(t["elements"], *ref))
 
 
 def test_config2_orbital_tiling_agrees():

@@ -1,4 +1,4 @@
-"""GPU parity of the reduced-precision modes (FP32 and 3-term split-BF16).
+"""GPU parity of the reduced-precision modes (FP32, 3-term and 6-term split-BF16).
 
 The check is split along the two error sources of tests/precision_model.py:
 
@@ -34,7 +34,7 @@ LP = C.POINTER(C.c_long)
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["fp32", "bf16x3"]
+MODES = ["fp32", "bf16x3", "bf16x6"]
 
 
 def fixture(name, mode):
@@ -91,12 +91,12 @@ def test_operands_are_the_oracle_conversion(name, mode):
                 assert np.array_equal(got[:, :K].view(np.uint32), x.astype(np.float32).view(np.uint32))
                 assert not got[:, K:].any()
             else:
-                hi, lo = got
-                assert hi.shape[0] == rows and hi.shape[1] >= K and hi.shape[1] % 32 == 0
-                rh, rl = oracle.split_bf16(x)
-                assert np.array_equal(hi[:, :K].ravel(), rh), (name, tid)
-                assert np.array_equal(lo[:, :K].ravel(), rl), (name, tid)
-                assert not hi[:, K:].any() and not lo[:, K:].any()
+                ref = oracle.split_bf16(x) if mode == "bf16x3" else oracle.split3_bf16(x)
+                assert len(got) == len(ref)
+                for part, r in zip(got, ref):
+                    assert part.shape[0] == rows and part.shape[1] >= K and part.shape[1] % 32 == 0
+                    assert np.array_equal(part[:, :K].ravel(), r), (name, tid)
+                    assert not part[:, K:].any()
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -156,7 +156,8 @@ def test_library_exports_every_header_symbol():
     exported = set(_lib.exported_symbols())
     missing = sorted(declared - exported)
     assert not missing, missing
-    assert len([s for s in declared if s.startswith("sd_t_")]) == 27
+    # the GPU surface (sd_t_*_cuda_) and the CPU surface (sd_t_*_N_), 27 each
+    assert len([s for s in declared if s.startswith("sd_t_")]) == 54
 
 
 def test_task_list_matches_oracle():

@@ -60,6 +60,20 @@ def test_split_is_round_to_nearest_even_bf16():
     assert np.all(np.abs(rep[nz] - x[nz]) <= 2.0 ** -17 * np.abs(x[nz]))
 
 
+def test_three_part_split_is_exact_to_2_to_the_24():
+    """hi, mid, lo: each the round-to-nearest-even bf16 of the remainder
+    left by the parts before it (in FP32); together x to 2^-24 relative."""
+    x = np.random.default_rng(5).standard_normal(200000) * 10.0 ** np.random.default_rng(6).integers(-5, 5, 200000)
+    hi, mid, lo = oracle.split3_bf16(x)
+    assert np.array_equal(hi, rne_bf16_by_definition(x.astype(np.float32)))
+    r1 = x - bf16_to_f64(hi)
+    assert np.array_equal(mid, rne_bf16_by_definition(r1.astype(np.float32)))
+    r2 = r1 - bf16_to_f64(mid)
+    assert np.array_equal(lo, rne_bf16_by_definition(r2.astype(np.float32)))
+    rep = bf16_to_f64(hi) + bf16_to_f64(mid) + bf16_to_f64(lo)
+    assert np.all(np.abs(rep - x) <= 2.0 ** -24 * np.abs(x))
+
+
 def test_planted_truncation_changes_the_split():
     x = np.random.default_rng(4).standard_normal(100000)
     _, lo = oracle.split_bf16(x)
@@ -72,7 +86,7 @@ def _system(name):
     return mg.build_system(kind, prm)
 
 
-@pytest.mark.parametrize("mode,dtype", [("fp32", np.float32), ("bf16x3", "bf16")])
+@pytest.mark.parametrize("mode,dtype", [("fp32", np.float32), ("bf16x3", "bf16"), ("bf16x6", "bf16")])
 def test_emulation_is_fp64_on_representable_inputs(mode, dtype):
     ot, b, tm = _system("no5_nv9_to2_tv4")
     if dtype == "bf16":
@@ -103,7 +117,7 @@ def test_abs_mode_bounds_every_element():
 def test_precision_references_are_current(name):
     ot, b, tm = _system(name)
     with np.load(os.path.join(GOLD, "precision.npz")) as z:
-        for mode in ("fp32", "bf16x3"):
+        for mode in ("fp32", "bf16x3", "bf16x6"):
             e1, e2, te = pm.emulated(tm, b, ot, mode)
             assert np.array_equal(z[f"{name}_{mode}_total"], [e1, e2])
             assert np.array_equal(z[f"{name}_{mode}_tasks"], te)
