@@ -62,6 +62,7 @@ struct Plan {
 struct Batch {
   std::vector<Job *> jobs;
   hipStream_t stream = nullptr;
+  unsigned long long *sched = nullptr;  // dynamic-schedule counters of the stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double *hE = nullptr;            // pinned: task energies of all jobs
   std::vector<long> eoff;          // per job: first task slot in hE
@@ -153,7 +154,7 @@ int enqueue(Batch &B, int precision, const Plan &P) {
       bool sparse = false;
       for (const Job *j : B.jobs) sparse = sparse || j->sparse;
       rc = launch_fused_mixed((const MixedSegment *)segs, P.nseg, P.total, precision, P.grid,
-                              B.stream, sparse);
+                              B.stream, sparse, sched_use(B.sched));
     }
     else if (P.choice.family == FAM_WG)
       rc = launch_fused_wg((const Segment *)segs, P.nseg, P.total, P.grid, B.stream);
@@ -276,6 +277,11 @@ extern "C" int tce_t_batch_create(tce_t_job *const *jobs, int njobs, tce_t_batch
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(e, "batch stream");
+  if (!(B->sched = sched_alloc())) {
+    (void)hipStreamDestroy(B->stream);
+    delete B;
+    return TCE_T_ERR_HIP;
+  }
   if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return fail(e, "event");
   if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return fail(e, "event");
   if ((e = hipHostMalloc(&B->hE, std::max<long>(1, 2 * B->ntasks) * sizeof(double))) != hipSuccess)
@@ -353,6 +359,7 @@ extern "C" int tce_t_batch_destroy(tce_t_batch *batch) {
     if (P.tab) (void)hipFree(P.tab);
   }
   if (B->hE) (void)hipHostFree(B->hE);
+  if (B->sched) (void)hipFree(B->sched);
   if (B->ev0) (void)hipEventDestroy(B->ev0);
   if (B->ev1) (void)hipEventDestroy(B->ev1);
   if (B->stream) (void)hipStreamDestroy(B->stream);

@@ -202,13 +202,59 @@ __device__ __forceinline__ long round_item(long r, unsigned b, unsigned G, bool 
   return r * (long)G + ((mirror && (r & 1)) ? (long)G - 1 - f : f);
 }
 
+// Dynamic schedule of a persistent launch (fused_triples_mixed): the global
+// items are cut into windows of 8 x SCHED_SLICE, and queue q (one per XCD)
+// holds slice q of every window, in window order.  A workgroup draws tickets
+// from the queue of its own XCD (b % 8) with a device-scope atomic add, so
+// the 32 CUs of an XCD share one slice of consecutive items at a time (their
+// operand rows meet in that XCD's L2) while each CU takes the next item when
+// it is free: no CU drifts ahead of the others over a long launch, and
+// items of uneven cost balance themselves.  A workgroup whose queue is
+// exhausted takes tickets from the next queues in turn, and leaves when all
+// eight are.  Counters: sched[0..7] the queues' next tickets, sched[8] the
+// workgroups that have left; they are zero before a launch, and the last
+// workgroup to leave zeroes them for the next launch on the same stream.
+constexpr long SCHED_SLICE = 32;
+constexpr int SCHED_WORDS = 9;
+
+__device__ __forceinline__ long queue_item(unsigned long long k, unsigned q) {
+  return (long)(k / SCHED_SLICE) * (8 * SCHED_SLICE) + (long)q * SCHED_SLICE +
+         (long)(k % SCHED_SLICE);
+}
+
+// Item of ticket k of queue q, or of a ticket of the next queues when q is
+// exhausted (q then moves on); -1 when every queue is.  One thread.
+__device__ __forceinline__ long sched_item(unsigned long long *sched, unsigned &q,
+                                           unsigned long long k, long n) {
+  for (int tried = 1;; ++tried) {
+    const long g = queue_item(k, q);
+    if (g < n) return g;
+    if (tried == 8) return -1;
+    q = (q + 1) & 7u;
+    k = atomicAdd(&sched[q], 1ull);
+  }
+}
+
+// A workgroup leaves the dynamic schedule (one thread, after its last
+// ticket): the last one to leave zeroes the counters.
+__device__ __forceinline__ void sched_leave(unsigned long long *sched) {
+  __threadfence();
+  if (atomicAdd(&sched[8], 1ull) == (unsigned long long)gridDim.x - 1) {
+    for (int i = 0; i < SCHED_WORDS; ++i) atomicExch(&sched[i], 0ull);
+    __threadfence();
+  }
+}
+
 // Segment of global index g for a workgroup whose previous item lay in
-// segment s (uniform).  A workgroup's global indices increase from round to
-// round, so its segment only moves forward: one read of the next segment's
-// start per item replaces a binary search of the table.  Empty segments are
-// stepped over.
+// segment s (uniform).  Under the static schedule, and within one queue of
+// the dynamic one, a workgroup's global indices increase from item to item,
+// so its segment only moves forward: one read of the next segment's start
+// per item replaces a binary search of the table.  An item taken from
+// another queue can lie before the previous one: the scan restarts.  Empty
+// segments are stepped over.
 template <class Args>
 __device__ __forceinline__ int next_segment(const SegmentT<Args> *segs, int nseg, int s, long g) {
+  if (g < segs[s].g0) s = 0;
   while (s + 1 < nseg && segs[s + 1].g0 <= g) ++s;
   return s;
 }
@@ -342,11 +388,20 @@ int launch_fused_wave(const Segment *segs, int nseg, long total, int grid, hipSt
 int launch_fused_wg(const Segment *segs, int nseg, long total, int grid, hipStream_t s);
 // sparse: some job of the table narrows its passes' contraction ranges
 // (TaskDev::x0/x1, spin- or irrep-labelled tiles).
+// sched: the stream's counters of the dynamic schedule (sched_alloc), or
+// null for the static round-robin.
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
-                       hipStream_t s, bool sparse);
+                       hipStream_t s, bool sparse, unsigned long long *sched);
 // One task's t3 tiles (fused_mixed.hip): `total` items of the single task at
 // args->f.tasks[0], tiles written to args->f.partials (doubles, then singles).
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
-                       hipStream_t s, bool sparse);
+                       hipStream_t s, bool sparse, unsigned long long *sched);
+// Zeroed counters of the dynamic schedule for one launching stream (every
+// launch leaves them zero; launches that may overlap need their own), or
+// null on failure (error set).  Free with hipFree.
+unsigned long long *sched_alloc();
+// The counters to pass for a stream's counters `own`: null (the static
+// round-robin) when TCE_T_SCHED=static.
+unsigned long long *sched_use(unsigned long long *own);
 
 }  // namespace tce_t

@@ -772,18 +772,37 @@ __device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double 
 // energy partials, TILES = true the task's t3 tiles (store_tiles).
 template <int PREC, bool TILES, bool SP>
 __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *segs, int nseg,
-                                                           long nglobal) {
+                                                           long nglobal,
+                                                           unsigned long long *sched) {
   __shared__ f4 lds[lds_quads<PREC>()];
+  __shared__ long next_item;  // dynamic schedule: the workgroup's next item
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const unsigned lds_base = (unsigned)(unsigned long)(__attribute__((address_space(3))) f4 *)lds;
   const int kq = lane >> 4, hi = (lane & 15) >> 2, lo = lane & 3;
   using Pr = Prec<PREC>;
-  // Persistent loop, static round-robin over the launch's items (uniform
-  // across the workgroup, so every barrier is reached by all 8 waves).
+  // Persistent loop over the launch's items, uniform across the workgroup
+  // (every barrier is reached by all 8 waves): the dynamic schedule of
+  // sched_item when `sched` is given, else a static round-robin.
+  const bool dyn = sched != nullptr;
+  unsigned q = blockIdx.x & 7u;  // thread 0: the queue it draws from
+  unsigned long long ticket = 0;
+  long g = 0;
+  if (dyn) {
+    if (tid == 0) next_item = sched_item(sched, q, atomicAdd(&sched[q], 1ull), nglobal);
+    __syncthreads();
+    g = next_item;
+  }
   int seg = 0;  // segment of the workgroup's current item
-  for (long r = 0; r * (long)gridDim.x < nglobal; ++r) {
-    const long g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
-    if (g >= nglobal) continue;  // last round: uniform per workgroup
+  for (long r = 0;; ++r) {
+    if (dyn) {
+      if (g < 0) break;
+      // the next item's ticket is drawn now and read at the end of this item
+      if (tid == 0) ticket = atomicAdd(&sched[q], 1ull);
+    } else {
+      if (r * (long)gridDim.x >= nglobal) break;
+      g = round_item(r, blockIdx.x, gridDim.x, nseg > 1);
+      if (g >= nglobal) continue;  // last round: uniform per workgroup
+    }
     seg = next_segment(segs, nseg, seg, g);
     const MixedSegment &sg = segs[seg];
     const long item = sg.item0 + (g - sg.g0);
@@ -857,8 +876,11 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
         A.partials[2 * (item * 8 + wave) + 1] = e5;
       }
     }
+    if (dyn && tid == 0) next_item = sched_item(sched, q, ticket, nglobal);
     __syncthreads();  // the next item's first loads overwrite the staged data
+    if (dyn) g = next_item;
   }
+  if (dyn && tid == 0) sched_leave(sched);
 }
 
 }  // namespace
@@ -868,38 +890,40 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 // sparse: some job of the table has contraction ranges narrower than its
 // spaces (spin / irrep labels), which the SP = true kernels follow.
 template <int PREC, bool TILES, bool SP>
-static void launch_one(const MixedSegment *segs, int nseg, long total, int grid, hipStream_t s) {
+static void launch_one(const MixedSegment *segs, int nseg, long total, int grid, hipStream_t s,
+                       unsigned long long *sched) {
   hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<PREC, TILES, SP>), dim3((unsigned)grid),
-                     dim3(512), 0, s, segs, nseg, total);
+                     dim3(512), 0, s, segs, nseg, total, sched);
 }
 template <bool TILES>
 static int launch_prec(const MixedSegment *segs, int nseg, long total, int precision, int grid,
-                       bool sparse, hipStream_t s) {
+                       bool sparse, hipStream_t s, unsigned long long *q) {
   if (precision == TCE_T_FP64)
-    sparse ? launch_one<TCE_T_FP64, TILES, true>(segs, nseg, total, grid, s)
-           : launch_one<TCE_T_FP64, TILES, false>(segs, nseg, total, grid, s);
+    sparse ? launch_one<TCE_T_FP64, TILES, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_FP64, TILES, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_FP32)
-    sparse ? launch_one<TCE_T_FP32, TILES, true>(segs, nseg, total, grid, s)
-           : launch_one<TCE_T_FP32, TILES, false>(segs, nseg, total, grid, s);
+    sparse ? launch_one<TCE_T_FP32, TILES, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_FP32, TILES, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_BF16X3)
-    sparse ? launch_one<TCE_T_BF16X3, TILES, true>(segs, nseg, total, grid, s)
-           : launch_one<TCE_T_BF16X3, TILES, false>(segs, nseg, total, grid, s);
+    sparse ? launch_one<TCE_T_BF16X3, TILES, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_BF16X3, TILES, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_BF16X6)
-    sparse ? launch_one<TCE_T_BF16X6, TILES, true>(segs, nseg, total, grid, s)
-           : launch_one<TCE_T_BF16X6, TILES, false>(segs, nseg, total, grid, s);
+    sparse ? launch_one<TCE_T_BF16X6, TILES, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_BF16X6, TILES, false>(segs, nseg, total, grid, s, q);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
-                       hipStream_t s, bool sparse) {
-  return launch_prec<false>(segs, nseg, total, precision, grid, sparse, s);
+                       hipStream_t s, bool sparse, unsigned long long *sched) {
+  return launch_prec<false>(segs, nseg, total, precision, grid, sparse, s, sched);
 }
 
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
-                       hipStream_t s, bool sparse) {
-  return launch_prec<true>(segs, 1, total, precision, grid, sparse, s);
+                       hipStream_t s, bool sparse, unsigned long long *sched) {
+  return launch_prec<true>(segs, 1, total, precision, grid, sparse, s, sched);
 }
+
 
 }  // namespace tce_t

@@ -34,6 +34,7 @@ struct Job {
   long ngroups() const { return (long)bound.size() - 1; }
   int device = 0;
   hipStream_t stream = nullptr;
+  unsigned long long *dsched = nullptr;  // dynamic-schedule counters of the job's stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double *dT1 = nullptr, *dT2e = nullptr, *dT2m = nullptr, *dVd2 = nullptr, *dWd1 = nullptr,
          *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;

@@ -151,6 +151,28 @@ int launch_task_reduce(const long *bounds, int ntasks, int ppi, const double *pa
                      part, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+unsigned long long *sched_alloc() {
+  unsigned long long *p = nullptr;
+  if (!hip_ok(hipMalloc(&p, SCHED_WORDS * sizeof(unsigned long long)), "schedule counters"))
+    return nullptr;
+  // completed before return: the launching streams do not wait for the null stream
+  if (!hip_ok(hipMemsetAsync(p, 0, SCHED_WORDS * sizeof(unsigned long long), nullptr),
+              "schedule counters") ||
+      !hip_ok(hipStreamSynchronize(nullptr), "schedule counters")) {
+    (void)hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+
+unsigned long long *sched_use(unsigned long long *own) {
+  static const bool stat = [] {
+    const char *e = getenv("TCE_T_SCHED");
+    return e && !strcmp(e, "static");
+  }();
+  return stat ? nullptr : own;
+}
 }  // namespace tce_t
 
 extern "C" int tce_t_device_memory(long *free_bytes, long *total_bytes) {
@@ -707,6 +729,7 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
     return hip_ok(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), what);
   };
   if (!J.stream) HIPCHK(hipStreamCreateWithFlags(&J.stream, hipStreamNonBlocking), "stream");
+  if (!J.dsched && !(J.dsched = sched_alloc())) return TCE_T_ERR_HIP;
   if (!J.ev0) HIPCHK(hipEventCreate(&J.ev0), "event");
   if (!J.ev1) HIPCHK(hipEventCreate(&J.ev1), "event");
   // Dense tensors: fetched and sorted on the device (pack.hip) from the
@@ -993,7 +1016,8 @@ extern "C" int tce_t_job_run(tce_t_job *job, int precision, double *e_bt, double
     if (!sg) return TCE_T_ERR_HIP;
     HIPCHK(hipEventRecord(J.ev0, J.stream), "event");
     lrc = launch_fused_mixed(sg, 1, J.nitems8, precision,
-                             (int)std::min<long>(std::max(1, cus), J.nitems8), J.stream, J.sparse);
+                             (int)std::min<long>(std::max(1, cus), J.nitems8), J.stream, J.sparse,
+                             sched_use(J.dsched));
   } else {
     const long grid = std::min<long>(J.grid, J.nitems);
     const Segment *sg = stage_table(J, A, J.nitems);
@@ -1087,7 +1111,7 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
         if (!sg) return TCE_T_ERR_HIP;
         lrc = launch_fused_mixed(sg, 1, i1 - i0, precision,
                                  (int)std::min<long>(std::max(1, cus), i1 - i0), J.stream,
-                                 J.sparse);
+                                 J.sparse, sched_use(J.dsched));
       } else {
         const Segment *sg = stage_table(J, A, i1, i0);
         if (!sg) return TCE_T_ERR_HIP;
@@ -1229,7 +1253,7 @@ static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, lo
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, J.device), "attr");
   if (launch_fused_tiles(sg, items, precision, (int)std::min<long>(std::max(1, cus), items),
-                         J.stream, J.sparse) != 0) {
+                         J.stream, J.sparse, sched_use(J.dsched)) != 0) {
     set_error("tile kernel launch failed");
     return TCE_T_ERR_HIP;
   }
@@ -1387,6 +1411,7 @@ extern "C" int tce_t_job_destroy(tce_t_job *job) {
   if (J->tab_host) (void)hipHostFree(J->tab_host);
   if (J->ev0) (void)hipEventDestroy(J->ev0);
   if (J->ev1) (void)hipEventDestroy(J->ev1);
+  if (J->dsched) (void)hipFree(J->dsched);
   if (J->stream) (void)hipStreamDestroy(J->stream);
   delete J;
   return TCE_T_OK;

@@ -86,6 +86,7 @@ struct Server {
   bool stop = false;
   std::thread worker;
   hipStream_t stream = nullptr;
+  unsigned long long *sched = nullptr;  // dynamic-schedule counters of the stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *tab_host = nullptr, *tab_dev = nullptr;  // launch tables (pinned / device)
   size_t tab_bytes = 0;
@@ -197,7 +198,7 @@ int run_slice(Server &S, int precision, const std::vector<Request *> &rs, float 
   const int grid = (int)std::min<long>(S.cus, g);
   const int rc =
       launch_fused_mixed((const MixedSegment *)(d + n * asz), (int)ns, g, precision, grid, S.stream,
-                         sparse);
+                         sparse, sched_use(S.sched));
   if (rc != 0) {
     set_error("slice launch failed");
     return TCE_T_ERR_HIP;
@@ -339,6 +340,10 @@ extern "C" int tce_t_server_create(int device, double slice_ms, int max_active, 
   if ((e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
   if ((e = hipEventCreate(&S->ev0)) != hipSuccess) return fail(e, "event");
   if ((e = hipEventCreate(&S->ev1)) != hipSuccess) return fail(e, "event");
+  if (!(S->sched = sched_alloc())) {
+    delete S;
+    return TCE_T_ERR_HIP;
+  }
   S->tab_bytes = (size_t)MAX_ACTIVE * (sizeof(MixedArgs) + 16 + sizeof(MixedSegment)) + 256;
   if ((e = hipHostMalloc(&S->tab_host, S->tab_bytes)) != hipSuccess) return fail(e, "slice table");
   if ((e = hipMalloc(&S->tab_dev, S->tab_bytes)) != hipSuccess) return fail(e, "slice table");
@@ -434,6 +439,7 @@ extern "C" int tce_t_server_destroy(tce_t_server *server) {
   for (auto &sy : S->systems) tce_t_job_destroy(sy->job);
   if (S->tab_dev) (void)hipFree(S->tab_dev);
   if (S->tab_host) (void)hipHostFree(S->tab_host);
+  if (S->sched) (void)hipFree(S->sched);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   if (S->stream) (void)hipStreamDestroy(S->stream);
