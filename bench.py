@@ -38,6 +38,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "(T) TFLOP/s & energies/s, FP64 vs split-BF16 at ≤1 µEh; serving p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6
+# dense MFMA peaks of the reduced-precision modes (MI355X): FP32, and BF16
+# for both split modes; products per FP64-equivalent product of each mode
+PEAK_MODE_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
+MODE_PRODUCTS = {"fp32": 1, "bf16x3": 3, "bf16x6": 6}
 
 
 def job_work(tm, precision: int = 0, tiling: int = 2) -> float:
@@ -63,19 +67,24 @@ def traffic_from_profiles(workload: str, kernel: str):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
     or None when no summary exists for this kernel and workload."""
-    path = os.path.join(ROOT, "profiles", kernel + "_pmc_" + workload + ".json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    if "hbm_bytes_per_launch" in d:
-        return d["hbm_bytes_per_launch"]
     # {kernel: figures} per kernel of the run, keyed by the template name
-    # (fused_triples_mixed_fp64 is fused_triples_mixed<0, false>)
+    # (fused_triples_mixed_fp64 is fused_triples_mixed<0, false, false>:
+    # precision, tile export, narrowed contraction ranges), in the kernel's
+    # own summary or in the summary of all precisions
     base, _, prec = kernel.rpartition("_")
-    key = f"{base}<{ {'fp64': 0, 'fp32': 1, 'bf16x3': 2}.get(prec, -1)}, false>"
-    v = d.get(key)
-    return v.get("hbm_bytes_per_launch") if isinstance(v, dict) else None
+    key = f"{base}<{ {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'bf16x6': 3}.get(prec, -1)}, false, false>"
+    for name in (kernel, base + "_all_precisions"):
+        path = os.path.join(ROOT, "profiles", name + "_pmc_" + workload + ".json")
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            d = json.load(f)
+        if "hbm_bytes_per_launch" in d:
+            return d["hbm_bytes_per_launch"]
+        v = d.get(key)
+        if isinstance(v, dict) and "hbm_bytes_per_launch" in v:
+            return v["hbm_bytes_per_launch"]
+    return None
 
 
 def host_cpu() -> dict:
@@ -483,6 +492,13 @@ def main() -> int:
             "kernel": f"fused_triples_mixed_{name}",
             "extra_device_bytes": job.precision_bytes(prec),
         }
+        # the mode's own MFMA rate: its products (1, 3 or 6 per FP64-equivalent
+        # product, contraction padding not counted) over the kernel time
+        ach = MODE_PRODUCTS[name] * job.work(prec)[0] / (np.mean(mk) * 1e-3) / 1e12
+        modes[name]["roofline"] = {
+            "bound": "mfma", "achieved": ach, "peak": PEAK_MODE_TFLOPS[name], "unit": "TFLOP/s",
+            "frac": ach / PEAK_MODE_TFLOPS[name], "products_per_fp64_product": MODE_PRODUCTS[name],
+            "traffic": traffic_from_profiles(workload, f"fused_triples_mixed_{name}")}
     tiles_out = None
     if args.task_tiles > 0:
         # tce_t_job_triples_tiles: the first tasks' t3s/t3d tiles back on the
