@@ -398,6 +398,38 @@ long tce_t_v2_offset(const tce_t_tiles *tiles, long *hash, long *size);
 /* tce_hash.F:271-316 -- offset of key, or -1. */
 long tce_t_hash(const long *hash, long key);
 
+/* Orbital-form two-electron integrals (intorb: a closed-shell run whose
+ * integrals are stored over spatial orbitals, get_hash_block_i).  The alpha
+ * tile list and the map of the spin-orbital tiles onto it, as
+ * tce_tile.F:1156-1212,1370-1380 set them (k_b2am, k_spin_alpha,
+ * k_sym_alpha, k_range_alpha), and the irrep of V (irrep_v). */
+typedef struct {
+  long noa, nva;             /* alpha occupied and virtual tiles            */
+  const long *b2am;          /* [noab+nvab]: alpha tile (1-based) of each    */
+                             /* spin-orbital tile                           */
+  const long *spin_alpha;    /* [noa+nva] (1 in a closed-shell run)         */
+  const long *sym_alpha;     /* [noa+nva]                                   */
+  const long *range_alpha;   /* [noa+nva]                                   */
+  long irrep_v;
+} tce_t_intorb;
+/* tce_mo2e_offset_intorb.F:1-110 -- the orbital-form block table
+ * k_v2_alpha_offset: hash[0] = segments L1, then per entry (L1+1 of them,
+ * the first block, every (blocks/idiv2e)-th block and the last) its key,
+ * offset and tiles g3, g4, g1, g2 (six columns).  hash may be NULL to query
+ * the length.  Returns the table's length in longs (-1: invalid, or fewer
+ * blocks than idiv2e); *size = doubles of the orbital-form array. */
+long tce_t_v2_alpha_offset(const tce_t_intorb *io, long idiv2e, long *hash, long *size);
+/* get_hash_block_i / get_block_ind_i (get_block_ind.F:818-1470, ioalg 2)
+ * over every block of v2_hash (tce_t_v2_offset's table, what
+ * tce_t_job_upload reads): the antisymmetrised spin-orbital blocks
+ * v(g3 g4 g1 g2) = (g3 g1|g4 g2) - (g3 g2|g4 g1), each term kept where its
+ * spins pair up, from the orbital-form integrals v2orb (the layout
+ * tce_mo2e_trans.F writes).  v2_alpha_offset (the caller's
+ * k_v2_alpha_offset, may be NULL) is checked against the alpha tiles.
+ * Host threads; TCE_T_ERR_KEY if a source block is not stored. */
+int tce_t_v2_from_intorb(const tce_t_tiles *tiles, const tce_t_intorb *io, const double *v2orb,
+                         const long *v2_alpha_offset, const long *v2_hash, double *v2);
+
 /* Seeded synthetic C1 spin-orbital inputs (Philox4x32-10 counters keyed by
  * global orbital indices, Box-Muller normals):
  *   t1 ~ N(0,0.01), t2 ~ N(0,0.02) antisymmetrised in both pairs,

@@ -35,6 +35,12 @@ class TilesC(C.Structure):
     ]
 
 
+class IntorbC(C.Structure):
+    """Mirror of tce_t_intorb (include/tce_t.h)."""
+    _fields_ = [("noa", C.c_long), ("nva", C.c_long), ("b2am", LP), ("spin_alpha", LP),
+                ("sym_alpha", LP), ("range_alpha", LP), ("irrep_v", C.c_long)]
+
+
 def build(force: bool = False) -> str:
     """Compile libtce_t.so in-tree (hipcc, gfx950)."""
     cmd = ["make", "-C", CSRC]
@@ -105,6 +111,9 @@ def _declare(L: C.CDLL) -> None:
         getattr(L, f).restype = C.c_long
     L.tce_t_hash.argtypes = [LP, C.c_long]
     L.tce_t_hash.restype = C.c_long
+    L.tce_t_v2_alpha_offset.argtypes = [C.POINTER(IntorbC), C.c_long, LP, LP]
+    L.tce_t_v2_alpha_offset.restype = C.c_long
+    L.tce_t_v2_from_intorb.argtypes = [C.POINTER(TilesC), C.POINTER(IntorbC), DP, LP, LP, DP]
     L.tce_t_synth_evl.argtypes = [C.c_ulonglong, C.c_long, C.c_long, DP]
     L.tce_t_synth_blocks.argtypes = [C.POINTER(TilesC), C.c_ulonglong, DP, LP, DP, LP, DP, LP]
     L.tce_t_synth_dense.argtypes = [C.c_ulonglong, C.c_long, C.c_long, DP, DP, DP, DP, DP]

@@ -27,6 +27,13 @@ module tce_t_mi355x
     integer(c_int) :: restricted
   end type tce_t_tiles
 
+  ! Orbital-form integrals (intorb): alpha tiles and k_b2am.
+  type, bind(C) :: tce_t_intorb
+    integer(c_long) :: noa, nva
+    type(c_ptr) :: b2am, spin_alpha, sym_alpha, range_alpha
+    integer(c_long) :: irrep_v
+  end type tce_t_intorb
+
   interface
     integer(c_int) function tce_t_init(device) bind(C, name="tce_t_init")
       import :: c_int
@@ -152,6 +159,30 @@ module tce_t_mi355x
       type(tce_t_tiles), intent(in) :: tiles
       integer(c_long), intent(out) :: hash(*)
       integer(c_long), intent(out) :: size
+    end function
+
+    ! k_v2_alpha_offset of tce_mo2e_offset_intorb.F (length query: pass a
+    ! one-element hash and size, then call again with the full table)
+    integer(c_long) function tce_t_v2_alpha_offset(io, idiv2e, hash, size) &
+        bind(C, name="tce_t_v2_alpha_offset")
+      import :: c_long, tce_t_intorb
+      type(tce_t_intorb), intent(in) :: io
+      integer(c_long), value :: idiv2e
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    ! get_hash_block_i over every block of v2_hash: the spin-orbital V2
+    ! blocks tce_t_job_upload reads, from the orbital-form d_v2orb
+    integer(c_int) function tce_t_v2_from_intorb(tiles, io, v2orb, v2_alpha_offset, v2_hash, v2) &
+        bind(C, name="tce_t_v2_from_intorb")
+      import :: c_int, c_long, c_double, tce_t_tiles, tce_t_intorb
+      type(tce_t_tiles), intent(in) :: tiles
+      type(tce_t_intorb), intent(in) :: io
+      real(c_double), intent(in) :: v2orb(*)
+      integer(c_long), intent(in) :: v2_alpha_offset(*)
+      integer(c_long), intent(in) :: v2_hash(*)
+      real(c_double), intent(out) :: v2(*)
     end function
 
     integer(c_int) function tce_t_synth_evl(seed, no, nv, evl) bind(C, name="tce_t_synth_evl")

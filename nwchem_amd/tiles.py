@@ -190,3 +190,51 @@ def synth_dense(seed: int, no: int, nv: int) -> dict:
     check(_lib.lib().tce_t_synth_dense(seed, no, nv, _dp(T1), _dp(T2), _dp(W), _dp(V), _dp(S)),
           "tce_t_synth_dense")
     return {"T1": T1, "T2": T2, "W": W, "V": V, "S": S}
+
+
+@dataclass
+class Intorb:
+    """Orbital-form integral description (tce_t_intorb): the alpha tile list
+    (k_spin_alpha, k_sym_alpha, k_range_alpha, noa occupied then nva
+    virtual), the alpha tile of every spin-orbital tile (k_b2am) and the
+    irrep of V."""
+
+    noa: int
+    nva: int
+    b2am: np.ndarray
+    spin_alpha: np.ndarray
+    sym_alpha: np.ndarray
+    range_alpha: np.ndarray
+    irrep_v: int = 0
+
+    def __post_init__(self) -> None:
+        for f in ("b2am", "spin_alpha", "sym_alpha", "range_alpha"):
+            setattr(self, f, np.ascontiguousarray(getattr(self, f), np.int64))
+
+    def c(self):
+        return _lib.IntorbC(self.noa, self.nva, _lp(self.b2am), _lp(self.spin_alpha),
+                            _lp(self.sym_alpha), _lp(self.range_alpha), self.irrep_v)
+
+    def alpha_offset(self, idiv2e: int = 2) -> tuple[np.ndarray, int]:
+        """k_v2_alpha_offset and the orbital-form array size
+        (tce_t_v2_alpha_offset)."""
+        L = _lib.lib()
+        size = C.c_long(0)
+        n = L.tce_t_v2_alpha_offset(C.byref(self.c()), idiv2e, None, C.byref(size))
+        if n < 0:
+            raise RuntimeError(L.tce_t_last_error().decode())
+        h = np.zeros(n, np.int64)
+        L.tce_t_v2_alpha_offset(C.byref(self.c()), idiv2e, _lp(h), C.byref(size))
+        return h, int(size.value)
+
+
+def v2_from_intorb(tiles: TileModel, io: Intorb, v2orb: np.ndarray, alpha_offset, v2h,
+                   size: int) -> np.ndarray:
+    """The spin-orbital V2 blocks of v2h from orbital-form integrals
+    (tce_t_v2_from_intorb, get_hash_block_i)."""
+    v2 = np.zeros(max(size, 1))
+    v2orb = np.ascontiguousarray(v2orb, np.float64)
+    check(_lib.lib().tce_t_v2_from_intorb(C.byref(tiles.c()), C.byref(io.c()), _dp(v2orb),
+                                          None if alpha_offset is None else _lp(alpha_offset),
+                                          _lp(v2h), _dp(v2)), "tce_t_v2_from_intorb")
+    return v2

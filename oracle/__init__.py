@@ -35,6 +35,11 @@ class OraTiles(C.Structure):
                 ("offset", LP), ("alpha", LP), ("evl", DP), ("restricted", C.c_int)]
 
 
+class OraIntorb(C.Structure):
+    _fields_ = [("noa", C.c_long), ("nva", C.c_long), ("b2am", LP), ("spin_alpha", LP),
+                ("sym_alpha", LP), ("range_alpha", LP), ("irrep_v", C.c_long)]
+
+
 class OraBackend(C.Structure):
     _fields_ = [("usedevice", C.c_int), ("s1", K6 * 9), ("d1", K7 * 9), ("d2", K7 * 9),
                 ("initmemmodule", V0), ("finalizememmodule", V0), ("dev_mem_s", M6),
@@ -97,6 +102,14 @@ def lib() -> C.CDLL:
         L.ora_brute_force.argtypes = [C.c_long, C.c_long] + [DP] * 7 + [DP, DP]
         L.ora_task_tiles.argtypes = [C.POINTER(OraTiles), DP, LP, DP, LP, DP, LP, LP, DP, DP]
         L.ora_task_tiles.restype = C.c_long
+        L.ora_v2_alpha_offset.argtypes = [C.POINTER(OraIntorb), C.c_long, LP, LP]
+        L.ora_v2_alpha_offset.restype = C.c_long
+        L.ora_hash_v2.argtypes = [C.POINTER(OraIntorb), LP, C.c_long]
+        L.ora_hash_v2.restype = C.c_long
+        L.ora_v2orb_pack.argtypes = [C.POINTER(OraIntorb), DP, C.c_long, DP]
+        L.ora_v2orb_pack.restype = None
+        L.ora_v2_from_intorb.argtypes = [C.POINTER(OraTiles), C.POINTER(OraIntorb), DP, LP, LP, DP]
+        L.ora_v2_from_intorb.restype = C.c_int
         L._declared = True
     return L
 
@@ -269,3 +282,49 @@ def set_mutation(family: int, n: int) -> None:
     """Plant (family, n >= 1) or remove (family = -1) a sign mistake in one
     built-in kernel variant (test of the checks' sensitivity)."""
     lib().ora_set_mutation(family, n)
+
+
+class Intorb:
+    """Owns the arrays behind an OraIntorb view (orbital-form integrals:
+    alpha tiles, k_b2am, irrep of V)."""
+
+    def __init__(self, noa, nva, b2am, spin_alpha, sym_alpha, range_alpha, irrep_v=0):
+        self.noa, self.nva, self.irrep_v = int(noa), int(nva), int(irrep_v)
+        self.b2am = np.ascontiguousarray(b2am, np.int64)
+        self.spin_alpha = np.ascontiguousarray(spin_alpha, np.int64)
+        self.sym_alpha = np.ascontiguousarray(sym_alpha, np.int64)
+        self.range_alpha = np.ascontiguousarray(range_alpha, np.int64)
+        self.c = OraIntorb(self.noa, self.nva, _lp(self.b2am), _lp(self.spin_alpha),
+                           _lp(self.sym_alpha), _lp(self.range_alpha), self.irrep_v)
+
+    def alpha_offset(self, idiv2e: int = 2):
+        """tce_mo2e_offset_intorb: (k_v2_alpha_offset, orbital-form size)."""
+        L = lib()
+        size = C.c_long(0)
+        n = L.ora_v2_alpha_offset(C.byref(self.c), idiv2e, None, C.byref(size))
+        if n < 0:
+            raise ValueError("idiv2e leaves no segment")
+        h = np.zeros(n, np.int64)
+        L.ora_v2_alpha_offset(C.byref(self.c), idiv2e, _lp(h), C.byref(size))
+        return h, int(size.value)
+
+    def hash_v2(self, hash_, key: int) -> int:
+        return int(lib().ora_hash_v2(C.byref(self.c), _lp(hash_), key))
+
+    def pack(self, orb, size: int):
+        """tce_mo2e_trans's orbital-form array from dense (pq|rs)."""
+        orb = np.ascontiguousarray(orb, np.float64)
+        out = np.zeros(max(size, 1))
+        lib().ora_v2orb_pack(C.byref(self.c), _dp(orb), orb.shape[0], _dp(out))
+        return out
+
+
+def v2_from_intorb(tiles: Tiles, io: Intorb, v2orb, alpha_offset, v2h, size: int):
+    """get_hash_block_i over every block of v2h (ora_v2_from_intorb)."""
+    out = np.zeros(max(size, 1))
+    v2orb = np.ascontiguousarray(v2orb, np.float64)
+    rc = lib().ora_v2_from_intorb(C.byref(tiles.c), C.byref(io.c), _dp(v2orb), _lp(alpha_offset),
+                                  _lp(v2h), _dp(out))
+    if rc != 0:
+        raise KeyError("orbital-form key not found")
+    return out

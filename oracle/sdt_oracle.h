@@ -193,6 +193,26 @@ void ora_brute_force(long no, long nv, const double *T1, const double *T2,
                      const double *eo, const double *ev, double *e4,
                      double *e5);
 
+/* Orbital-form integral input (intorb_oracle.c): the alpha tile list
+ * (k_spin_alpha, k_sym_alpha, k_range_alpha), the map of spin-orbital tiles
+ * onto it (k_b2am) and the irrep of V. */
+typedef struct {
+  long noa, nva;
+  const long *b2am, *spin_alpha, *sym_alpha, *range_alpha;
+  long irrep_v;
+} ora_intorb_t;
+
+long ora_v2_alpha_offset(const ora_intorb_t *io, long idiv2e, long *hash, long *size);
+long ora_hash_v2(const ora_intorb_t *io, const long *hash, long key);
+void ora_v2orb_pack(const ora_intorb_t *io, const double *orb, long n, double *v2orb);
+void ora_sortacc_4(const double *unsorted, double *sorted, long a, long b, long c, long d,
+                   long i, long j, long k, long l, double factor);
+int ora_get_block_ind_i(const ora_tiles_t *t, const ora_intorb_t *io, const double *v2orb,
+                        const long *hash_alpha, long w2b, long w1b, long w4b, long w3b,
+                        double *array);
+int ora_v2_from_intorb(const ora_tiles_t *t, const ora_intorb_t *io, const double *v2orb,
+                       const long *hash_alpha, const long *v2_hash, double *v2);
+
 #ifdef __cplusplus
 }
 #endif

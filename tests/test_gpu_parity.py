@@ -307,6 +307,36 @@ def test_spin_tilings_match_oracle(prm, kernel, monkeypatch):
     assert_parity(e, te, ref)
 
 
+@pytest.mark.parametrize("name", ["c1_ragged", "irreps"])
+def test_orbital_form_integrals_through_the_job(name):
+    """A closed-shell system given as orbital-form integrals (intorb):
+    converted by tce_t_v2_from_intorb, uploaded and run.  The blocks equal
+    the spin-orbital ones built directly (CPU tests), so the energies must
+    equal the job's on those bit for bit, and the oracle's within the FP64
+    tolerance."""
+    import test_intorb as ti
+
+    ss, orb, am = ti.build(name)
+    no_s, nv_s = ss.no // 2, ss.nv // 2
+    ss.W, ss.V, ss.S = ti.spin_orbital_v(orb, no_s, nv_s)
+    direct = ss._fill_blocks()
+    io_o = oracle.Intorb(**am)
+    h, osize = io_o.alpha_offset(2)
+    v2orb = io_o.pack(orb, osize)
+    T = ss.tiles
+    v2 = tl.v2_from_intorb(T, tl.Intorb(**am), v2orb, h, direct.v2h, T.sizes()["v2"])
+    conv = tl.Blocks(direct.t1, direct.t1h, direct.t2, direct.t2h, v2, direct.v2h)
+    ref = oracle.ccsd_t(ss.oracle_tiles(), conv.t1, conv.t1h, conv.t2, conv.t2h, conv.v2, conv.v2h)
+    print(f"intorb case {name}: tasks={len(ref[2])}", flush=True)
+    out = []
+    for blocks in (conv, direct):
+        with TriplesJob(T) as job:
+            job.upload(blocks)
+            out.append((job.run(), job.task_energies()))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    assert_parity(out[0][0], out[0][1], ref)
+
+
 @pytest.mark.parametrize("precision", [0, 2], ids=["fp64", "bf16x3"])
 @pytest.mark.parametrize("prm", SPIN_GPU, ids=["spin_irreps", "spin_8_16", "restricted"])
 def test_forbidden_blocks_are_skipped_exactly(prm, precision, monkeypatch):
