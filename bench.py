@@ -413,6 +413,9 @@ def main() -> int:
                     help="timed runs per reduced-precision mode (at most --steps)")
     ap.add_argument("--serve", type=int, default=200, help="serving requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-low", type=int, default=60,
+                    help="requests of a second serving run below the device's capacity (0: skip)")
+    ap.add_argument("--serve-low-rate", type=float, default=4.0)
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--slice-ms", type=float, default=50.0)
     ap.add_argument("--reference-tiling", type=int, default=1,
@@ -560,6 +563,9 @@ def main() -> int:
         rates[3] = modes["bf16x6"]["fp64_equiv_tflops"] * 1e12
     serve = bench_serve(args.serve, args.serve_rate, args.serve_pool, args.seed, args.slice_ms,
                         rates if len(rates) == 2 else None) if args.serve else None
+    # the same pool and request sequence at an arrival rate the device keeps up with
+    serve_low = bench_serve(args.serve_low, args.serve_low_rate, args.serve_pool, args.seed,
+                            args.slice_ms, rates if len(rates) == 2 else None) if args.serve_low else None
     cpu = None
     cfg1 = None
     if args.cpu_baseline:
@@ -601,6 +607,7 @@ def main() -> int:
         "task_tiles": tiles_out,
         "batch": batch,
         "serving": serve,
+        "serving_below_capacity": serve_low,
     }
     print(json.dumps(out))
     return 0
