@@ -163,8 +163,20 @@ __device__ __forceinline__ long particle_of(const ItemGeo &G, int t, int slot) {
 // This wave's cube coordinate along particle x (0 = a, 1 = b, 2 = c).
 __device__ __forceinline__ int cube(int wave, int x) { return (wave >> (2 - x)) & 1; }
 
-__device__ __forceinline__ int x_swz(int slot, int r) { return (slot & 3) ^ (r & 2); }
-__device__ __forceinline__ int y_swz(int p1, int p2) { return (p1 & 3) ^ (p2 & 2); }
+// Quad swizzles of the operand rows: LDS position k of a row holds the row's
+// global quad k ^ swz, for both the loads (which write each row linearly)
+// and the fragment reads.  Measured with gfx950's SQ_LDS_BANK_CONFLICT over
+// every swizzle that is linear in the row bits: a ds_read_b128 whose 16
+// lanes of one kq read rows (slot or p1 from lane bits 2-3, r or p2 from
+// bits 0-1) is conflict-free with s(row) = -(slot & 3) mod 4 (FP32 and bf16
+// X reads, every Y read), while (slot & 3) ^ (r & 2) took two passes (4
+// conflict cycles per read).  The FP64 X reads put slot on lane bits 0-1 and
+// r on bits 2-3, for which (slot & 3) ^ (r & 2) is conflict-free.
+template <int PREC>
+__device__ __forceinline__ int x_swz(int slot, int r) {
+  return PREC == TCE_T_FP64 ? (slot & 3) ^ (r & 2) : (4 - (slot & 3)) & 3;
+}
+__device__ __forceinline__ int y_swz(int p1, int) { return (4 - (p1 & 3)) & 3; }
 
 // Direct global->LDS load of 16 bytes per lane: lane l writes LDS bytes
 // [m0 + 16 l, m0 + 16 l + 16).  Issued by inline assembly so that the
@@ -207,7 +219,7 @@ __device__ __forceinline__ Src<PREC> pass_src(const MixedArgsG &M, const ItemGeo
   // X block `wave`: s-index wave>>1, slots 4*(wave&1) + lane>>4
   const int xs = wave >> 1, slot = 4 * (wave & 1) + (lane >> 4), xr = (lane >> 2) & 3;
   const long xe = (((long)hole_t<r>(G, xr) * no + hole_t<s>(G, xs)) * nv + particle_t<L>(G, slot)) *
-                      Kp + QE * ((lane & 3) ^ x_swz(slot, xr));
+                      Kp + QE * ((lane & 3) ^ x_swz<PREC>(slot, xr));
   // Y blocks 2w+t: h-index bq>>2, p1 slots 2*(bq&3) + lane>>5
   long ye[2];
 #pragma unroll
@@ -432,7 +444,7 @@ __device__ __forceinline__ int2 frag_offsets(int wave, int lane) {
   const int xL = PREC == TCE_T_FP64 ? lo : hi, xr = PREC == TCE_T_FP64 ? hi : lo;
   const int slot = 4 * cube(wave, Lp) + xL;
   const int p1s = 4 * cube(wave, p1) + hi, p2s = 4 * cube(wave, p2) + lo;
-  return make_int2((slot * 4 + xr) * 4 + (kq ^ x_swz(slot, xr)),
+  return make_int2((slot * 4 + xr) * 4 + (kq ^ x_swz<PREC>(slot, xr)),
                    (p1s * 8 + p2s) * 4 + (kq ^ y_swz(p1s, p2s)));
 }
 
