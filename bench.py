@@ -247,11 +247,13 @@ def config1(threads: int) -> dict | None:
     return out
 
 
-def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2, 3)) -> dict:
-    """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256}, occupied tiles
-    6-20, virtual tiles 16-48, ragged last tiles) in one persistent launch per
-    run (autotuned, replayed as a hipGraph), against the same jobs run one
-    after another.  The jobs use the orbital tiling (E[T], E(T) per job)."""
+def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2, 3), no_range=(6, 30),
+                nv_range=(48, 256)) -> dict:
+    """Many independent jobs (no ~ U{6..30}, nv ~ U{48..256} by default,
+    occupied tiles 6-20, virtual tiles 16-48, ragged last tiles) in one
+    persistent launch per run (autotuned, replayed as a hipGraph), against the
+    same jobs run one after another.  The jobs use the orbital tiling (E[T],
+    E(T) per job)."""
     from nwchem_amd import TILING_ORBITAL, TriplesBatch, TriplesJob, tiles as tl
 
     rng = np.random.default_rng(seed)
@@ -259,7 +261,7 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2, 3)) -> dict:
     Fm = {p: 0.0 for p in modes}
     t0 = time.perf_counter()
     for k in range(njobs):
-        no, nv = int(rng.integers(6, 31)), int(rng.integers(48, 257))
+        no, nv = int(rng.integers(no_range[0], no_range[1] + 1)), int(rng.integers(nv_range[0], nv_range[1] + 1))
         to, tv = int(rng.integers(6, 21)), int(rng.integers(16, 49))
         tm = tl.TileModel.synthetic(no, nv, 0, 0, seed + k, ragged(no, to), ragged(nv, tv))
         b = tl.synth_blocks(tm, seed + k)
@@ -409,6 +411,9 @@ def main() -> int:
                     help="batch of independent jobs (config 4: 64; 0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="timed batch runs per precision (graph replays)")
+    ap.add_argument("--batch-small-jobs", type=int, default=128,
+                    help="a second batch of small jobs (no in [6,10], nv in [48,64]) that "
+                         "underfill the GPU one by one (0: skip)")
     ap.add_argument("--mode-steps", type=int, default=5,
                     help="timed runs per reduced-precision mode (at most --steps)")
     ap.add_argument("--serve", type=int, default=200, help="serving requests (0: skip)")
@@ -558,6 +563,9 @@ def main() -> int:
                 "kernel": kernel, "kernel_ms": k_avg, "flops_per_launch": F_run,
                 "items_per_launch": items}
     batch = bench_batch(args.batch_jobs, args.seed, max(1, args.batch_steps)) if args.batch_jobs else None
+    batch_small = (bench_batch(args.batch_small_jobs, args.seed + 7, max(1, args.batch_steps),
+                               no_range=(6, 10), nv_range=(48, 64))
+                   if args.batch_small_jobs else None)
     rates = {0: F_run * args.steps / wall}
     if "bf16x6" in modes:
         rates[3] = modes["bf16x6"]["fp64_equiv_tflops"] * 1e12
@@ -606,6 +614,7 @@ def main() -> int:
         "precision_modes": modes or None,
         "task_tiles": tiles_out,
         "batch": batch,
+        "batch_small_jobs": batch_small,
         "serving": serve,
         "serving_below_capacity": serve_low,
     }

@@ -825,6 +825,12 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     int total = 0;  // chunks of the item (the passes' contraction ranges)
 #pragma unroll
     for (int p = 0; p < 18; ++p) total += chunks_of<PREC, SP>(M, Tp, p).n;
+    // boxes of a multi-box task (reference and orbital tilings): hole boxes
+    // outermost, particle c fastest.  Measured against hole k fastest on the
+    // orbital-tiling jobs of the 64-job batch (up to 32 particle boxes per
+    // index against at most 8 hole boxes): 6-7% faster in split-BF16, 2% in
+    // FP64.  (Sub-tile tables have one box per task and order their tasks
+    // themselves.)
     long box = item - T.item0;
     const int bc = box % T.nb6; box /= T.nb6;
     const int bb = box % T.nb5; box /= T.nb5;
