@@ -354,6 +354,12 @@ double tce_t_batch_last_ms(const tce_t_batch *batch);
  * the process-wide cache). */
 int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family, int *cost_order,
                      double *tuned_ms);
+/* The work-queue slice width the plan chose for fused_triples_mixed (the
+ * consecutive items each XCD takes at a time) and slice_ms[3], the tuning
+ * runs at widths 32, 64, 128 (-1 where not tried: other kernel families, or
+ * a plan from the process-wide cache).  No reference counterpart. */
+int tce_t_batch_plan_schedule(const tce_t_batch *batch, int precision, int *slice,
+                              double *slice_ms);
 int tce_t_batch_destroy(tce_t_batch *batch);
 
 /* ---- serving: continuous batching of (T) requests on one GPU ---------- */

@@ -85,6 +85,7 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_batch_last_ms.argtypes = [C.c_void_p]
     L.tce_t_batch_last_ms.restype = C.c_double
     L.tce_t_batch_plan.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), DP]
+    L.tce_t_batch_plan_schedule.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), DP]
     L.tce_t_batch_destroy.argtypes = [C.c_void_p]
     L.tce_t_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_long, C.POINTER(C.c_void_p)]
     L.tce_t_server_add_system.argtypes = [C.c_void_p, C.POINTER(TilesC), DP, LP, DP, LP, DP, LP]

@@ -9,11 +9,13 @@
 //   autotuner  candidates are the kernel family for FP64 (fused_triples_wg
 //              with 8-wide particle boxes on 4x4x4 MFMA, fused_triples_mixed
 //              on 16x16x4, fused_triples with 4-wide boxes) and
-//              the segment order (as given; costliest items first, so the
-//              static round-robin schedule ends on the cheapest items).
-//              Each candidate runs once, timed with HIP events; the fastest
-//              is kept and remembered per batch shape (the sorted list of
-//              job dimensions and tilings) for later batches in the process.
+//              the segment order (as given; costliest items first).  For
+//              fused_triples_mixed the slice width of its work queues (the
+//              consecutive items an XCD takes at a time: 32, 64 or 128) is
+//              then tuned on the best of those.  Each candidate runs once,
+//              timed with HIP events; the fastest is kept and remembered per
+//              batch shape (the sorted list of job dimensions and tilings)
+//              for later batches in the process.
 //   graph      the chosen launch sequence is captured once into a hipGraph
 //              (kernel, reductions, device-to-host copies; no memset nodes)
 //              and replayed by every later run.
@@ -43,7 +45,9 @@ constexpr int NCAND = 6;  // at most 3 families x 2 orders
 struct Choice {
   int family = FAM_WG;
   bool cost_order = true;
+  int shift = 5;  // log2 of the work-queue slice width (fused_triples_mixed)
 };
+constexpr int NSLICE = 3;  // slice widths tried: 2^5, 2^6, 2^7
 
 std::mutex g_tune_mu;
 std::map<std::string, Choice> g_tuned;  // batch shape -> choice
@@ -57,6 +61,7 @@ struct Plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms[NCAND] = {-1, -1, -1, -1, -1, -1};  // candidate times (family x order)
+  double slice_ms[NSLICE] = {-1, -1, -1};             // slice widths 32, 64, 128
 };
 
 struct Batch {
@@ -182,6 +187,14 @@ int enqueue(Batch &B, int precision, const Plan &P) {
   return TCE_T_OK;
 }
 
+// Slice width of the batch stream's work queues (read by every later
+// launch of fused_triples_mixed on it, graph replays included).
+int set_slice(Batch &B, int shift) {
+  const unsigned long long v = (unsigned long long)shift;
+  HIPCHK(hipMemcpy(B.sched + 9, &v, sizeof v, hipMemcpyHostToDevice), "slice width");
+  return TCE_T_OK;
+}
+
 // Time one eager run of a plan (kernel and reductions).
 int time_plan(Batch &B, int precision, const Plan &P, float &ms) {
   HIPCHK(hipEventRecord(B.ev0, B.stream), "event");
@@ -220,8 +233,10 @@ int plan_batch(Batch &B, int precision) {
     for (int fam : fams)
       for (bool co : {true, false}) cands.push_back(Choice{fam, co});
     float best_ms = 0.f;
+    int rc = set_slice(B, 5);
+    if (rc != TCE_T_OK) return rc;
     for (size_t i = 0; i < cands.size(); ++i) {
-      int rc = build_table(B, precision, cands[i], P);
+      rc = build_table(B, precision, cands[i], P);
       if (rc != TCE_T_OK) return rc;
       float ms = 0.f;
       rc = time_plan(B, precision, P, ms);
@@ -229,10 +244,28 @@ int plan_batch(Batch &B, int precision) {
       P.tuned_ms[i] = ms;
       if (i == 0 || ms < best_ms) best_ms = ms, best = cands[i];
     }
+    // the work-queue slice width of the 16x16 kernel, on the best plan
+    if (mixed || best.family == FAM_WG16) {
+      P.slice_ms[0] = best_ms;
+      rc = build_table(B, precision, best, P);
+      if (rc != TCE_T_OK) return rc;
+      const Choice base = best;
+      for (int k = 1; k < NSLICE; ++k) {
+        Choice c = base;
+        c.shift = 5 + k;
+        if ((rc = set_slice(B, c.shift)) != TCE_T_OK) return rc;
+        float ms = 0.f;
+        if ((rc = time_plan(B, precision, P, ms)) != TCE_T_OK) return rc;
+        P.slice_ms[k] = ms;
+        if (ms < best_ms) best_ms = ms, best = c;
+      }
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[key] = best;
   }
-  int rc = build_table(B, precision, best, P);
+  int rc = set_slice(B, best.shift);
+  if (rc != TCE_T_OK) return rc;
+  rc = build_table(B, precision, best, P);
   if (rc != TCE_T_OK) return rc;
   HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal), "capture");
   rc = enqueue(B, precision, P);
@@ -346,6 +379,17 @@ extern "C" int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *fa
   if (cost_order) *cost_order = P.choice.cost_order ? 1 : 0;
   if (tuned_ms)
     for (int i = 0; i < NCAND; ++i) tuned_ms[i] = P.tuned_ms[i];
+  return TCE_T_OK;
+}
+
+extern "C" int tce_t_batch_plan_schedule(const tce_t_batch *batch, int precision, int *slice,
+                                         double *slice_ms) {
+  if (!batch || !valid_precision(precision)) return TCE_T_ERR_ARG;
+  const Plan &P = reinterpret_cast<const Batch *>(batch)->plan[precision];
+  if (!P.ready) return TCE_T_ERR_STATE;
+  if (slice) *slice = 1 << P.choice.shift;
+  if (slice_ms)
+    for (int i = 0; i < NSLICE; ++i) slice_ms[i] = P.slice_ms[i];
   return TCE_T_OK;
 }
 

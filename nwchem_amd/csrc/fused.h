@@ -203,8 +203,8 @@ __device__ __forceinline__ long round_item(long r, unsigned b, unsigned G, bool 
 }
 
 // Dynamic schedule of a persistent launch (fused_triples_mixed): the global
-// items are cut into windows of 8 x SCHED_SLICE, and queue q (one per XCD)
-// holds slice q of every window, in window order.  A workgroup draws tickets
+// items are cut into windows of 8 slices, and queue q (one per XCD) holds
+// slice q of every window, in window order.  A workgroup draws tickets
 // from the queue of its own XCD (b % 8) with a device-scope atomic add, so
 // the 32 CUs of an XCD share one slice of consecutive items at a time (their
 // operand rows meet in that XCD's L2) while each CU takes the next item when
@@ -214,20 +214,20 @@ __device__ __forceinline__ long round_item(long r, unsigned b, unsigned G, bool 
 // eight are.  Counters: sched[0..7] the queues' next tickets, sched[8] the
 // workgroups that have left; they are zero before a launch, and the last
 // workgroup to leave zeroes them for the next launch on the same stream.
-constexpr long SCHED_SLICE = 32;
-constexpr int SCHED_WORDS = 9;
+// sched[9] holds log2 of the slice width (set once with the counters).
+constexpr int SCHED_WORDS = 9;     // counters the last workgroup zeroes
+constexpr int SCHED_ALLOC = 10;    // counters + the slice width
 
-__device__ __forceinline__ long queue_item(unsigned long long k, unsigned q) {
-  return (long)(k / SCHED_SLICE) * (8 * SCHED_SLICE) + (long)q * SCHED_SLICE +
-         (long)(k % SCHED_SLICE);
+__device__ __forceinline__ long queue_item(unsigned long long k, unsigned q, int sh) {
+  return (long)((k >> sh) << (sh + 3)) + ((long)q << sh) + (long)(k & ((1ull << sh) - 1));
 }
 
 // Item of ticket k of queue q, or of a ticket of the next queues when q is
 // exhausted (q then moves on); -1 when every queue is.  One thread.
 __device__ __forceinline__ long sched_item(unsigned long long *sched, unsigned &q,
-                                           unsigned long long k, long n) {
+                                           unsigned long long k, long n, int sh) {
   for (int tried = 1;; ++tried) {
-    const long g = queue_item(k, q);
+    const long g = queue_item(k, q, sh);
     if (g < n) return g;
     if (tried == 8) return -1;
     q = (q + 1) & 7u;

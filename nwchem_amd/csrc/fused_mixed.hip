@@ -796,11 +796,12 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
   // (every barrier is reached by all 8 waves): the dynamic schedule of
   // sched_item when `sched` is given, else a static round-robin.
   const bool dyn = sched != nullptr;
+  const int sh = dyn ? (int)sched[9] : 0;  // log2 of the slice width
   unsigned q = blockIdx.x & 7u;  // thread 0: the queue it draws from
   unsigned long long ticket = 0;
   long g = 0;
   if (dyn) {
-    if (tid == 0) next_item = sched_item(sched, q, atomicAdd(&sched[q], 1ull), nglobal);
+    if (tid == 0) next_item = sched_item(sched, q, atomicAdd(&sched[q], 1ull), nglobal, sh);
     __syncthreads();
     g = next_item;
   }
@@ -894,7 +895,7 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
         A.partials[2 * (item * 8 + wave) + 1] = e5;
       }
     }
-    if (dyn && tid == 0) next_item = sched_item(sched, q, ticket, nglobal);
+    if (dyn && tid == 0) next_item = sched_item(sched, q, ticket, nglobal, sh);
     __syncthreads();  // the next item's first loads overwrite the staged data
     if (dyn) g = next_item;
   }

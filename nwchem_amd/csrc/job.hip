@@ -153,13 +153,25 @@ int launch_task_reduce(const long *bounds, int ntasks, int ppi, const double *pa
 }
 
 unsigned long long *sched_alloc() {
+  // slice width 2^shift items per XCD and window (TCE_T_SCHED_SLICE, a power
+  // of two from 1 to 1024; default 32)
+  static const unsigned long long shift = [] {
+    unsigned long long s = 5;
+    if (const char *e = getenv("TCE_T_SCHED_SLICE")) {
+      const long v = atol(e);
+      for (unsigned long long b = 0; b <= 10; ++b)
+        if (v == (1L << b)) s = b;
+    }
+    return s;
+  }();
   unsigned long long *p = nullptr;
-  if (!hip_ok(hipMalloc(&p, SCHED_WORDS * sizeof(unsigned long long)), "schedule counters"))
+  if (!hip_ok(hipMalloc(&p, SCHED_ALLOC * sizeof(unsigned long long)), "schedule counters"))
     return nullptr;
-  // completed before return: the launching streams do not wait for the null stream
-  if (!hip_ok(hipMemsetAsync(p, 0, SCHED_WORDS * sizeof(unsigned long long), nullptr),
-              "schedule counters") ||
-      !hip_ok(hipStreamSynchronize(nullptr), "schedule counters")) {
+  unsigned long long init[SCHED_ALLOC] = {};
+  init[9] = shift;
+  // synchronous copy from pageable memory: complete on return, before any
+  // launch on the streams that use the counters
+  if (!hip_ok(hipMemcpy(p, init, sizeof init, hipMemcpyHostToDevice), "schedule counters")) {
     (void)hipFree(p);
     return nullptr;
   }

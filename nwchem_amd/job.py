@@ -237,8 +237,13 @@ class TriplesBatch:
         tms = np.zeros(6)
         check(_lib.lib().tce_t_batch_plan(self._h, precision, C.byref(fam), C.byref(co),
                                           tms.ctypes.data_as(DP)), "tce_t_batch_plan")
+        sl, sms = C.c_int(), np.zeros(3)
+        check(_lib.lib().tce_t_batch_plan_schedule(self._h, precision, C.byref(sl),
+                                                   sms.ctypes.data_as(DP)),
+              "tce_t_batch_plan_schedule")
         return {"kernel": FAMILIES[fam.value], "cost_order": bool(co.value),
-                "tuned_ms": [float(x) for x in tms if x >= 0]}
+                "tuned_ms": [float(x) for x in tms if x >= 0], "slice": int(sl.value),
+                "slice_ms": [float(x) for x in sms if x >= 0]}
 
     def close(self) -> None:
         if self._h:

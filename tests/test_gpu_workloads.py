@@ -62,7 +62,14 @@ def test_batch_at_the_batch_workload_sizes():
         with TriplesBatch(jobs) as batch:
             for p in (0, 2):
                 e = batch.run(p)
-                print(f"precision {p}: plan {batch.plan(p)}, {batch.last_ms:.1f} ms", flush=True)
+                plan = batch.plan(p)
+                print(f"precision {p}: plan {plan}, {batch.last_ms:.1f} ms", flush=True)
+                # the autotuner's schedule: a work-queue slice of 32, 64 or 128
+                # items, with its tuning runs when the 16x16 kernel was chosen
+                # (none when the plan came from the process-wide cache)
+                assert plan["slice"] in (32, 64, 128)
+                if plan["kernel"].startswith("fused_triples_mixed") and plan["tuned_ms"]:
+                    assert len(plan["slice_ms"]) == 3 and min(plan["slice_ms"]) > 0
                 for k in range(len(jobs)):
                     assert tuple(e[k]) == single[p][k], (p, k, e[k], single[p][k])
         for k, (tm, b, _) in enumerate(systems[:len(SMALL)]):
