@@ -126,8 +126,7 @@ bool half_source(const Alpha &A, const std::unordered_map<long, long> &offs, Pai
   if (it == offs.end()) return false;
   s.offset = it->second;
   // stored [g1'][g2'][g3'][g4']
-  const long r1 = A.rg(k1), r2 = A.rg(k2), r3 = A.rg(k3), r4 = A.rg(k4);
-  (void)r1;
+  const long r2 = A.rg(k2), r3 = A.rg(k3), r4 = A.rg(k4);
   s.stride[e4] = 1;
   s.stride[e3] = r4;
   s.stride[e2] = r3 * r4;
@@ -229,8 +228,16 @@ extern "C" int tce_t_v2_from_intorb(const tce_t_tiles *tiles, const tce_t_intorb
     }
   }
   const long L = v2_hash[0];
-  std::vector<long> order(L);
-  for (long i = 0; i < L; ++i) order[i] = i;
+  const long nkeys = nso * nso * nso * nso;
+  if (L < 0) {
+    set_error("tce_t_v2_from_intorb: invalid block table");
+    return TCE_T_ERR_ARG;
+  }
+  for (long i = 0; i < L; ++i)
+    if (v2_hash[1 + i] < 0 || v2_hash[1 + i] >= nkeys || v2_hash[1 + L + i] < 0) {
+      set_error("tce_t_v2_from_intorb: a block key or offset is outside the tiles");
+      return TCE_T_ERR_ARG;
+    }
   int err = 0;
   auto work = [&](long i0, long i1) {
     for (long i = i0; i < i1; ++i) {

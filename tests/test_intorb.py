@@ -288,3 +288,7 @@ def test_bad_inputs_are_rejected():
     with pytest.raises(RuntimeError):
         tl.v2_from_intorb(T, tl.Intorb(**am2), v2orb, None, v2h, T.sizes()["v2"])
     assert _lib.lib().tce_t_v2_alpha_offset(None, 2, None, None) == -1
+    badh = v2h.copy()
+    badh[1] = (T.noab + T.nvab) ** 4  # a key past the last tile quadruple
+    with pytest.raises(RuntimeError):
+        tl.v2_from_intorb(T, io, v2orb, h, badh, T.sizes()["v2"])
