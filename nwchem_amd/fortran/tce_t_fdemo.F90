@@ -5,6 +5,9 @@
 ! E[T], E(T) once more from the task tiles the library hands back, summed
 ! here as the tiles' other consumers (cr_ccsd_t.F, lambda_ccsd_t.F) do.
 ! Usage: tce_t_fdemo [no nv tile_o tile_v]   (default 8 32 8 16)
+!        tce_t_fdemo intorb   (host only: the orbital-form integral input of a
+!        closed-shell tiling through tce_t_v2_alpha_offset and
+!        tce_t_v2_from_intorb, printed as exact checksums)
 program tce_t_fdemo
   use iso_c_binding
   use tce_t_mi355x
@@ -27,6 +30,13 @@ program tce_t_fdemo
   integer(c_long_long), parameter :: seed = 1234_c_long_long
 
   no = 8; nv = 32; to = 8; tv = 16
+  if (command_argument_count() == 1) then
+    call get_command_argument(1, arg)
+    if (trim(arg) == 'intorb') then
+      call intorb_demo()
+      stop
+    end if
+  end if
   if (command_argument_count() >= 4) then
     call get_command_argument(1, arg); read(arg, *) no
     call get_command_argument(2, arg); read(arg, *) nv
@@ -129,4 +139,57 @@ program tce_t_fdemo
   end do
   write(*, '(A,ES24.16,A,ES24.16)') 'tiles E[T]=', ta, ' E(T)=', tb
   rc = tce_t_job_destroy(job)
+
+contains
+
+  ! Closed-shell tiling as tce_tile.F lays it out with intorb: alpha holes
+  ! (3, 2), beta holes, alpha particles (4, 5, 3), beta particles; k_b2am
+  ! maps each tile to its alpha tile.  The orbital-form array holds exact
+  ! binary fractions, so the checksums of the converted blocks are exact.
+  subroutine intorb_demo()
+    integer(c_long), parameter :: noa = 2, nva = 3, nt = 10, idiv2e = 2
+    integer(c_long), target :: r(nt), o(nt), sp(nt), sy(nt), al(nt), b2am(nt)
+    integer(c_long), target :: spa(noa + nva), sya(noa + nva), ra(noa + nva)
+    real(c_double), target :: ev(34)   ! one energy per orbital (sum of r)
+    integer(c_long), allocatable :: hv2(:)
+    integer(c_long) :: ha(6 * (idiv2e + 2) + 1), words, osize, nv2, vsize, k
+    real(c_double), allocatable :: orb(:), v(:)
+    type(tce_t_tiles) :: ti
+    type(tce_t_intorb) :: io
+    real(c_double) :: total, check
+    integer(c_int) :: rc2
+    r = [3, 2, 3, 2, 4, 5, 3, 4, 5, 3]
+    sp = [1, 1, 2, 2, 1, 1, 1, 2, 2, 2]
+    sy = 0
+    al = [1, 2, 1, 2, 5, 6, 7, 5, 6, 7]
+    b2am = [1, 2, 1, 2, 3, 4, 5, 3, 4, 5]
+    o(1) = 0
+    do k = 2, nt
+      o(k) = o(k - 1) + r(k - 1)
+    end do
+    do k = 1, 34
+      ev(k) = -2.0d0 + 0.125d0 * k
+    end do
+    spa = 1; sya = 0; ra = [3, 2, 4, 5, 3]
+    ti = tce_t_tiles(4_c_long, 6_c_long, c_loc(sp), c_loc(sy), c_loc(r), c_loc(o), c_loc(al), &
+                     c_loc(ev), 1)
+    io = tce_t_intorb(noa, nva, c_loc(b2am), c_loc(spa), c_loc(sya), c_loc(ra), 0_c_long)
+    words = tce_t_v2_alpha_offset(io, idiv2e, ha, osize)
+    if (words < 0) stop 11
+    allocate(hv2(2 * nt**4 + 1))
+    nv2 = tce_t_v2_offset(ti, hv2, vsize)
+    allocate(orb(osize), v(vsize))
+    do k = 1, osize
+      orb(k) = dble(mod(k * 7919_c_long, 1000_c_long)) / 1024.0d0
+    end do
+    rc2 = tce_t_v2_from_intorb(ti, io, orb, ha, hv2, v)
+    if (rc2 /= TCE_T_OK) stop 12
+    total = 0.0d0; check = 0.0d0
+    do k = 1, vsize
+      total = total + v(k)
+      check = check + v(k) * dble(mod(k, 17_c_long))
+    end do
+    write(*, '(a,i0,a,i0,a,es26.17e3,a,es26.17e3)') 'intorb blocks=', nv2, ' size=', vsize, &
+        ' sum=', total, ' check=', check
+  end subroutine intorb_demo
 end program tce_t_fdemo

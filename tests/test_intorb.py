@@ -292,3 +292,37 @@ def test_bad_inputs_are_rejected():
     badh[1] = (T.noab + T.nvab) ** 4  # a key past the last tile quadruple
     with pytest.raises(RuntimeError):
         tl.v2_from_intorb(T, io, v2orb, h, badh, T.sizes()["v2"])
+
+
+def test_fortran_caller_converts_the_same_blocks():
+    """tce_t_v2_alpha_offset and tce_t_v2_from_intorb through the Fortran
+    module (nwchem_amd/fortran, `tce_t_fdemo intorb`): a closed-shell tiling
+    with alpha holes (3, 2) and particles (4, 5, 3), orbital-form values
+    mod(7919 k, 1000) / 1024 (exact, so the checksums are exact), against the
+    Python binding on the same inputs."""
+    import re
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "nwchem_amd", "fortran", "tce_t_fdemo")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran demo not built")
+    out = subprocess.run([exe, "intorb"], check=True, capture_output=True, text=True,
+                         timeout=60).stdout
+    m = re.search(r"intorb blocks=(\d+) size=(\d+) sum=\s*(\S+) check=\s*(\S+)", out)
+    assert m, out
+    r = [3, 2, 3, 2, 4, 5, 3, 4, 5, 3]
+    T = tl.TileModel(range=np.array(r), noab=4, evl=-2.0 + 0.125 * np.arange(1, 35),
+                     spin=np.array([1, 1, 2, 2, 1, 1, 1, 2, 2, 2]), sym=np.zeros(10, int),
+                     alpha=np.array([1, 2, 1, 2, 5, 6, 7, 5, 6, 7]), restricted=True)
+    io = tl.Intorb(noa=2, nva=3, b2am=[1, 2, 1, 2, 3, 4, 5, 3, 4, 5], spin_alpha=[1] * 5,
+                   sym_alpha=[0] * 5, range_alpha=[3, 2, 4, 5, 3])
+    h, osize = io.alpha_offset(2)
+    k = np.arange(1, osize + 1, dtype=np.int64)
+    orb = ((k * 7919) % 1000).astype(np.float64) / 1024.0
+    v2h = T.v2_hash()
+    vsize = T.sizes()["v2"]
+    v = tl.v2_from_intorb(T, io, orb, h, v2h, vsize)
+    assert int(m.group(1)) == int(v2h[0]) and int(m.group(2)) == vsize
+    w = (np.arange(1, vsize + 1) % 17).astype(np.float64)
+    assert float(m.group(3)) == float(np.sum(v)) and float(m.group(4)) == float(np.sum(v * w))
