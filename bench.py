@@ -315,6 +315,73 @@ def bench_batch(njobs: int, seed: int, steps: int, modes=(0, 2, 3), no_range=(6,
     return out
 
 
+def lambda_inputs(tm, seed: int):
+    """Synthetic Lambda inputs of the job's tiles: the y1/y2 blocks of a
+    second seeded amplitude set, laid out as tce_y1_offset / tce_y2_offset
+    keep them ([h][p], [h][h'][p][p']), and a seeded one-electron operator
+    over the tce_mo1e_offset table."""
+    from nwchem_amd import tiles as tl
+
+    b = tl.synth_blocks(tm, seed)
+    noab, r = tm.noab, tm.range
+    nvab = len(r) - noab
+    N = noab + nvab
+
+    def table(h):
+        n = int(h[0])
+        return [(int(h[1 + i]), int(h[1 + n + i])) for i in range(n)]
+
+    t1 = dict(table(b.t1h))
+    t2 = dict(table(b.t2h))
+    y1h, y2h, f1h = tm.y1_hash(), tm.y2_hash(), tm.f1_hash()
+    y1 = np.zeros(b.t1.size)
+    for key, off in table(y1h):
+        h, p = key // nvab + 1, key % nvab + noab + 1
+        rh, rp = int(r[h - 1]), int(r[p - 1])
+        to = t1[h - 1 + noab * (p - noab - 1)]
+        y1[off:off + rh * rp] = b.t1[to:to + rh * rp].reshape(rp, rh).T.ravel()
+    y2 = np.zeros(b.t2.size)
+    for key, off in table(y2h):
+        p2 = key % nvab + noab + 1
+        q = key // nvab
+        p1 = q % nvab + noab + 1
+        q //= nvab
+        h4, h3 = q % noab + 1, q // noab + 1
+        shp = tuple(int(r[x - 1]) for x in (p1, p2, h3, h4))
+        n = int(np.prod(shp))
+        to = t2[h4 - 1 + noab * (h3 - 1 + noab * (p2 - noab - 1 + nvab * (p1 - noab - 1)))]
+        y2[off:off + n] = b.t2[to:to + n].reshape(shp).transpose(2, 3, 0, 1).ravel()
+    nf = sum(int(r[k // N] * r[k % N]) for k, _ in table(f1h))
+    f1 = np.random.default_rng(seed).uniform(-0.05, 0.05, max(nf, 1))
+    return b, y1, y1h, y2, y2h, f1, f1h
+
+
+def bench_lambda(tm, blocks, seed: int, F_ref: float):
+    """tce_t_lambda_ccsd_t over every task of the job (both pairings'
+    work is the same; the orbital one is timed): two tile exports per
+    task (right t3d, left y3s/y3d: the reference tiles' work each) and the
+    element-wise energy kernel."""
+    from nwchem_amd import TriplesJob
+    from nwchem_amd.job import LAMBDA_ORBITAL, lambda_ccsd_t
+
+    t0 = time.perf_counter()
+    _, y1, y1h, y2, y2h, f1, f1h = lambda_inputs(tm, seed)
+    t_in = time.perf_counter() - t0
+    with TriplesJob(tm) as right, TriplesJob(tm) as left:
+        right.upload(blocks)
+        left.upload_lambda(y1, y1h, y2, y2h, blocks.v2, blocks.v2h)
+        t1 = time.perf_counter()
+        e1, e2, te = lambda_ccsd_t(right, left, f1, f1h, LAMBDA_ORBITAL)
+        wall = time.perf_counter() - t1
+    out = {"tasks": int(te.shape[0]), "seconds": wall, "energy1": e1, "energy2": e2,
+           "tile_flops": 2.0 * F_ref, "tflops": 2.0 * F_ref / wall / 1e12,
+           "inputs_seconds": t_in}
+    print(f"[bench] lambda-ccsd(t): {wall:.2f} s for {out['tasks']} tasks "
+          f"({out['tflops']:.1f} TF/s of tile work), energy1={e1:.12e} energy2={e2:.12e}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
                 rates: dict | None = None) -> dict:
     """Serving: a Poisson stream (rate lam) of requests for a pool of seeded
@@ -427,6 +494,8 @@ def main() -> int:
                     help="also time the job with the reference tiles as kernel tasks (0: skip)")
     ap.add_argument("--task-tiles", type=int, default=1,
                     help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
+    ap.add_argument("--lambda", dest="lam", type=int, default=0,
+                    help="also time tce_t_lambda_ccsd_t over the whole job (1) or skip it (0)")
     ap.add_argument("--modes", default="fp32,bf16x3,bf16x6",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
@@ -529,6 +598,7 @@ def main() -> int:
               f"{16.0 * sum(elems) / tw / 1e9:.1f} GB/s of tiles to the host)",
               file=sys.stderr, flush=True)
     job.close()
+    lam = bench_lambda(tm, blocks, args.seed + 1, F) if args.lam else None
     reftile = None
     if args.reference_tiling:
         # the same job with the reference's tiles as kernel tasks: the work the
@@ -613,6 +683,7 @@ def main() -> int:
         "config1": cfg1,
         "precision_modes": modes or None,
         "task_tiles": tiles_out,
+        "lambda_ccsd_t": lam,
         "batch": batch,
         "batch_small_jobs": batch_small,
         "serving": serve,

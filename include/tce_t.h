@@ -284,6 +284,41 @@ int tce_t_job_triples_tiles_6dts(tce_t_job *job, int precision, long h1b, long h
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);
+
+/* ---- Lambda-CCSD(T) (lambda_ccsd_t.F) ---------------------------------- */
+/* Upload the Lambda amplitudes as a job's amplitudes: d_y1/k_l1_offset,
+ * d_y2/k_l2_offset of lambda_ccsd_t.F:1-3 (host copies; the block tables of
+ * tce_y1_offset.F and tce_y2_offset.F, blocks y(h p) as [h][p] and
+ * y(h h' p p') as [h][h'][p][p']) and the same d_v2/k_v2_offset as the
+ * right-hand job.  Each block is placed where tce_t_job_upload expects the
+ * t1/t2 block of the same tiles, then the job is uploaded.  Such a job's
+ * singles tiles are P(9) y(h4 p1) v(h5 h6 p2 p3) and its doubles tiles the
+ * two v2 terms of lambda_ccsd_t_left.F:7-10 (real orbitals: the (T)
+ * orientation of each integral block serves).  TCE_T_ERR_ARG if a y table
+ * lacks a block the t tables hold. */
+int tce_t_job_upload_lambda(tce_t_job *job, const double *y1, const long *k_y1_offset,
+                            const double *y2, const long *k_y2_offset, const double *v2,
+                            const long *k_v2_offset);
+/* Element pairing of tce_t_lambda_ccsd_t: the t3 and y3 elements of the
+ * same orbitals, or element i of tdoubles (order [p4][p5][p6][h1][h2][h3])
+ * with element i of the left tiles (order [h1][h2][h3][p4][p5][p6]), as the
+ * energy loop of lambda_ccsd_t.F:156-178 pairs them.  The two agree when
+ * every tile holds one orbital. */
+#define TCE_T_LAMBDA_ORBITAL 0
+#define TCE_T_LAMBDA_REFERENCE_ORDER 1
+/* Replaces lambda_ccsd_t(..., energy1, energy2) (lambda_ccsd_t.F:1-214):
+ * right holds t1/t2/v2 (tce_t_job_upload), left the Lambda amplitudes
+ * (tce_t_job_upload_lambda) over the same tiles.  f1/k_f1_offset: the
+ * one-electron operator d_f1/k_f1_offset (block table of tce_mo1e_offset.F),
+ * whose hole-particle blocks enter P(9) y(h4 h5 p1 p2) f(h6 p3); both NULL
+ * drops that term.  Per task of the (T) loop, in `precision`:
+ *   energy1 += factor * tdoubles * ydoubles / D
+ *   energy2 += factor * tdoubles * (ysingles + ydoubles) / D
+ * task_energies (2 * ntasks doubles, or NULL) receives {energy1, energy2}
+ * of each task in loop order.  Synchronous. */
+int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const double *f1,
+                        const long *k_f1_offset, int pairing, int precision, double *energy1,
+                        double *energy2, double *task_energies, long ntasks);
 /* Host packer of tce_t_job_upload, for inspection and tests: the dense tensors
  * the job places in HBM, built from the same blocks and hash tables.  With
  * Ep = no. of virtuals and Mp = no. of occupieds rounded up to a multiple of 8:
@@ -401,6 +436,11 @@ long tce_t_tile_ranges(long n, long tilesize, long *ranges);
 long tce_t_t1_offset(const tce_t_tiles *tiles, long *hash, long *size);
 long tce_t_t2_offset(const tce_t_tiles *tiles, long *hash, long *size);
 long tce_t_v2_offset(const tce_t_tiles *tiles, long *hash, long *size);
+/* tce_y1_offset.F, tce_y2_offset.F (Lambda amplitudes, irrep 0) and
+ * tce_mo1e_offset.F (one-electron operator, irrep 0), same conventions. */
+long tce_t_y1_offset(const tce_t_tiles *tiles, long *hash, long *size);
+long tce_t_y2_offset(const tce_t_tiles *tiles, long *hash, long *size);
+long tce_t_f1_offset(const tce_t_tiles *tiles, long *hash, long *size);
 /* tce_hash.F:271-316 -- offset of key, or -1. */
 long tce_t_hash(const long *hash, long key);
 

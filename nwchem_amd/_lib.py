@@ -97,6 +97,9 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_run_restart.argtypes = [C.c_void_p, C.c_int, LP, DP, C.c_long, DP]
     L.tce_t_job_triples_tiles.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 6 + [DP, DP]
     L.tce_t_job_triples_tiles_6dts.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 12 + [DP, DP]
+    L.tce_t_job_upload_lambda.argtypes = [C.c_void_p, DP, LP, DP, LP, DP, LP]
+    L.tce_t_lambda_ccsd_t.argtypes = [C.c_void_p, C.c_void_p, DP, LP, C.c_int, C.c_int, DP, DP,
+                                      DP, C.c_long]
     L.tce_t_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.tce_t_job_kernel.argtypes = [C.c_void_p]
     L.tce_t_job_kernel.restype = C.c_int
@@ -107,7 +110,8 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_pack_dense.argtypes = [C.POINTER(TilesC), DP, LP, DP, LP, DP, LP] + [DP] * 6
     L.tce_t_tile_ranges.argtypes = [C.c_long, C.c_long, LP]
     L.tce_t_tile_ranges.restype = C.c_long
-    for f in ("tce_t_t1_offset", "tce_t_t2_offset", "tce_t_v2_offset"):
+    for f in ("tce_t_t1_offset", "tce_t_t2_offset", "tce_t_v2_offset", "tce_t_y1_offset",
+              "tce_t_y2_offset", "tce_t_f1_offset"):
         getattr(L, f).argtypes = [C.POINTER(TilesC), LP, LP]
         getattr(L, f).restype = C.c_long
     L.tce_t_hash.argtypes = [LP, C.c_long]

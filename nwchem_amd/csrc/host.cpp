@@ -103,6 +103,48 @@ template <class F> static void each_v2_block(const Tiles &T, F &&f) {
         }
 }
 
+// De-excitation amplitudes y(h p) (tce_y1_offset.F:39-58): hole tile outer,
+// key p - noab - 1 + nvab (h - 1), block [h][p].
+template <class F> static void each_y1_block(const Tiles &T, F &&f) {
+  long noab = T.noab, nvab = T.nvab;
+  for (long h4 = 1; h4 <= noab; ++h4)
+    for (long p1 = noab + 1; p1 <= noab + nvab; ++p1) {
+      if (T.spn(h4) != T.spn(p1)) continue;
+      if ((T.sy(h4) ^ T.sy(p1)) != 0) continue;
+      if (T.restricted && T.spn(h4) + T.spn(p1) == 4) continue;
+      f(p1 - noab - 1 + nvab * (h4 - 1), h4, p1, 0, 0);
+    }
+}
+
+// y(h7 h8 p1 p2), h7 <= h8, p1 <= p2 (tce_y2_offset.F:43-62), block
+// [h7][h8][p1][p2].
+template <class F> static void each_y2_block(const Tiles &T, F &&f) {
+  long noab = T.noab, nvab = T.nvab;
+  for (long h7 = 1; h7 <= noab; ++h7)
+    for (long h8 = h7; h8 <= noab; ++h8)
+      for (long p1 = noab + 1; p1 <= noab + nvab; ++p1)
+        for (long p2 = p1; p2 <= noab + nvab; ++p2) {
+          if (T.spn(h7) + T.spn(h8) != T.spn(p1) + T.spn(p2)) continue;
+          if ((T.sy(h7) ^ T.sy(h8) ^ T.sy(p1) ^ T.sy(p2)) != 0) continue;
+          if (T.restricted && T.spn(h7) + T.spn(h8) + T.spn(p1) + T.spn(p2) == 8) continue;
+          f(p2 - noab - 1 + nvab * (p1 - noab - 1 + nvab * (h8 - 1 + noab * (h7 - 1))), h7, h8,
+            p1, p2);
+        }
+}
+
+// One-electron operator f(g6 g5) over every tile pair (tce_mo1e_offset.F:
+// 35-50), key g5 - 1 + (noab + nvab)(g6 - 1), block [g6][g5].
+template <class F> static void each_f1_block(const Tiles &T, F &&f) {
+  long N = T.noab + T.nvab;
+  for (long g6 = 1; g6 <= N; ++g6)
+    for (long g5 = 1; g5 <= N; ++g5) {
+      if (T.spn(g6) != T.spn(g5)) continue;
+      if ((T.sy(g6) ^ T.sy(g5)) != 0) continue;
+      if (T.restricted && T.spn(g6) + T.spn(g5) == 4) continue;
+      f(g5 - 1 + N * (g6 - 1), g6, g5, 0, 0);
+    }
+}
+
 template <class Each> static long build_hash(Each each, long *hash, long *size,
                                              const Tiles &T, int nidx) {
   std::vector<long> keys, sizes;
@@ -136,6 +178,16 @@ long t2_offset(const Tiles &T, long *hash, long *size) {
 }
 long v2_offset(const Tiles &T, long *hash, long *size) {
   return build_hash([&](auto &&g) { each_v2_block(T, g); }, hash, size, T, 4);
+}
+
+long y1_offset(const Tiles &T, long *hash, long *size) {
+  return build_hash([&](auto &&g) { each_y1_block(T, g); }, hash, size, T, 2);
+}
+long y2_offset(const Tiles &T, long *hash, long *size) {
+  return build_hash([&](auto &&g) { each_y2_block(T, g); }, hash, size, T, 4);
+}
+long f1_offset(const Tiles &T, long *hash, long *size) {
+  return build_hash([&](auto &&g) { each_f1_block(T, g); }, hash, size, T, 2);
 }
 
 long hash_lookup(const long *hash, long key) {
@@ -393,6 +445,18 @@ extern "C" long tce_t_t2_offset(const tce_t_tiles *t, long *hash, long *size) {
 extern "C" long tce_t_v2_offset(const tce_t_tiles *t, long *hash, long *size) {
   WITH_TILES(t, -1);
   return v2_offset(T, hash, size);
+}
+extern "C" long tce_t_y1_offset(const tce_t_tiles *t, long *hash, long *size) {
+  WITH_TILES(t, -1);
+  return y1_offset(T, hash, size);
+}
+extern "C" long tce_t_y2_offset(const tce_t_tiles *t, long *hash, long *size) {
+  WITH_TILES(t, -1);
+  return y2_offset(T, hash, size);
+}
+extern "C" long tce_t_f1_offset(const tce_t_tiles *t, long *hash, long *size) {
+  WITH_TILES(t, -1);
+  return f1_offset(T, hash, size);
 }
 extern "C" long tce_t_hash(const long *hash, long key) { return hash ? hash_lookup(hash, key) : -1; }
 

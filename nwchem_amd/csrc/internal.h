@@ -40,6 +40,9 @@ long tile_ranges(long n, long tilesize, long *ranges);
 long t1_offset(const Tiles &T, long *hash, long *size);
 long t2_offset(const Tiles &T, long *hash, long *size);
 long v2_offset(const Tiles &T, long *hash, long *size);
+long y1_offset(const Tiles &T, long *hash, long *size);
+long y2_offset(const Tiles &T, long *hash, long *size);
+long f1_offset(const Tiles &T, long *hash, long *size);
 long hash_lookup(const long *hash, long key);
 
 void philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

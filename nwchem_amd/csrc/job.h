@@ -70,6 +70,14 @@ struct Job {
 
 // Records a HIP failure (tce_t_last_error) and returns false.
 bool hip_ok(hipError_t e, const char *what);
+// One task's (or one p4 x p5 slice's) t3 tiles of an uploaded job computed
+// on the device, without the copy to the host: doubles at *tiles, then
+// singles (n doubles later, n = the tile's elements, from d's ranges), in
+// the layout of tce_t_job_triples_tiles; queued on J.stream.  d receives
+// the task entry (orbital offsets and ranges of the slice).
+int tile_export_device(Job &J, int precision, long h1b, long h2b, long h3b, long p4b, long p5b,
+                       long p6b, long i0, long ni, long j0, long nj, long ld5, TaskDev &d,
+                       double **tiles);
 #define HIPCHK(x, what)                             \
   do {                                              \
     if (!::tce_t::hip_ok((x), what)) return TCE_T_ERR_HIP; \

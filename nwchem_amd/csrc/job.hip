@@ -1191,9 +1191,9 @@ extern "C" int tce_t_job_run_restart(tce_t_job *job, int precision, long *begin,
 // arrays; ld5 = the p5 extent of the caller's layout (>= nj): element
 // (p4', p5', p6, h1, h2, h3) of the slice at
 // ((((p4' ld5 + p5') r6 + p6) r1 + h1) r2 + h2) r3 + h3.
-static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, long p4b, long p5b,
-                         long p6b, long i0, long ni, long j0, long nj, long ld5, double *singles,
-                         double *doubles) {
+int tce_t::tile_export_device(Job &J, int precision, long h1b, long h2b, long h3b, long p4b,
+                              long p5b, long p6b, long i0, long ni, long j0, long nj, long ld5,
+                              TaskDev &d, double **tiles) {
   if (!J.uploaded) {
     set_error("tce_t_job_triples_tiles before tce_t_job_upload");
     return TCE_T_ERR_STATE;
@@ -1219,7 +1219,7 @@ static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, lo
     set_error("tile slice outside the task's tiles");
     return TCE_T_ERR_ARG;
   }
-  TaskDev d;
+  d = TaskDev();
   d.op4 = (int)(T.ofs(p4b) - T.no + i0);
   d.op5 = (int)(T.ofs(p5b) - T.no + j0);
   d.op6 = (int)(T.ofs(p6b) - T.no);
@@ -1287,6 +1287,21 @@ static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, lo
     set_error("tile kernel launch failed");
     return TCE_T_ERR_HIP;
   }
+  *tiles = M.f.partials;
+  return TCE_T_OK;
+}
+
+static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, long p4b, long p5b,
+                         long p6b, long i0, long ni, long j0, long nj, long ld5, double *singles,
+                         double *doubles) {
+  TaskDev d;
+  double *dev = nullptr;
+  {
+    const int rc = tile_export_device(J, precision, h1b, h2b, h3b, p4b, p5b, p6b, i0, ni, j0, nj,
+                                      ld5, d, &dev);
+    if (rc != TCE_T_OK) return rc;
+  }
+  const long n = (long)d.r1 * d.r2 * d.r3 * d.r4 * d.r5 * d.r6;
   // Back to the host in chunks through two pinned buffers: the copy of
   // chunk k+1 runs while host threads add chunk k into the caller's arrays.
   // Device layout: doubles [0, n), singles [n, 2n); a NULL array's half is
@@ -1307,7 +1322,7 @@ static int triples_tiles(Job &J, int precision, long h1b, long h2b, long h3b, lo
   const long nch = (last - first + CH - 1) / CH;
   auto issue = [&](long k) {
     const long off = first + k * CH, len = std::min(CH, last - off);
-    return hip_ok(hipMemcpyAsync(stage + (k & 1) * CH, M.f.partials + off, len * sizeof(double),
+    return hip_ok(hipMemcpyAsync(stage + (k & 1) * CH, dev + off, len * sizeof(double),
                                  hipMemcpyDeviceToHost, J.stream),
                   "D2H") &&
            hip_ok(hipEventRecord(ev[k & 1], J.stream), "event");

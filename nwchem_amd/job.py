@@ -73,6 +73,17 @@ class TriplesJob:
                                  b.v2.ctypes.data_as(DP), b.v2h.ctypes.data_as(LP)),
               "tce_t_job_upload")
 
+    def upload_lambda(self, y1: np.ndarray, y1h: np.ndarray, y2: np.ndarray, y2h: np.ndarray,
+                      v2: np.ndarray, v2h: np.ndarray) -> None:
+        """The Lambda amplitudes (tce_y1_offset / tce_y2_offset blocks) as
+        the job's amplitudes, with the right-hand job's integrals: the
+        left-hand job of lambda_ccsd_t."""
+        arrs = [np.ascontiguousarray(a, dtype=dt) for a, dt in
+                ((y1, np.float64), (y1h, np.int64), (y2, np.float64), (y2h, np.int64),
+                 (v2, np.float64), (v2h, np.int64))]
+        ptrs = [a.ctypes.data_as(DP if a.dtype == np.float64 else LP) for a in arrs]
+        check(_lib.lib().tce_t_job_upload_lambda(self._h, *ptrs), "tce_t_job_upload_lambda")
+
     def run(self, precision: int = FP64) -> tuple[float, float]:
         e1, e2 = C.c_double(0.0), C.c_double(0.0)
         check(_lib.lib().tce_t_job_run(self._h, precision, C.byref(e1), C.byref(e2)), "tce_t_job_run")
@@ -205,6 +216,32 @@ class TriplesJob:
             self.close()
         except Exception:
             pass
+
+
+LAMBDA_ORBITAL = 0          # t3 and y3 elements of the same orbitals
+LAMBDA_REFERENCE_ORDER = 1  # element i of tdoubles with element i of the left tiles
+
+
+def lambda_ccsd_t(right: TriplesJob, left: TriplesJob, f1: np.ndarray | None = None,
+                  f1h: np.ndarray | None = None, pairing: int = LAMBDA_ORBITAL,
+                  precision: int = FP64) -> tuple[float, float, np.ndarray]:
+    """lambda_ccsd_t.F: (energy1, energy2, per-task energies) from the
+    right-hand job (t1, t2, v2) and the left-hand job (upload_lambda).
+    f1/f1h: the one-electron operator (tce_mo1e_offset blocks) or None to
+    drop the Fock term."""
+    e1, e2 = C.c_double(0.0), C.c_double(0.0)
+    te = np.zeros((right.ntasks, 2))
+    if (f1 is None) != (f1h is None):
+        raise ValueError("f1 and f1h go together")
+    fp = fh = None
+    if f1 is not None:
+        f1 = np.ascontiguousarray(f1, dtype=np.float64)
+        f1h = np.ascontiguousarray(f1h, dtype=np.int64)
+        fp, fh = f1.ctypes.data_as(DP), f1h.ctypes.data_as(LP)
+    check(_lib.lib().tce_t_lambda_ccsd_t(right._h, left._h, fp, fh, int(pairing), int(precision),
+                                         C.byref(e1), C.byref(e2), te.ctypes.data_as(DP),
+                                         right.ntasks), "tce_t_lambda_ccsd_t")
+    return e1.value, e2.value, te
 
 
 FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed",

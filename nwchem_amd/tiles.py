@@ -113,6 +113,18 @@ class TileModel:
     def v2_hash(self) -> np.ndarray:
         return self._hash("tce_t_v2_offset")
 
+    def y1_hash(self) -> np.ndarray:
+        """tce_y1_offset.F: Lambda singles y(h p), blocks [h][p]."""
+        return self._hash("tce_t_y1_offset")
+
+    def y2_hash(self) -> np.ndarray:
+        """tce_y2_offset.F: Lambda doubles y(h h' p p'), blocks [h][h'][p][p']."""
+        return self._hash("tce_t_y2_offset")
+
+    def f1_hash(self) -> np.ndarray:
+        """tce_mo1e_offset.F: one-electron operator f(g g'), blocks [g][g']."""
+        return self._hash("tce_t_f1_offset")
+
     @staticmethod
     def block_size(hash_: np.ndarray, tiles: "TileModel", kind: str) -> int:
         f = {"t1": "tce_t_t1_offset", "t2": "tce_t_t2_offset", "v2": "tce_t_v2_offset"}[kind]

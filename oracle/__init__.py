@@ -73,7 +73,8 @@ def lib() -> C.CDLL:
     if not getattr(L, "_declared", False):
         L.ora_tile_ranges.argtypes = [C.c_long, C.c_long, LP]
         L.ora_tile_ranges.restype = C.c_long
-        for f in ("ora_t1_offset", "ora_t2_offset"):
+        for f in ("ora_t1_offset", "ora_t2_offset", "ora_y1_offset", "ora_y2_offset",
+                  "ora_f1_offset"):
             getattr(L, f).argtypes = [C.POINTER(OraTiles), LP, LP]
             getattr(L, f).restype = C.c_long
         L.ora_v2_offset.argtypes = [C.POINTER(OraTiles), C.c_int, LP, LP]
@@ -110,6 +111,8 @@ def lib() -> C.CDLL:
         L.ora_v2orb_pack.restype = None
         L.ora_v2_from_intorb.argtypes = [C.POINTER(OraTiles), C.POINTER(OraIntorb), DP, LP, LP, DP]
         L.ora_v2_from_intorb.restype = C.c_int
+        L.ora_sortacc_6.argtypes = [DP, DP] + [C.c_long] * 12 + [C.c_double]
+        L.ora_sortacc_6.restype = None
         L._declared = True
     return L
 
@@ -328,3 +331,209 @@ def v2_from_intorb(tiles: Tiles, io: Intorb, v2orb, alpha_offset, v2h, size: int
     if rc != 0:
         raise KeyError("orbital-form key not found")
     return out
+
+
+# ---- Lambda-CCSD(T) (lambda_ccsd_t.F) -----------------------------------
+
+def _table(h):
+    n = int(h[0])
+    return [(int(h[1 + i]), int(h[1 + n + i])) for i in range(n)]
+
+
+def lambda_blocks(tiles: Tiles, y1, y1h, y2, y2h):
+    """The Lambda amplitudes placed as t1/t2 blocks of the same tiles: y(h p)
+    blocks [h][p] (tce_y1_offset.F) -> [p][h] under the t1 key, y(h h' p p')
+    blocks [h][h'][p][p'] (tce_y2_offset.F) -> [p][p'][h][h'] under the t2
+    key.  Returns (t1, t1h, t2, t2h)."""
+    L = lib()
+    noab, nvab = tiles.noab, tiles.nvab
+    r = tiles.range
+    t1h, s1 = tiles.hash("t1")
+    t2h, s2 = tiles.hash("t2")
+    y1 = np.ascontiguousarray(y1, np.float64)
+    y2 = np.ascontiguousarray(y2, np.float64)
+    y1h = np.ascontiguousarray(y1h, np.int64)
+    y2h = np.ascontiguousarray(y2h, np.int64)
+    t1 = np.zeros(max(s1, 1))
+    t2 = np.zeros(max(s2, 1))
+    for key, off in _table(t1h):
+        h, p = key % noab + 1, key // noab + noab + 1
+        yo = L.ora_hash(_lp(y1h), p - noab - 1 + nvab * (h - 1))
+        if yo < 0:
+            raise ValueError("y1 lacks a block of the t1 table")
+        rh, rp = int(r[h - 1]), int(r[p - 1])
+        t1[off:off + rh * rp] = y1[yo:yo + rh * rp].reshape(rh, rp).T.ravel()
+    for key, off in _table(t2h):
+        h4 = key % noab + 1
+        q = key // noab
+        h3 = q % noab + 1
+        q //= noab
+        p2, p1 = q % nvab + noab + 1, q // nvab + noab + 1
+        yo = L.ora_hash(_lp(y2h), p2 - noab - 1 + nvab * (p1 - noab - 1 + nvab * (h4 - 1 + noab * (h3 - 1))))
+        if yo < 0:
+            raise ValueError("y2 lacks a block of the t2 table")
+        r1, r2, r3, r4 = (int(r[b - 1]) for b in (p1, p2, h3, h4))
+        n = r1 * r2 * r3 * r4
+        t2[off:off + n] = y2[yo:yo + n].reshape(r3, r4, r1, r2).transpose(2, 3, 0, 1).ravel()
+    return t1, t1h, t2, t2h
+
+
+def fock_hole_particle(tiles: Tiles, f1, f1h):
+    """F[h][p] over every orbital from the hole-particle blocks of the
+    one-electron operator (tce_mo1e_offset.F blocks [g6][g5]; the f(h6 p3)
+    reads of lambda_ccsd_t_left.F:397-400, all-beta pairs from their alpha
+    image in a restricted run, tce_restricted.F:1-32)."""
+    L = lib()
+    noab = tiles.noab
+    N = noab + tiles.nvab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    nv = int(r[noab:].sum())
+    F = np.zeros((no, nv))
+    f1 = np.ascontiguousarray(f1, np.float64)
+    f1h = np.ascontiguousarray(f1h, np.int64)
+    for hb in range(1, noab + 1):
+        for pb in range(noab + 1, N + 1):
+            if tiles.spin[hb - 1] != tiles.spin[pb - 1] or (tiles.sym[hb - 1] ^ tiles.sym[pb - 1]):
+                continue
+            h, p = hb, pb
+            if tiles.restricted and tiles.spin[hb - 1] + tiles.spin[pb - 1] == 4:
+                h, p = int(tiles.alpha[hb - 1]), int(tiles.alpha[pb - 1])
+            off = L.ora_hash(_lp(f1h), p - 1 + N * (h - 1))
+            if off < 0:
+                raise ValueError("f1 lacks a hole-particle block")
+            rh, rp = int(r[hb - 1]), int(r[pb - 1])
+            oh, op = int(o[hb - 1]), int(o[pb - 1]) - no
+            F[oh:oh + rh, op:op + rp] = f1[off:off + rh * rp].reshape(rh, rp)
+    return F
+
+
+def _dense_y2(tiles: Tiles, t2, t2h):
+    """Y[a, b, i, j] over every orbital from t2-layout blocks (canonical
+    p1 <= p2, h3 <= h4; the other orderings by antisymmetry, all-beta blocks
+    from their alpha image in a restricted run)."""
+    L = lib()
+    noab, nvab = tiles.noab, tiles.nvab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    nv = int(r[noab:].sum())
+    Y = np.zeros((nv, nv, no, no))
+    sp, sy = tiles.spin, tiles.sym
+
+    def sl(b):
+        s = int(o[b - 1]) - (no if b > noab else 0)
+        return slice(s, s + int(r[b - 1]))
+
+    for p1 in range(noab + 1, noab + nvab + 1):
+        for p2 in range(p1, noab + nvab + 1):
+            for h3 in range(1, noab + 1):
+                for h4 in range(h3, noab + 1):
+                    if sp[p1 - 1] + sp[p2 - 1] != sp[h3 - 1] + sp[h4 - 1]:
+                        continue
+                    if sy[p1 - 1] ^ sy[p2 - 1] ^ sy[h3 - 1] ^ sy[h4 - 1]:
+                        continue
+                    q = (p1, p2, h3, h4)
+                    if tiles.restricted and sum(int(sp[b - 1]) for b in q) == 8:
+                        q = tuple(int(tiles.alpha[b - 1]) for b in q)
+                    key = q[3] - 1 + noab * (q[2] - 1 + noab * (q[1] - noab - 1 + nvab * (q[0] - noab - 1)))
+                    off = L.ora_hash(_lp(t2h), key)
+                    if off < 0:
+                        raise ValueError("t2 lacks a block")
+                    shp = tuple(int(r[b - 1]) for b in (p1, p2, h3, h4))
+                    blk = t2[off:off + int(np.prod(shp))].reshape(shp)
+                    Y[sl(p1), sl(p2), sl(h3), sl(h4)] = blk
+                    Y[sl(p2), sl(p1), sl(h3), sl(h4)] = -blk.transpose(1, 0, 2, 3)
+                    Y[sl(p1), sl(p2), sl(h4), sl(h3)] = -blk.transpose(0, 1, 3, 2)
+                    Y[sl(p2), sl(p1), sl(h4), sl(h3)] = blk.transpose(1, 0, 3, 2)
+    return Y
+
+
+def _task_list(tiles: Tiles):
+    """Tasks of the (T) loop in order (lambda_ccsd_t.F:57-84, the loop and
+    tests of ccsd_t_gpu.F)."""
+    noab, nvab = tiles.noab, tiles.nvab
+    sp, sy = tiles.spin, tiles.sym
+    out = []
+    for p4 in range(noab + 1, noab + nvab + 1):
+        for p5 in range(p4, noab + nvab + 1):
+            for p6 in range(p5, noab + nvab + 1):
+                for h1 in range(1, noab + 1):
+                    for h2 in range(h1, noab + 1):
+                        for h3 in range(h2, noab + 1):
+                            q = (p4, p5, p6, h1, h2, h3)
+                            if sp[p4 - 1] + sp[p5 - 1] + sp[p6 - 1] != sp[h1 - 1] + sp[h2 - 1] + sp[h3 - 1]:
+                                continue
+                            if tiles.restricted and sum(int(sp[b - 1]) for b in q) > 8:
+                                continue
+                            if sy[p4 - 1] ^ sy[p5 - 1] ^ sy[p6 - 1] ^ sy[h1 - 1] ^ sy[h2 - 1] ^ sy[h3 - 1]:
+                                continue
+                            out.append(q)
+    return out
+
+
+def lambda_ccsd_t(tiles: Tiles, t1, t1h, t2, t2h, y1, y1h, y2, y2h, v2, v2h, f1=None, f1h=None,
+                  pairing: int = 0):
+    """lambda_ccsd_t.F:56-200: (energy1, energy2, per-task energies).  The
+    t3 doubles tile (ccsd_t_doubles) and the left tiles (lambda_ccsd_t_left:
+    the y1 and v2 terms as ora_task_tiles over the Lambda amplitudes placed
+    as t1/t2, the Fock term P(c/ab) P(k/ij) y(a b i j) f(k c) in numpy),
+    then factor * t * y / D summed per task.  pairing 0: elements of the
+    same orbitals; 1: element i of the t3 order [p4][p5][p6][h1][h2][h3]
+    with element i of the left order [h1][h2][h3][p4][p5][p6]
+    (lambda_ccsd_t.F:156-178)."""
+    lt1, lt1h, lt2, lt2h = lambda_blocks(tiles, y1, y1h, y2, y2h)
+    F = None if f1 is None else fock_hole_particle(tiles, f1, f1h)
+    Y = _dense_y2(tiles, lt2, lt2h) if F is not None else None
+    noab = tiles.noab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    ev = tiles.evl
+
+    def orb(b):
+        s = int(o[b - 1]) - (no if b > noab else 0)
+        return np.arange(s, s + int(r[b - 1]))
+
+    tasks = _task_list(tiles)
+    te = np.zeros((len(tasks), 2))
+    for n, q in enumerate(tasks):
+        p4, p5, p6, h1, h2, h3 = q
+        _, td = task_tiles(tiles, t1, t1h, t2, t2h, v2, v2h, q)
+        ys, yd = task_tiles(tiles, lt1, lt1h, lt2, lt2h, v2, v2h, q)
+        A, B, Cc, I, J, K = (orb(b) for b in q)
+        if F is not None:
+            def yy(p, pp, h, hh):
+                return Y[np.ix_(p, pp, h, hh)]
+
+            def ff(h, p):
+                return F[np.ix_(h, p)]
+
+            e = np.einsum
+            yf = (e("abij,kc->abcijk", yy(A, B, I, J), ff(K, Cc))
+                  - e("cbij,ka->abcijk", yy(Cc, B, I, J), ff(K, A))
+                  - e("acij,kb->abcijk", yy(A, Cc, I, J), ff(K, B))
+                  - e("abkj,ic->abcijk", yy(A, B, K, J), ff(I, Cc))
+                  + e("cbkj,ia->abcijk", yy(Cc, B, K, J), ff(I, A))
+                  + e("ackj,ib->abcijk", yy(A, Cc, K, J), ff(I, B))
+                  - e("abik,jc->abcijk", yy(A, B, I, K), ff(J, Cc))
+                  + e("cbik,ja->abcijk", yy(Cc, B, I, K), ff(J, A))
+                  + e("acik,jb->abcijk", yy(A, Cc, I, K), ff(J, B)))
+            yd = yd + yf
+        if pairing == 1:
+            shp = yd.shape
+            yd = yd.transpose(3, 4, 5, 0, 1, 2).reshape(shp)
+            ys = ys.transpose(3, 4, 5, 0, 1, 2).reshape(shp)
+        D = (-ev[no + A][:, None, None, None, None, None] - ev[no + B][None, :, None, None, None, None]
+             - ev[no + Cc][None, None, :, None, None, None] + ev[I][None, None, None, :, None, None]
+             + ev[J][None, None, None, None, :, None] + ev[K][None, None, None, None, None, :])
+        f = 2.0 if tiles.restricted else 1.0
+        if p4 == p5 == p6:
+            f /= 6.0
+        elif p4 == p5 or p5 == p6:
+            f /= 2.0
+        if h1 == h2 == h3:
+            f /= 6.0
+        elif h1 == h2 or h2 == h3:
+            f /= 2.0
+        te[n, 0] = f * np.sum(td * yd / D)
+        te[n, 1] = f * np.sum(td * (ys + yd) / D)
+    return float(te[:, 0].sum()), float(te[:, 1].sum()), te

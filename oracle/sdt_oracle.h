@@ -213,6 +213,16 @@ int ora_get_block_ind_i(const ora_tiles_t *t, const ora_intorb_t *io, const doub
 int ora_v2_from_intorb(const ora_tiles_t *t, const ora_intorb_t *io, const double *v2orb,
                        const long *hash_alpha, const long *v2_hash, double *v2);
 
+/* Lambda-CCSD(T) inputs (lambda_oracle.c): block tables of the Lambda
+ * amplitudes and of the one-electron operator, conventions of
+ * ora_t1_offset; the six-index accumulating sort. */
+long ora_y1_offset(const ora_tiles_t *t, long *hash, long *size);
+long ora_y2_offset(const ora_tiles_t *t, long *hash, long *size);
+long ora_f1_offset(const ora_tiles_t *t, long *hash, long *size);
+void ora_sortacc_6(const double *unsorted, double *sorted, long a, long b, long c, long d,
+                   long e, long f, long i, long j, long k, long l, long m, long n,
+                   double factor);
+
 #ifdef __cplusplus
 }
 #endif

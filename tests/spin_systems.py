@@ -109,6 +109,8 @@ class SpinSystem:
         self.V = make("vvov", (1, 1, -1, -1), [(0, 1)], 0.05)
         self.S = make("vvoo", (1, 1, -1, -1), [(0, 1), (2, 3)], 0.05)
 
+        self._make = make  # later tensors (Lambda inputs) continue the same stream
+
         evl = np.concatenate([self.eo, self.ev])
         self.tiles = tl.TileModel(range=ranges, noab=noab, evl=evl, spin=spin, sym=sym,
                                   alpha=alpha, restricted=restricted)
@@ -212,3 +214,102 @@ class SpinSystem:
 
         return oracle.brute_force(self.no, self.nv, self.T1, self.T2, self.W, self.V, self.S,
                                   self.eo, self.ev)
+
+    # ---- Lambda-CCSD(T) inputs ------------------------------------------
+
+    def lambda_inputs(self):
+        """Dense Lambda amplitudes Y1[i, a] = y(i a), Y2[i, j, a, b] =
+        y(i j a b) and one-electron operator blocks Fhh, Fhp[i, a] = f(i a),
+        Fph, Fpp (not symmetric, so a transposed read shows), plus their TCE
+        blocks at the offsets of the tce_y1/y2/mo1e tables.  Returns a dict
+        with the dense tensors and (y1, y1h, y2, y2h, f1, f1h)."""
+        T = self.tiles
+        mk = self._make
+        d = dict(Y1=mk("ov", (1, -1), [], 0.02),
+                 Y2=mk("oovv", (1, 1, -1, -1), [(0, 1), (2, 3)], 0.03),
+                 Fhh=mk("oo", (1, -1), [], 0.1), Fhp=mk("ov", (1, -1), [], 0.05),
+                 Fph=mk("vo", (1, -1), [], 0.05), Fpp=mk("vv", (1, -1), [], 0.1))
+        noab, nvab = T.noab, T.nvab
+        N = noab + nvab
+        rng_, ofs = T.range, T.offset
+        no = self.no
+
+        def sl(b):
+            o = ofs[b - 1] - (no if b > noab else 0)
+            return slice(int(o), int(o + rng_[b - 1]))
+
+        def table(h):
+            L = int(h[0])
+            return [(int(h[1 + i]), int(h[1 + L + i])) for i in range(L)]
+
+        y1h, y2h, f1h = T.y1_hash(), T.y2_hash(), T.f1_hash()
+        y1_blocks = []
+        for key, off in table(y1h):
+            h, p = key // nvab + 1, key % nvab + noab + 1
+            y1_blocks.append((off, d["Y1"][sl(h), sl(p)].ravel()))
+        y1 = np.zeros(max(sum(b.size for _, b in y1_blocks), 1))
+        for off, b in y1_blocks:
+            y1[off:off + b.size] = b
+        y2_blocks = []
+        for key, off in table(y2h):
+            p2 = key % nvab + noab + 1
+            q = key // nvab
+            p1 = q % nvab + noab + 1
+            q //= nvab
+            h8, h7 = q % noab + 1, q // noab + 1
+            y2_blocks.append((off, d["Y2"][sl(h7), sl(h8), sl(p1), sl(p2)].ravel()))
+        y2 = np.zeros(max(sum(b.size for _, b in y2_blocks), 1))
+        for off, b in y2_blocks:
+            y2[off:off + b.size] = b
+        f_blocks = []
+        for key, off in table(f1h):
+            g5, g6 = key % N + 1, key // N + 1
+            dense = d[("Fh" if g6 <= noab else "Fp") + ("h" if g5 <= noab else "p")]
+            f_blocks.append((off, dense[sl(g6), sl(g5)].ravel()))
+        f1 = np.zeros(max(sum(b.size for _, b in f_blocks), 1))
+        for off, b in f_blocks:
+            f1[off:off + b.size] = b
+        d.update(y1=y1, y1h=y1h, y2=y2, y2h=y2h, f1=f1, f1h=f1h)
+        return d
+
+    def dense_lambda(self, L, fock=True, y3f_sign=1.0):
+        """Textbook Lambda-CCSD(T) energies over every spin-orbital sextuple,
+        written from the term list of lambda_ccsd_t_left.F:7-10 with the
+        integrals in the orientation written there (real orbitals:
+        <jk||bc> = S[b,c,j,k], <jk||mc> = W[m,c,j,k], <ke||bc> = V[b,c,k,e]):
+            ys  = P(i/jk) P(a/bc) y(i a) <jk||bc>
+            yf  = P(k/ij) P(c/ab) y(i j a b) f(k c)
+            yd1 = -P(i/jk) P(c/ab) sum_m y(i m a b) <jk||mc>
+            yd2 = -P(k/ij) P(a/bc) sum_e y(i j a e) <ke||bc>
+        paired with t3d of the same orbitals (lambda_ccsd_t.F:156-178):
+            E1 = 1/36 sum t3d (yd1 + yd2 + yf) / D
+            E2 = 1/36 sum t3d (ys + yd1 + yd2 + yf) / D."""
+        def tr(x, pa, ph):
+            return np.transpose(x, tuple(pa) + tuple(3 + q for q in ph))
+
+        id3 = ((0, 1, 2), 1.0)
+        p_a = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]
+        p_c = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]
+        h_k = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]
+        h_i = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]
+
+        def P(x, pp, hh):
+            return sum(sa * sh * tr(x, pa, ph) for pa, sa in pp for ph, sh in hh)
+
+        v_oovv = self.S.transpose(2, 3, 0, 1)
+        v_ooov = self.W.transpose(2, 3, 0, 1)
+        v_ovvv = self.V.transpose(2, 3, 0, 1)
+        Y1, Y2 = L["Y1"], L["Y2"]
+        ys = P(np.einsum("ia,jkbc->abcijk", Y1, v_oovv), p_a, h_i)
+        yd = P(-np.einsum("imab,jkmc->abcijk", Y2, v_ooov), p_c, h_i)
+        yd = yd + P(-np.einsum("ijae,kebc->abcijk", Y2, v_ovvv), p_a, h_k)
+        if fock:
+            yd = yd + y3f_sign * P(np.einsum("ijab,kc->abcijk", Y2, L["Fhp"]), p_c, h_k)
+        _, t3d = self.dense_t3()
+        eo, ev = self.eo, self.ev
+        D = (-ev[:, None, None, None, None, None] - ev[None, :, None, None, None, None]
+             - ev[None, None, :, None, None, None] + eo[None, None, None, :, None, None]
+             + eo[None, None, None, None, :, None] + eo[None, None, None, None, None, :])
+        e1 = float(np.sum(t3d * yd / D)) / 36.0
+        e2 = float(np.sum(t3d * (ys + yd) / D)) / 36.0
+        return e1, e2

@@ -1,0 +1,98 @@
+"""Lambda-CCSD(T) on the GPU (tce_t_job_upload_lambda, tce_t_lambda_ccsd_t)
+against the oracle's composition (pinned against the dense textbook in
+tests/test_lambda.py) on the same spin-orbital systems.
+
+Tolerance: FP64, |dE| <= 1e-11 |E| + 1e-15 for both energies and every
+task's pair (the device sums each task in fixed-order blocks, the oracle in
+numpy's order).  Reduced precision, against FP64: the t3 and y3 tiles
+carry the d1/d2 product error of the mode, so |dE| <= tol |E| with tol
+1e-6 for FP32 and BF16X6 (products to 2^-24) and 3e-5 for BF16X3 (the
+dropped lo*lo term, 2^-16 of each product).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from nwchem_amd import TriplesJob
+from nwchem_amd.job import (BF16X3, BF16X6, FP32, LAMBDA_ORBITAL, LAMBDA_REFERENCE_ORDER,
+                            lambda_ccsd_t)
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import spin_systems  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LAMBDA_GPU = [
+    dict(sizes_o=[3, 2], sizes_v=[4, 3], irreps_o=[0, 1], irreps_v=[1, 0], seed=5),
+    dict(sizes_o=[8], sizes_v=[16, 8], irreps_o=[0], irreps_v=[0, 1], seed=9),
+    dict(sizes_o=[4, 4], sizes_v=[8, 8], seed=10, flip_symmetric=True, restricted=True),
+]
+IDS = ["spin_irreps", "spin_8_16", "restricted"]
+
+
+def _run(s, L, pairing, precision=0, fock=True):
+    b = s.blocks
+    with TriplesJob(s.tiles) as right, TriplesJob(s.tiles) as left:
+        right.upload(b)
+        left.upload_lambda(L["y1"], L["y1h"], L["y2"], L["y2h"], b.v2, b.v2h)
+        return lambda_ccsd_t(right, left, L["f1"] if fock else None, L["f1h"] if fock else None,
+                             pairing, precision)
+
+
+@pytest.mark.parametrize("pairing", [LAMBDA_ORBITAL, LAMBDA_REFERENCE_ORDER],
+                         ids=["orbital", "reference_order"])
+@pytest.mark.parametrize("prm", LAMBDA_GPU, ids=IDS)
+def test_lambda_matches_oracle(prm, pairing):
+    s = spin_systems.SpinSystem(**prm)
+    L = s.lambda_inputs()
+    b = s.blocks
+    want = oracle.lambda_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, L["y1"], L["y1h"],
+                                L["y2"], L["y2h"], b.v2, b.v2h, L["f1"], L["f1h"], pairing)
+    print(f"lambda case {prm} pairing={pairing} tasks={len(want[2])}", flush=True)
+    e1, e2, te = _run(s, L, pairing)
+    np.testing.assert_allclose([e1, e2], want[:2], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
+
+
+def test_lambda_without_fock_term_matches_oracle():
+    s = spin_systems.SpinSystem(**LAMBDA_GPU[0])
+    L = s.lambda_inputs()
+    b = s.blocks
+    want = oracle.lambda_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, L["y1"], L["y1h"],
+                                L["y2"], L["y2h"], b.v2, b.v2h, None, None, LAMBDA_ORBITAL)
+    print("lambda case without f", flush=True)
+    e1, e2, te = _run(s, L, LAMBDA_ORBITAL, fock=False)
+    np.testing.assert_allclose([e1, e2], want[:2], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
+
+
+@pytest.mark.parametrize("precision,rtol", [(FP32, 1e-6), (BF16X3, 3e-5), (BF16X6, 1e-6)],
+                         ids=["fp32", "bf16x3", "bf16x6"])
+def test_lambda_reduced_precision_near_fp64(precision, rtol):
+    s = spin_systems.SpinSystem(**LAMBDA_GPU[1])
+    L = s.lambda_inputs()
+    print(f"lambda precision={precision}", flush=True)
+    ref = _run(s, L, LAMBDA_ORBITAL)
+    got = _run(s, L, LAMBDA_ORBITAL, precision)
+    np.testing.assert_allclose(got[:2], ref[:2], rtol=rtol)
+
+
+def test_lambda_rejects_mismatched_jobs():
+    """Jobs over different tiles, or one job as both sides, are refused
+    before any device work."""
+    a = spin_systems.SpinSystem(**LAMBDA_GPU[0])
+    c = spin_systems.SpinSystem(**LAMBDA_GPU[2])
+    La = a.lambda_inputs()
+    with TriplesJob(a.tiles) as ja, TriplesJob(c.tiles) as jc:
+        ja.upload(a.blocks)
+        jc.upload(c.blocks)
+        with pytest.raises(RuntimeError, match="differ"):
+            lambda_ccsd_t(ja, jc)
+        with pytest.raises(RuntimeError, match="two distinct jobs"):
+            lambda_ccsd_t(ja, ja)
+        with pytest.raises(RuntimeError, match="y1 lacks"):
+            jc.upload_lambda(La["y1"], np.array([0], np.int64), La["y2"], La["y2h"],
+                             c.blocks.v2, c.blocks.v2h)
