@@ -102,6 +102,11 @@ struct MixedArgs {
   const void *x2[3], *y2[3];  // d2: T2e, Vd2 (parts: FP32 | hi, lo | hi, mid, lo)
   const void *x1[3], *y1[3];  // d1: Wd1, T2m
   int Ek, Mk;
+  // Lambda-CCSD(T) passes (lambda.hip): the window of left-hand tiles of
+  // items [lam_item0, lam_item0 + window) read by the right-hand pass, and
+  // the Fock hole-particle block F[h][p] (or null) the left-hand pass adds.
+  const double *lam_y = nullptr, *lam_f = nullptr;
+  long lam_item0 = 0;
 };
 
 // A launch covers a table of segments: work items [item0, item1) of the job
@@ -138,6 +143,8 @@ struct MixedArgsG {
   gptr<const void> x2[3], y2[3];
   gptr<const void> x1[3], y1[3];
   int Ek, Mk;
+  gptr<const double> lam_y, lam_f;
+  long lam_item0;
 };
 static_assert(sizeof(FusedArgsG) == sizeof(FusedArgs), "argument mirror");
 static_assert(sizeof(MixedArgsG) == sizeof(MixedArgs), "argument mirror");
@@ -175,6 +182,9 @@ __device__ __forceinline__ MixedArgsG load_args(const MixedArgs *p) {
   }
   M.Ek = q->Ek;
   M.Mk = q->Mk;
+  M.lam_y = (gptr<const double>)q->lam_y;
+  M.lam_f = (gptr<const double>)q->lam_f;
+  M.lam_item0 = q->lam_item0;
   return M;
 }
 
@@ -396,6 +406,15 @@ int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int preci
 // args->f.tasks[0], tiles written to args->f.partials (doubles, then singles).
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
                        hipStream_t s, bool sparse, unsigned long long *sched);
+// The Lambda-CCSD(T) passes over one segment of a job's items (8-wide
+// boxes).  left = true: the left-hand job's doubles (plus the Fock term of
+// args->lam_f) and singles of each wave's 4096 elements, in register order,
+// into args->f.partials at window slot (item - lam_item0) * 8 + wave (8192
+// doubles per slot).  left = false: the right-hand job's energy partials
+// factor * t3d * (y3 doubles) / D and factor * t3d * (y3 doubles + singles)
+// / D with the left-hand values read from args->lam_y at the same slots.
+int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int grid,
+                        hipStream_t s, bool sparse, unsigned long long *sched, bool left);
 // Zeroed counters of the dynamic schedule for one launching stream (every
 // launch leaves them zero; launches that may overlap need their own), or
 // null on failure (error set).  Free with hipFree.

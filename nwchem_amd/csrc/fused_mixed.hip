@@ -725,24 +725,10 @@ __device__ __forceinline__ void energy(const AccT (&acc)[64], const double *e,
 // of the task at ((((a r5 + b) r6 + c) r1 + i) r2 + j) r3 + k, i.e. the
 // reference's triplesx(h3,h2,h1,p6,p5,p4); doubles at A.partials, singles
 // n = |tile| doubles after them.  One task per launch (A.tasks[0]).
-template <class AccT>
-__device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double *e,
-                                            const FusedArgsG &A, const TaskDev &T, int sa, int sb,
-                                            int sc, int la, int lb, int lc, int i0, int j0,
-                                            int k0) {
-  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
-  const long n = (long)T.r1 * T.r2 * T.r3 * T.r4 * T.r5 * T.r6;
-  const gptr<double> td = A.partials, ts = A.partials + n;
-  const long base = (((long)la * T.r5 + lb) * T.r6 + lc) * T.r1;
-#pragma unroll
-  for (int ii = 0; ii < 4; ++ii)
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj)
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-        if (i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3)
-          td[((base + i0 + ii) * T.r2 + j0 + jj) * T.r3 + k0 + kk] = (double)acc[16 * ii + 4 * jj + kk];
-  double sv[64];
+// The 9 singles products of the lane's 64 elements (the energy's singles
+// term without the denominator), from the staged epilogue operands.
+__device__ __forceinline__ void singles_products(const double *e, int sa, int sb, int sc,
+                                                 double (&sv)[64]) {
 #pragma unroll
   for (int r = 0; r < 64; ++r) sv[r] = 0.0;
   const int obc = 0 * 64 + sb * 8 + sc, oac = 1 * 64 + sa * 8 + sc, oab = 2 * 64 + sa * 8 + sb;
@@ -770,6 +756,27 @@ __device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double 
         }
       }
   }
+}
+
+template <class AccT>
+__device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double *e,
+                                            const FusedArgsG &A, const TaskDev &T, int sa, int sb,
+                                            int sc, int la, int lb, int lc, int i0, int j0,
+                                            int k0) {
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  const long n = (long)T.r1 * T.r2 * T.r3 * T.r4 * T.r5 * T.r6;
+  const gptr<double> td = A.partials, ts = A.partials + n;
+  const long base = (((long)la * T.r5 + lb) * T.r6 + lc) * T.r1;
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        if (i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3)
+          td[((base + i0 + ii) * T.r2 + j0 + jj) * T.r3 + k0 + kk] = (double)acc[16 * ii + 4 * jj + kk];
+  double sv[64];
+  singles_products(e, sa, sb, sc, sv);
 #pragma unroll
   for (int ii = 0; ii < 4; ++ii)
 #pragma unroll
@@ -780,9 +787,117 @@ __device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double 
           ts[((base + i0 + ii) * T.r2 + j0 + jj) * T.r3 + k0 + kk] = sv[16 * ii + 4 * jj + kk];
 }
 
-// The item loop of both kernels below: TILES = false writes each item's
-// energy partials, TILES = true the task's t3 tiles (store_tiles).
-template <int PREC, bool TILES, bool SP>
+// Epilogues of the item loop: each item's energy partials, one task's t3
+// tiles (store_tiles), or the two Lambda-CCSD(T) passes (lambda.hip).
+enum { EPI_ENERGY = 0, EPI_TILES = 1, EPI_LAMBDA_LEFT = 2, EPI_LAMBDA_RIGHT = 3 };
+
+// Left-hand pass: the wave's y3 doubles (the accumulators plus the Fock
+// term P(c/ab) P(k/ij) y(a b i j) f(k c), y from the job's dense T2e,
+// T2e[i][j][a][b] = y(a b i j)) and y3 singles, register r of the lane at
+// out[r * 64 + lane] and out[(64 + r) * 64 + lane].  Elements outside the
+// task are written too (the right-hand pass skips them).
+template <class AccT>
+__device__ __forceinline__ void lambda_left(const AccT (&acc)[64], const double *e,
+                                            const MixedArgsG &M, const TaskDev &T, int sa, int sb,
+                                            int sc, int la, int lb, int lc, const ItemGeo &G,
+                                            gptr<double> out, int lane) {
+  const FusedArgsG &A = M.f;
+  double yd[64];
+#pragma unroll
+  for (int r = 0; r < 64; ++r) yd[r] = (double)acc[r];
+  if (M.lam_f && la < T.r4 && lb < T.r5 && lc < T.r6) {
+    const int no = A.no, nv = A.nv;
+    const long Ep = A.Ep;
+    const int a = T.op4 + la, b = T.op5 + lb, c = T.op6 + lc;
+    int hI[4], hJ[4], hK[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      hI[u] = hole(G.I, u);
+      hJ[u] = hole(G.J, u);
+      hK[u] = hole(G.K, u);
+    }
+    const gptr<const double> Y = A.T2e, F = M.lam_f;
+    auto y = [&](int p, int q, int h, int g) { return Y[(((long)h * no + g) * nv + p) * Ep + q]; };
+    // g(h1, h2, h3) = y(a b h1 h2) f(h3 c) - y(c b h1 h2) f(h3 a) - y(a c h1 h2) f(h3 b);
+    // the term is g(i, j, k) - g(k, j, i) - g(i, k, j).  T = 0: g(i, j, k);
+    // T = 1: -g(k, j, i); T = 2: -g(i, k, j).  Outer loops over (h1, h2),
+    // inner over h3, whose three f values are loaded once per term.
+    // hole u of set 0 (i), 1 (j) or 2 (k); every index is a constant after unrolling
+    auto H = [&](int set, int u) { return set == 0 ? hI[u] : (set == 1 ? hJ[u] : hK[u]); };
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int s1 = t == 1 ? 2 : 0, s2 = t == 2 ? 2 : 1, s3 = t == 0 ? 2 : (t == 1 ? 0 : 1);
+      const double sg = t == 0 ? 1.0 : -1.0;
+      double fc[4], fa[4], fb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        fc[u] = F[(long)H(s3, u) * nv + c];
+        fa[u] = F[(long)H(s3, u) * nv + a];
+        fb[u] = F[(long)H(s3, u) * nv + b];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const int h1 = H(s1, u), h2 = H(s2, w);
+          const double yab = sg * y(a, b, h1, h2), ycb = sg * y(c, b, h1, h2),
+                       yac = sg * y(a, c, h1, h2);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            // (h1, h2, h3) = (u, w, m) in the order of (i, j, k) for the term
+            const int ii = t == 1 ? m : u, jj = t == 2 ? m : w, kk = t == 0 ? m : (t == 1 ? u : w);
+            yd[16 * ii + 4 * jj + kk] += yab * fc[m] - ycb * fa[m] - yac * fb[m];
+          }
+        }
+    }
+  }
+  double sv[64];
+  singles_products(e, sa, sb, sc, sv);
+#pragma unroll
+  for (int r = 0; r < 64; ++r) {
+    out[r * 64 + lane] = yd[r];
+    out[(64 + r) * 64 + lane] = sv[r];
+  }
+}
+
+// Right-hand pass: den = t3d * factor / D as in energy(), e1 += den * yd,
+// e2 += den * (yd + ys) with the left-hand values of the same elements.
+template <class AccT>
+__device__ __forceinline__ void lambda_right(const AccT (&acc)[64], const FusedArgsG &A,
+                                             const TaskDev &T, int la, int lb, int lc, int i0,
+                                             int j0, int k0, const ItemGeo &G,
+                                             gptr<const double> in, int lane, double &e1,
+                                             double &e2) {
+  e1 = 0.0;
+  e2 = 0.0;
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  const int no = A.no;
+  const double denom_0 =
+      -(A.evl[no + T.op4 + la] + A.evl[no + T.op5 + lb] + A.evl[no + T.op6 + lc]);
+  double eh[3][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    eh[0][u] = A.evl[hole(G.I, u)];
+    eh[1][u] = A.evl[hole(G.J, u)];
+    eh[2][u] = A.evl[hole(G.K, u)];
+  }
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int r = 16 * ii + 4 * jj + kk;
+        const bool ok = i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3;
+        const double dn = (double)acc[r] * T.factor / (eh[0][ii] + eh[1][jj] + eh[2][kk] + denom_0);
+        const double yd = in[r * 64 + lane], ys = in[(64 + r) * 64 + lane];
+        e1 += ok ? dn * yd : 0.0;
+        e2 += ok ? dn * (yd + ys) : 0.0;
+      }
+}
+
+// The item loop of the kernels below, with the epilogue EPI.
+template <int PREC, int EPI, bool SP>
 __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *segs, int nseg,
                                                            long nglobal,
                                                            unsigned long long *sched) {
@@ -877,18 +992,26 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     all_passes<PREC, SP, 0>(acc, lds, lds_base, M, Tp, G, wave, lane, j, buf, total, F);
 
     // Epilogue: the chunk buffers are free (the last chunk ended with a
-    // barrier and no load is outstanding).
+    // barrier and no load is outstanding).  The right-hand Lambda pass needs
+    // no singles operands.
     double *epi = reinterpret_cast<double *>(lds);
-    stage_epilogue(epi, A, tid, G);
+    if constexpr (EPI != EPI_LAMBDA_RIGHT) stage_epilogue(epi, A, tid, G);
     __syncthreads();
     // final layout: L = c, lane = 4a' + b' + 16c'
     const int sa = 4 * cube(wave, 0) + hi, sb = 4 * cube(wave, 1) + lo, sc = 4 * cube(wave, 2) + kq;
-    if constexpr (TILES) {
+    if constexpr (EPI == EPI_TILES) {
       store_tiles(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk);
+    } else if constexpr (EPI == EPI_LAMBDA_LEFT) {
+      lambda_left(acc, epi, M, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, G,
+                  A.partials + ((item - M.lam_item0) * 8 + wave) * 8192, lane);
     } else {
       double e4, e5;
-      energy(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G, e4,
-             e5);
+      if constexpr (EPI == EPI_LAMBDA_RIGHT)
+        lambda_right(acc, A, T, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G,
+                     M.lam_y + ((item - M.lam_item0) * 8 + wave) * 8192, lane, e4, e5);
+      else
+        energy(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G,
+               e4, e5);
       wave_sum(e4, e5);
       if (lane == 0) {
         A.partials[2 * (item * 8 + wave)] = e4;
@@ -908,27 +1031,27 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
 // the precision's chunk (16 FP32 / 32 bf16 values); the job layer pads them.
 // sparse: some job of the table has contraction ranges narrower than its
 // spaces (spin / irrep labels), which the SP = true kernels follow.
-template <int PREC, bool TILES, bool SP>
+template <int PREC, int EPI, bool SP>
 static void launch_one(const MixedSegment *segs, int nseg, long total, int grid, hipStream_t s,
                        unsigned long long *sched) {
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<PREC, TILES, SP>), dim3((unsigned)grid),
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(fused_triples_mixed<PREC, EPI, SP>), dim3((unsigned)grid),
                      dim3(512), 0, s, segs, nseg, total, sched);
 }
-template <bool TILES>
+template <int EPI>
 static int launch_prec(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        bool sparse, hipStream_t s, unsigned long long *q) {
   if (precision == TCE_T_FP64)
-    sparse ? launch_one<TCE_T_FP64, TILES, true>(segs, nseg, total, grid, s, q)
-           : launch_one<TCE_T_FP64, TILES, false>(segs, nseg, total, grid, s, q);
+    sparse ? launch_one<TCE_T_FP64, EPI, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_FP64, EPI, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_FP32)
-    sparse ? launch_one<TCE_T_FP32, TILES, true>(segs, nseg, total, grid, s, q)
-           : launch_one<TCE_T_FP32, TILES, false>(segs, nseg, total, grid, s, q);
+    sparse ? launch_one<TCE_T_FP32, EPI, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_FP32, EPI, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_BF16X3)
-    sparse ? launch_one<TCE_T_BF16X3, TILES, true>(segs, nseg, total, grid, s, q)
-           : launch_one<TCE_T_BF16X3, TILES, false>(segs, nseg, total, grid, s, q);
+    sparse ? launch_one<TCE_T_BF16X3, EPI, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_BF16X3, EPI, false>(segs, nseg, total, grid, s, q);
   else if (precision == TCE_T_BF16X6)
-    sparse ? launch_one<TCE_T_BF16X6, TILES, true>(segs, nseg, total, grid, s, q)
-           : launch_one<TCE_T_BF16X6, TILES, false>(segs, nseg, total, grid, s, q);
+    sparse ? launch_one<TCE_T_BF16X6, EPI, true>(segs, nseg, total, grid, s, q)
+           : launch_one<TCE_T_BF16X6, EPI, false>(segs, nseg, total, grid, s, q);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -936,12 +1059,18 @@ static int launch_prec(const MixedSegment *segs, int nseg, long total, int preci
 
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        hipStream_t s, bool sparse, unsigned long long *sched) {
-  return launch_prec<false>(segs, nseg, total, precision, grid, sparse, s, sched);
+  return launch_prec<EPI_ENERGY>(segs, nseg, total, precision, grid, sparse, s, sched);
 }
 
 int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int grid,
                        hipStream_t s, bool sparse, unsigned long long *sched) {
-  return launch_prec<true>(segs, 1, total, precision, grid, sparse, s, sched);
+  return launch_prec<EPI_TILES>(segs, 1, total, precision, grid, sparse, s, sched);
+}
+
+int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int grid,
+                        hipStream_t s, bool sparse, unsigned long long *sched, bool left) {
+  return left ? launch_prec<EPI_LAMBDA_LEFT>(segs, 1, total, precision, grid, sparse, s, sched)
+              : launch_prec<EPI_LAMBDA_RIGHT>(segs, 1, total, precision, grid, sparse, s, sched);
 }
 
 

@@ -93,6 +93,10 @@ int launch_task_reduce(const long *bounds, int ntasks, int ppi, const double *pa
 int ensure_tasks8(Job &J);
 // Reduced-precision copies of the contraction tensors (converted once).
 int prepare_mixed(Job &J, int precision);
+// The arguments M and one segment of items [item0, item1) in the job's
+// launch table, copied on J.stream (the host staging is reused by the next
+// call: synchronise J.stream first); the device segment, or null.
+const MixedSegment *stage_mixed(Job &J, const MixedArgs &M, long item1, long item0);
 // Kernel arguments of a job: FP64 with the job's own task table (wide =
 // false) or the 8-wide one (wide = true); reduced precision.
 FusedArgs job_args(const Job &J, bool wide);

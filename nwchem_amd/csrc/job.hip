@@ -845,6 +845,11 @@ static const SegmentT<Args> *stage_table(Job &J, const Args &args, long item1, l
   return (const SegmentT<Args> *)(d + off);
 }
 
+const MixedSegment *tce_t::stage_mixed(Job &J, const MixedArgs &M, long item1, long item0) {
+  if (ensure_table(J) != TCE_T_OK) return nullptr;
+  return stage_table(J, M, item1, item0);
+}
+
 // Contraction extent of a reduced-precision mode: K padded to the kernel's
 // chunk (16 FP32 or 32 bf16 values).
 static int mixed_extent(long K, int precision) {

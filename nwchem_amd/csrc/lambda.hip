@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "job.h"
@@ -158,6 +160,89 @@ int fock_hole_particle(const Tiles &T, const double *f1, const long *f1h,
   return TCE_T_OK;
 }
 
+// Orbital pairing on the job's own kernel tasks (sub-tiles of the reference
+// tiles, each box once with its symmetry factor: the product t3 y3 / D is
+// symmetric under the permutations within equal tiles, as t3 t3 / D is).
+// The items run in windows of W: the left-hand pass leaves each item's y3
+// doubles (Fock term added) and singles in a window buffer in register
+// order, the right-hand pass forms the energy partials of the same items
+// from its t3 doubles and the window; the job's task reduction then gives
+// the per-task sums.  No tile leaves the registers in the tile layout.
+int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<double> &te) {
+  for (Job *J : {&JR, &JL}) {
+    int rc = ensure_tasks8(*J);
+    if (rc == TCE_T_OK && precision != TCE_T_FP64) rc = prepare_mixed(*J, precision);
+    if (rc != TCE_T_OK) return rc;
+  }
+  if (JR.nitems8 != JL.nitems8 || JR.tasks8.size() != JL.tasks8.size()) {
+    set_error("tce_t_lambda_ccsd_t: the jobs' kernel tasks differ");
+    return TCE_T_ERR_ARG;
+  }
+  const long N = JR.nitems8, nt = JR.ntask();
+  constexpr long SLOT = 8L * 8192 * sizeof(double);  // window bytes per item (8 waves)
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b), "memory info");
+  const long budget = std::min<long>(16L << 30, (long)(free_b / 4));
+  long W = std::max(1L, std::min(N, budget / SLOT));
+  if (const char *w = getenv("TCE_T_LAMBDA_WINDOW")) {  // items per window (tests, tuning)
+    const long x = atol(w);
+    if (x > 0) W = std::min(W, x);
+  }
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, JR.device), "attr");
+  double *win = nullptr;
+  HIPCHK(hipMalloc(&win, (size_t)W * SLOT), "lambda window");
+  int rc = TCE_T_OK;
+  for (long i0 = 0; i0 < N && rc == TCE_T_OK; i0 += W) {
+    const long i1 = std::min(N, i0 + W);
+    const int grid = (int)std::min<long>(std::max(1, cus), i1 - i0);
+    MixedArgs ML = job_margs(JL, precision);
+    ML.f.partials = win;
+    ML.lam_f = dF;
+    ML.lam_item0 = i0;
+    const MixedSegment *sl = stage_mixed(JL, ML, i1, i0);
+    if (!sl || launch_fused_lambda(sl, i1 - i0, precision, grid, JL.stream, JL.sparse,
+                                   sched_use(JL.dsched), true) != 0) {
+      set_error("lambda left pass: launch failed");
+      rc = TCE_T_ERR_HIP;
+      break;
+    }
+    if (!hip_ok(hipStreamSynchronize(JL.stream), "lambda left pass")) {
+      rc = TCE_T_ERR_HIP;
+      break;
+    }
+    MixedArgs MR = job_margs(JR, precision);
+    MR.lam_y = win;
+    MR.lam_item0 = i0;
+    const MixedSegment *sr = stage_mixed(JR, MR, i1, i0);
+    if (!sr || launch_fused_lambda(sr, i1 - i0, precision, grid, JR.stream, JR.sparse,
+                                   sched_use(JR.dsched), false) != 0) {
+      set_error("lambda right pass: launch failed");
+      rc = TCE_T_ERR_HIP;
+      break;
+    }
+    if (!hip_ok(hipStreamSynchronize(JR.stream), "lambda right pass")) rc = TCE_T_ERR_HIP;
+  }
+  if (rc == TCE_T_OK) {
+    te.assign(2 * nt, 0.0);
+    if (launch_task_reduce(job_bounds(JR, true), (int)nt, 8, JR.dpart8, JR.dtaskE, JR.stream) != 0) {
+      set_error("task_reduce launch failed");
+      rc = TCE_T_ERR_HIP;
+    } else if (!hip_ok(hipMemcpyAsync(te.data(), JR.dtaskE, 2 * nt * sizeof(double),
+                                      hipMemcpyDeviceToHost, JR.stream),
+                       "D2H") ||
+               !hip_ok(hipStreamSynchronize(JR.stream), "lambda task sums")) {
+      rc = TCE_T_ERR_HIP;
+    }
+  }
+  if (rc != TCE_T_OK) {  // nothing left in flight on either stream
+    (void)hipStreamSynchronize(JL.stream);
+    (void)hipStreamSynchronize(JR.stream);
+  }
+  (void)hipFree(win);
+  return rc;
+}
+
 }  // namespace
 }  // namespace tce_t
 
@@ -274,6 +359,25 @@ extern "C" int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const doub
     release();
     return TCE_T_ERR_HIP;
   }
+  const char *mode = getenv("TCE_T_LAMBDA");
+  if (pairing == TCE_T_LAMBDA_ORBITAL && JR.tiling == JL.tiling &&
+      JR.tiling != TCE_T_TILING_ORBITAL && !(mode && !strcmp(mode, "tiles"))) {
+    std::vector<double> te;
+    const int rc = lambda_fused(JR, JL, dF, precision, te);
+    release();
+    if (rc != TCE_T_OK) return rc;
+    double s1 = 0.0, s2 = 0.0;
+    for (long t = 0; t < nt; ++t) {
+      s1 += te[2 * t];
+      s2 += te[2 * t + 1];
+    }
+    if (task_energies) std::copy(te.begin(), te.end(), task_energies);
+    if (energy1) *energy1 = s1;
+    if (energy2) *energy2 = s2;
+    return TCE_T_OK;
+  }
+  // Whole tiles per task: the reference-order pairing (and ORBITAL-tiled jobs,
+  // whose kernel tasks do not follow the reference tasks)
   if (!hip_ok(hipMalloc(&dpart, 2 * LBLOCKS * sizeof(double)), "lambda partials")) {
     release();
     return TCE_T_ERR_HIP;

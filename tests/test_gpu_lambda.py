@@ -42,16 +42,27 @@ def _run(s, L, pairing, precision=0, fock=True):
                              pairing, precision)
 
 
-@pytest.mark.parametrize("pairing", [LAMBDA_ORBITAL, LAMBDA_REFERENCE_ORDER],
-                         ids=["orbital", "reference_order"])
+# orbital pairing: the fused passes over the job's kernel tasks (default,
+# in one window or in windows of 3 items), or whole tiles per task
+PATHS = {"fused": {}, "fused_windows": {"TCE_T_LAMBDA_WINDOW": "3"},
+         "tiles": {"TCE_T_LAMBDA": "tiles"}}
+
+
+@pytest.mark.parametrize("case", [(LAMBDA_ORBITAL, "fused"), (LAMBDA_ORBITAL, "fused_windows"),
+                                  (LAMBDA_ORBITAL, "tiles"), (LAMBDA_REFERENCE_ORDER, "tiles")],
+                         ids=["orbital_fused", "orbital_fused_windows", "orbital_tiles",
+                              "reference_order"])
 @pytest.mark.parametrize("prm", LAMBDA_GPU, ids=IDS)
-def test_lambda_matches_oracle(prm, pairing):
+def test_lambda_matches_oracle(prm, case, monkeypatch):
+    pairing, path = case
+    for k, v in PATHS[path].items():
+        monkeypatch.setenv(k, v)
     s = spin_systems.SpinSystem(**prm)
     L = s.lambda_inputs()
     b = s.blocks
     want = oracle.lambda_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, L["y1"], L["y1h"],
                                 L["y2"], L["y2h"], b.v2, b.v2h, L["f1"], L["f1h"], pairing)
-    print(f"lambda case {prm} pairing={pairing} tasks={len(want[2])}", flush=True)
+    print(f"lambda case {prm} pairing={pairing} path={path} tasks={len(want[2])}", flush=True)
     e1, e2, te = _run(s, L, pairing)
     np.testing.assert_allclose([e1, e2], want[:2], rtol=1e-11, atol=1e-15)
     np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
@@ -69,15 +80,41 @@ def test_lambda_without_fock_term_matches_oracle():
     np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
 
 
+@pytest.mark.parametrize("path", ["fused", "tiles"])
 @pytest.mark.parametrize("precision,rtol", [(FP32, 1e-6), (BF16X3, 3e-5), (BF16X6, 1e-6)],
                          ids=["fp32", "bf16x3", "bf16x6"])
-def test_lambda_reduced_precision_near_fp64(precision, rtol):
+def test_lambda_reduced_precision_near_fp64(precision, rtol, path, monkeypatch):
+    for k, v in PATHS[path].items():
+        monkeypatch.setenv(k, v)
     s = spin_systems.SpinSystem(**LAMBDA_GPU[1])
     L = s.lambda_inputs()
-    print(f"lambda precision={precision}", flush=True)
+    print(f"lambda precision={precision} path={path}", flush=True)
     ref = _run(s, L, LAMBDA_ORBITAL)
     got = _run(s, L, LAMBDA_ORBITAL, precision)
     np.testing.assert_allclose(got[:2], ref[:2], rtol=rtol)
+
+
+def test_lambda_fused_equals_tiles_on_a_synthetic_job(monkeypatch):
+    """A C1 job of ragged tiles (several kernel tasks per reference task,
+    sub-tile factors): the fused passes and the whole-tile path give the
+    same energies (different summation orders)."""
+    import bench
+    from nwchem_amd import tiles as tl
+
+    tm = tl.TileModel.synthetic(10, 36, 4, 10, 21)
+    b, y1, y1h, y2, y2h, f1, f1h = bench.lambda_inputs(tm, 22)
+    print(f"lambda synthetic no=10 nv=36 tasks={len(tm.tasks())}", flush=True)
+    out = {}
+    for path in ("fused", "tiles"):
+        monkeypatch.delenv("TCE_T_LAMBDA", raising=False)
+        for k, v in PATHS[path].items():
+            monkeypatch.setenv(k, v)
+        with TriplesJob(tm) as right, TriplesJob(tm) as left:
+            right.upload(b)
+            left.upload_lambda(y1, y1h, y2, y2h, b.v2, b.v2h)
+            out[path] = lambda_ccsd_t(right, left, f1, f1h)
+    np.testing.assert_allclose(out["fused"][:2], out["tiles"][:2], rtol=1e-12)
+    np.testing.assert_allclose(out["fused"][2], out["tiles"][2], rtol=1e-10, atol=1e-14)
 
 
 def test_lambda_rejects_mismatched_jobs():
