@@ -356,11 +356,12 @@ def lambda_inputs(tm, seed: int):
     return b, y1, y1h, y2, y2h, f1, f1h
 
 
-def bench_lambda(tm, blocks, seed: int, F_ref: float):
-    """tce_t_lambda_ccsd_t over every task of the job (both pairings'
-    work is the same; the orbital one is timed): two tile exports per
-    task (right t3d, left y3s/y3d: the reference tiles' work each) and the
-    element-wise energy kernel."""
+def bench_lambda(tm, blocks, seed: int, F_run: float, t_step: float):
+    """tce_t_lambda_ccsd_t over every task of the job, orbital pairing: two
+    passes of the fused kernel over the job's kernel tasks (the left-hand
+    job's y3 into a window, the right-hand job's t3d with the energy), so
+    the executed work is twice the (T) job's F; reported against two (T)
+    energies' time."""
     from nwchem_amd import TriplesJob
     from nwchem_amd.job import LAMBDA_ORBITAL, lambda_ccsd_t
 
@@ -374,11 +375,12 @@ def bench_lambda(tm, blocks, seed: int, F_ref: float):
         e1, e2, te = lambda_ccsd_t(right, left, f1, f1h, LAMBDA_ORBITAL)
         wall = time.perf_counter() - t1
     out = {"tasks": int(te.shape[0]), "seconds": wall, "energy1": e1, "energy2": e2,
-           "tile_flops": 2.0 * F_ref, "tflops": 2.0 * F_ref / wall / 1e12,
-           "inputs_seconds": t_in}
+           "flops": 2.0 * F_run, "tflops": 2.0 * F_run / wall / 1e12,
+           "frac_of_fp64_peak": 2.0 * F_run / wall / 1e12 / PEAK_FP64_TFLOPS,
+           "vs_two_triples_energies": wall / (2.0 * t_step), "inputs_seconds": t_in}
     print(f"[bench] lambda-ccsd(t): {wall:.2f} s for {out['tasks']} tasks "
-          f"({out['tflops']:.1f} TF/s of tile work), energy1={e1:.12e} energy2={e2:.12e}",
-          file=sys.stderr, flush=True)
+          f"({out['tflops']:.1f} TF/s, {wall / (2.0 * t_step):.3f}x two (T) energies), "
+          f"energy1={e1:.12e} energy2={e2:.12e}", file=sys.stderr, flush=True)
     return out
 
 
@@ -598,7 +600,7 @@ def main() -> int:
               f"{16.0 * sum(elems) / tw / 1e9:.1f} GB/s of tiles to the host)",
               file=sys.stderr, flush=True)
     job.close()
-    lam = bench_lambda(tm, blocks, args.seed + 1, F) if args.lam else None
+    lam = bench_lambda(tm, blocks, args.seed + 1, F_run, wall / args.steps) if args.lam else None
     reftile = None
     if args.reference_tiling:
         # the same job with the reference's tiles as kernel tasks: the work the

@@ -3,7 +3,9 @@
 ! fills T1/T2/V2 canonical blocks with the seeded generator, uploads them with
 ! their hash tables and prints E[T], E(T); then E(T) by restart blocks, and
 ! E[T], E(T) once more from the task tiles the library hands back, summed
-! here as the tiles' other consumers (cr_ccsd_t.F, lambda_ccsd_t.F) do.
+! here as the tiles' other consumers (cr_ccsd_t.F, lambda_ccsd_t.F) do;
+! then the Lambda-CCSD(T) energies (lambda_ccsd_t.F) of a second seeded
+! amplitude set taken as the Lambda amplitudes.
 ! Usage: tce_t_fdemo [no nv tile_o tile_v]   (default 8 32 8 16)
 !        tce_t_fdemo intorb   (host only: the orbital-form integral input of a
 !        closed-shell tiling through tce_t_v2_alpha_offset and
@@ -138,9 +140,83 @@ program tce_t_fdemo
   end do
   end do
   write(*, '(A,ES24.16,A,ES24.16)') 'tiles E[T]=', ta, ' E(T)=', tb
+  call lambda_demo()
   rc = tce_t_job_destroy(job)
 
 contains
+
+  ! The Lambda amplitudes y1, y2 laid out as tce_y1_offset / tce_y2_offset
+  ! keep them (blocks [h][p], [h][h'][p][p']), taken from the t-layout blocks
+  ! of the generator's seed 4321; f1 over the tce_mo1e_offset table as
+  ! 0.05 sin(k); the left-hand job uploaded from them, and the energies of
+  ! lambda_ccsd_t with the job above as the right-hand side.
+  subroutine lambda_demo()
+    integer(c_long), allocatable :: hy1(:), hy2(:), hf(:)
+    real(c_double), allocatable :: u1(:), u2(:), uv(:), y1(:), y2(:), f1(:), te(:)
+    integer(c_long) :: ly1, ly2, lf, sy1, sy2, sf, k, key, off, tof, h, p, q1, q2, g3, g4
+    integer(c_long) :: ra, rb, ri, rj, a, b, ii, jj, nt
+    type(c_ptr) :: left
+    real(c_double) :: l1, l2
+    allocate(hy1(2 * noab * nvab + 1), hy2(2 * (noab * nvab)**2 + 1), hf(2 * n * n + 1))
+    ly1 = tce_t_y1_offset(tiles, hy1, sy1)
+    ly2 = tce_t_y2_offset(tiles, hy2, sy2)
+    lf = tce_t_f1_offset(tiles, hf, sf)
+    allocate(u1(s1), u2(s2), uv(sv), y1(sy1), y2(sy2), f1(sf))
+    rc = tce_t_synth_blocks(tiles, 4321_c_long_long, u1, h1, u2, h2, uv, hv)
+    if (rc /= TCE_T_OK) stop 9
+    do k = 1, ly1  ! y(h p) block [h][p] from the t1 block [p][h]
+      key = hy1(1 + k)
+      off = hy1(1 + ly1 + k)
+      h = key / nvab + 1
+      p = mod(key, nvab) + noab + 1
+      tof = tce_t_hash(h1, h - 1 + noab * (p - noab - 1))
+      ri = range(h)
+      ra = range(p)
+      do ii = 0, ri - 1
+        do a = 0, ra - 1
+          y1(off + ii * ra + a + 1) = u1(tof + a * ri + ii + 1)
+        end do
+      end do
+    end do
+    do k = 1, ly2  ! y(h3 h4 p1 p2) block [h3][h4][p1][p2] from the t2 block [p1][p2][h3][h4]
+      key = hy2(1 + k)
+      off = hy2(1 + ly2 + k)
+      q2 = mod(key, nvab) + noab + 1
+      key = key / nvab
+      q1 = mod(key, nvab) + noab + 1
+      key = key / nvab
+      g4 = mod(key, noab) + 1
+      g3 = key / noab + 1
+      tof = tce_t_hash(h2, g4 - 1 + noab * (g3 - 1 + noab * (q2 - noab - 1 + nvab * (q1 - noab - 1))))
+      ra = range(q1)
+      rb = range(q2)
+      ri = range(g3)
+      rj = range(g4)
+      do ii = 0, ri - 1
+        do jj = 0, rj - 1
+          do a = 0, ra - 1
+            do b = 0, rb - 1
+              y2(off + ((ii * rj + jj) * ra + a) * rb + b + 1) = &
+                  u2(tof + ((a * rb + b) * ri + ii) * rj + jj + 1)
+            end do
+          end do
+        end do
+      end do
+    end do
+    do k = 1, sf
+      f1(k) = 0.05d0 * sin(dble(k))
+    end do
+    rc = tce_t_job_create(tiles, left)
+    if (rc /= TCE_T_OK) stop 10
+    rc = tce_t_job_upload_lambda(left, y1, hy1, y2, hy2, v2, hv)
+    if (rc /= TCE_T_OK) stop 11
+    nt = tce_t_job_num_tasks(job)
+    allocate(te(2 * nt))
+    rc = tce_t_lambda_ccsd_t(job, left, f1, hf, TCE_T_LAMBDA_ORBITAL, TCE_T_FP64, l1, l2, te, nt)
+    if (rc /= TCE_T_OK) stop 12
+    write(*, '(A,ES24.16,A,ES24.16)') 'lambda energy1=', l1, ' energy2=', l2
+    rc = tce_t_job_destroy(left)
+  end subroutine lambda_demo
 
   ! Closed-shell tiling as tce_tile.F lays it out with intorb: alpha holes
   ! (3, 2), beta holes, alpha particles (4, 5, 3), beta particles; k_b2am

@@ -16,6 +16,7 @@ module tce_t_mi355x
 
   integer(c_int), parameter :: TCE_T_OK = 0
   integer(c_int), parameter :: TCE_T_FP64 = 0
+  integer(c_int), parameter :: TCE_T_LAMBDA_ORBITAL = 0, TCE_T_LAMBDA_REFERENCE_ORDER = 1
   integer(c_int), parameter :: TCE_T_TILING_REFERENCE = 0, TCE_T_TILING_SUBTILE = 1, &
                                TCE_T_TILING_ORBITAL = 2
 
@@ -129,6 +130,32 @@ module tce_t_mi355x
       integer(c_long), value :: ntasks
     end function
 
+    ! Lambda-CCSD(T) (lambda_ccsd_t.F): the left-hand job from d_y1/k_l1_offset,
+    ! d_y2/k_l2_offset and the right-hand job's d_v2/k_v2_offset
+    integer(c_int) function tce_t_job_upload_lambda(job, y1, k_y1_offset, y2, k_y2_offset, &
+                                                     v2, k_v2_offset) &
+        bind(C, name="tce_t_job_upload_lambda")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      real(c_double), intent(in) :: y1(*), y2(*), v2(*)
+      integer(c_long), intent(in) :: k_y1_offset(*), k_y2_offset(*), k_v2_offset(*)
+    end function
+
+    ! energy1, energy2 of lambda_ccsd_t from the right-hand (t1, t2) and
+    ! left-hand (Lambda) jobs and d_f1/k_f1_offset; task_energies(2, ntasks)
+    integer(c_int) function tce_t_lambda_ccsd_t(right, left, f1, k_f1_offset, pairing, &
+                                                 precision, energy1, energy2, task_energies, &
+                                                 ntasks) bind(C, name="tce_t_lambda_ccsd_t")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: right, left
+      real(c_double), intent(in) :: f1(*)
+      integer(c_long), intent(in) :: k_f1_offset(*)
+      integer(c_int), value :: pairing, precision
+      real(c_double), intent(out) :: energy1, energy2
+      real(c_double), intent(out) :: task_energies(*)
+      integer(c_long), value :: ntasks
+    end function
+
     integer(c_int) function tce_t_job_destroy(job) bind(C, name="tce_t_job_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: job
@@ -155,6 +182,35 @@ module tce_t_mi355x
     end function
 
     integer(c_long) function tce_t_v2_offset(tiles, hash, size) bind(C, name="tce_t_v2_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    ! tce_hash.F: offset of key in a block table, or -1
+    integer(c_long) function tce_t_hash(hash, key) bind(C, name="tce_t_hash")
+      import :: c_long
+      integer(c_long), intent(in) :: hash(*)
+      integer(c_long), value :: key
+    end function
+
+    ! tce_y1_offset.F, tce_y2_offset.F, tce_mo1e_offset.F
+    integer(c_long) function tce_t_y1_offset(tiles, hash, size) bind(C, name="tce_t_y1_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    integer(c_long) function tce_t_y2_offset(tiles, hash, size) bind(C, name="tce_t_y2_offset")
+      import :: c_long, tce_t_tiles
+      type(tce_t_tiles), intent(in) :: tiles
+      integer(c_long), intent(out) :: hash(*)
+      integer(c_long), intent(out) :: size
+    end function
+
+    integer(c_long) function tce_t_f1_offset(tiles, hash, size) bind(C, name="tce_t_f1_offset")
       import :: c_long, tce_t_tiles
       type(tce_t_tiles), intent(in) :: tiles
       integer(c_long), intent(out) :: hash(*)

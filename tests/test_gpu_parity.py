@@ -255,6 +255,16 @@ def test_fortran_caller_matches_oracle():
     t = re.search(r"tiles E\[T\]=\s*(\S+)\s+E\(T\)=\s*(\S+)", out)
     assert t, out
     assert abs(float(t.group(1)) - e1) <= tol(e1) and abs(float(t.group(2)) - e2) <= tol(e2)
+    # Lambda-CCSD(T) from Fortran: the left-hand job from y blocks the caller
+    # laid out (seed 4321 amplitudes), f1 = 0.05 sin(k); against the oracle
+    import bench
+    lam = re.search(r"lambda energy1=\s*(\S+)\s+energy2=\s*(\S+)", out)
+    assert lam, out
+    _, y1, y1h, y2, y2h, f1, f1h = bench.lambda_inputs(tm, 4321)
+    f1 = 0.05 * np.sin(np.arange(1, f1.size + 1, dtype=np.float64))
+    want = oracle.lambda_ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, y1, y1h, y2, y2h, b.v2, b.v2h, f1,
+                                f1h, 0)
+    np.testing.assert_allclose([float(lam.group(1)), float(lam.group(2))], want[:2], rtol=1e-11)
 
 
 # ---- golden vectors (reference kernels, tests/golden) and spin/irrep/restricted tilings
