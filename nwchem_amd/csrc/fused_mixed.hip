@@ -851,13 +851,13 @@ __device__ __forceinline__ void lambda_left(const AccT (&acc)[64], const double 
         }
     }
   }
+  // doubles out before the singles are formed: one 64-value array live at a time
+#pragma unroll
+  for (int r = 0; r < 64; ++r) out[r * 64 + lane] = yd[r];
   double sv[64];
   singles_products(e, sa, sb, sc, sv);
 #pragma unroll
-  for (int r = 0; r < 64; ++r) {
-    out[r * 64 + lane] = yd[r];
-    out[(64 + r) * 64 + lane] = sv[r];
-  }
+  for (int r = 0; r < 64; ++r) out[(64 + r) * 64 + lane] = sv[r];
 }
 
 // Right-hand pass: den = t3d * factor / D as in energy(), e1 += den * yd,

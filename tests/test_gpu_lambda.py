@@ -68,6 +68,30 @@ def test_lambda_matches_oracle(prm, case, monkeypatch):
     np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
 
 
+@pytest.mark.parametrize("tiling", ["reference", "orbital", "mixed"])
+def test_lambda_other_tilings_match_oracle(tiling):
+    """Jobs with the reference tiles as kernel tasks (fused passes over
+    them), ORBITAL-tiled jobs and jobs of different tilings (whole tiles
+    per task) give the oracle's energies and per-task pairs."""
+    from nwchem_amd.job import TILING_ORBITAL, TILING_REFERENCE, TILING_SUBTILE
+
+    tl_r, tl_l = {"reference": (TILING_REFERENCE, TILING_REFERENCE),
+                  "orbital": (TILING_ORBITAL, TILING_ORBITAL),
+                  "mixed": (TILING_SUBTILE, TILING_REFERENCE)}[tiling]
+    s = spin_systems.SpinSystem(**LAMBDA_GPU[0])
+    L = s.lambda_inputs()
+    b = s.blocks
+    want = oracle.lambda_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, L["y1"], L["y1h"],
+                                L["y2"], L["y2h"], b.v2, b.v2h, L["f1"], L["f1h"], LAMBDA_ORBITAL)
+    print(f"lambda tiling={tiling}", flush=True)
+    with TriplesJob(s.tiles, tiling=tl_r) as right, TriplesJob(s.tiles, tiling=tl_l) as left:
+        right.upload(b)
+        left.upload_lambda(L["y1"], L["y1h"], L["y2"], L["y2h"], b.v2, b.v2h)
+        e1, e2, te = lambda_ccsd_t(right, left, L["f1"], L["f1h"], LAMBDA_ORBITAL)
+    np.testing.assert_allclose([e1, e2], want[:2], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(te, want[2], rtol=1e-11, atol=1e-15)
+
+
 def test_lambda_without_fock_term_matches_oracle():
     s = spin_systems.SpinSystem(**LAMBDA_GPU[0])
     L = s.lambda_inputs()
