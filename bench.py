@@ -21,7 +21,9 @@ same job with the reference's tiles as kernel tasks (F_ref of work).  The
 same line reports the reduced-precision modes (`--modes`): the d1/d2
 contractions in FP32 and in 3-term split-BF16, each timed the same way on the
 same resident inputs, with energies/s, the speedup over FP64 and |dE| in
-micro-Eh against the FP64 result of the run.
+micro-Eh against the FP64 result of the run; and the Lambda-CCSD(T)
+energies of lambda_ccsd_t.F over the same tiles (`--lambda`), a second
+seeded amplitude set taken as the Lambda amplitudes.
 """
 from __future__ import annotations
 
@@ -496,7 +498,7 @@ def main() -> int:
                     help="also time the job with the reference tiles as kernel tasks (0: skip)")
     ap.add_argument("--task-tiles", type=int, default=1,
                     help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
-    ap.add_argument("--lambda", dest="lam", type=int, default=0,
+    ap.add_argument("--lambda", dest="lam", type=int, default=1,
                     help="also time tce_t_lambda_ccsd_t over the whole job (1) or skip it (0)")
     ap.add_argument("--modes", default="fp32,bf16x3,bf16x6",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
