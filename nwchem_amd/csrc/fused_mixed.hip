@@ -63,7 +63,9 @@ struct Prec<TCE_T_FP64> {
   static constexpr int KC = 8;     // contraction values per chunk
   static constexpr int NPART = 1;  // operand parts
   static constexpr int ES = 8;     // bytes per element
-  static constexpr int NBUF = 3;   // LDS chunk buffers
+  // LDS chunk buffers (24 KB each), chunks loaded NBUF - 1 ahead.  Config 2
+  // (no=40, nv=320), kernel ms: 3 buffers 8283, 4 buffers 8176, 5 buffers 8251.
+  static constexpr int NBUF = 4;
   static constexpr bool SPLIT = false;  // chunk loop: see pass_step
   typedef double acc;
 };
@@ -505,6 +507,28 @@ __device__ __forceinline__ void load_ahead(const Src<PREC> &s0, const MixedArgsG
 // s_waitcnt lgkmcnt(0) with the other counters left alone (gfx9 encoding).
 __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
+// Wait until at most min(p, MAXP) chunks of NL loads each are outstanding
+// (p <= 0: none); the chunks complete in issue order.
+template <int NL, int MAXP>
+__device__ __forceinline__ void wait_pending(int p) {
+  if constexpr (MAXP >= 3) {
+    if (p >= 3) {
+      wait_vm<3 * NL>();
+      return;
+    }
+  }
+  if constexpr (MAXP >= 2) {
+    if (p >= 2) {
+      wait_vm<2 * NL>();
+      return;
+    }
+  }
+  if (p >= 1)
+    wait_vm<NL>();
+  else
+    wait_vm<0>();
+}
+
 // Pass P: layout change, then its chunks, with three LDS chunk buffers.
 //
 // SPLIT (FP32): on entry F holds the fragments of the pass's
@@ -588,20 +612,22 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       const f4 *cb = lds + buf * buf_quads<PREC>();
       read_x<PREC, 0, 4>(f, cb, o.x);
       read_y<PREC>(f, cb, o.y);
-      if constexpr (Pr::NBUF == 3) {
-        // the loads of chunk j+2 (buffer (j + 2) % 3, free since the last
-        // barrier) go between the two halves of the MFMAs, so their address
-        // arithmetic issues while the matrix core works (measured 0.7% faster
-        // than before the MFMAs, 3% faster than after them)
+      if constexpr (Pr::NBUF >= 3) {
+        // NBUF buffers, AH = NBUF - 1 chunks ahead: the loads of chunk j+AH
+        // (buffer (j + AH) % NBUF, which held chunk j-1 and is free since the
+        // last barrier) go between the two halves of the MFMAs, so their
+        // address arithmetic issues while the matrix core works (measured
+        // 0.7% faster than before the MFMAs, 3% faster than after them)
+        constexpr int AH = Pr::NBUF - 1;
         mma(I0(), I2(), f);
-        if (more) load_ahead<PREC, SP, P, 2>(s0, M, T, G, wave, lane, c + 2, lds_base, buf == 0 ? 2 : buf - 1);
+        if (j + AH < total)
+          load_ahead<PREC, SP, P, AH>(s0, M, T, G, wave, lane, c + AH, lds_base,
+                                      buf + AH < Pr::NBUF ? buf + AH : buf + AH - Pr::NBUF);
         mma(I2(), I4(), f);
-        if (more)
-          wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
-        else
-          wait_vm<0>();
+        // chunk j+1 landed; chunks j+2 .. j+AH (those issued) may be pending
+        wait_pending<NLOAD, AH - 1>(total - j - 2);
         block_barrier();
-        buf = buf == 2 ? 0 : buf + 1;
+        buf = buf + 1 < Pr::NBUF ? buf + 1 : 0;
       } else {
         // two buffers: chunk j+2 goes into this chunk's buffer once every
         // wave has read it (after the barrier), with the whole of chunk
@@ -968,9 +994,9 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     for (int r = 0; r < 64; ++r) acc[r] = 0;
 
     Frag<PREC> F;
-    {  // chunks 0 .. 2 (SPLIT) or 0 .. 1 of the pass sequence into buffers
-       // 0 .. 2 / 0 .. 1; SPLIT: then the fragments of chunk 0
-      constexpr int PRO = Pr::SPLIT ? 3 : 2;
+    {  // chunks 0 .. 2 (SPLIT) or 0 .. NBUF-2 of the pass sequence into
+       // buffers 0 ..; SPLIT: then the fragments of chunk 0
+      constexpr int PRO = Pr::SPLIT ? 3 : (Pr::NBUF > 3 ? Pr::NBUF - 1 : 2);
       const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
 #pragma unroll
       for (int k = 0; k < PRO; ++k)
