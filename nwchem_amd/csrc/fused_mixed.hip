@@ -74,7 +74,7 @@ struct Prec<TCE_T_FP32> {
   static constexpr int KC = 16;
   static constexpr int NPART = 1;
   static constexpr int ES = 4;
-  static constexpr int NBUF = 3;
+  static constexpr int NBUF = 3;  // config 2 kernel ms: 3 buffers 4310, 4 buffers 4349
   static constexpr bool SPLIT = true;  // measured 0.7% faster than the direct loop
   typedef float acc;
 };
@@ -534,17 +534,17 @@ __device__ __forceinline__ void wait_pending(int p) {
 // SPLIT (FP32): on entry F holds the fragments of the pass's
 // first chunk j.  Per chunk: the MFMAs of X quads 0-1; this thread's loads
 // of chunk j+1 and its LDS reads of chunk j complete, then the barrier that
-// publishes chunk j+1 and frees buffer j % 3, which receives the loads of
-// chunk j+3; the fragments of X quads 0-1 of chunk j+1 are read while the
+// publishes chunk j+1 and frees buffer j % NBUF, which receives the loads of
+// chunk j+NBUF; the fragments of X quads 0-1 of chunk j+1 are read while the
 // MFMAs of X quads 2-3 of chunk j run, then the rest of chunk j+1's, so the
 // matrix core never waits for an LDS read after the barrier.
 //
 // Otherwise (FP64, whose 64 double accumulators leave no room to keep a
 // chunk's fragments live across the barrier, and split-BF16, whose hi and lo
 // fragments take 64 registers): per chunk the fragments of chunk j are read
-// and its MFMAs run, with the loads of chunk j+2 (into the buffer freed by
-// the last barrier) issued halfway through them; then this thread's loads
-// of chunk j+1 complete and the barrier publishes it.
+// and its MFMAs run, with the loads of chunk j+NBUF-1 (into the buffer freed
+// by the last barrier) issued halfway through them; then this thread's
+// loads of chunk j+1 complete and the barrier publishes it.
 template <int PREC, bool SP, int P>
 __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], const f4 *lds,
                                           unsigned lds_base, const MixedArgsG &M,
@@ -581,17 +581,17 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
   // chunk loop's live registers to the accumulators, fragments and s0
   const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
   if constexpr (Pr::SPLIT) {
+    constexpr int NB = Pr::NBUF;
     for (int c = 0; c < nch0; ++c, ++j) {
       const bool next = j + 1 < total;
       mma(I0(), I2(), F);
-      if (j + 2 < total)
-        wait_vm<NLOAD>();  // chunk j+1 landed; j+2 may be pending
-      else
-        wait_vm<0>();
+      // chunk j+1 landed; chunks j+2 .. j+NB-1 may be pending
+      wait_pending<NLOAD, NB - 2>(total - j - 2);
       wait_lgkm0();
       block_barrier();
-      if (j + 3 < total) load_ahead<PREC, SP, P, 3>(s0, M, T, G, wave, lane, c + 3, lds_base, buf);
-      const int nbuf = buf == 2 ? 0 : buf + 1;
+      // every wave has read chunk j: its buffer takes chunk j+NB
+      if (j + NB < total) load_ahead<PREC, SP, P, NB>(s0, M, T, G, wave, lane, c + NB, lds_base, buf);
+      const int nbuf = buf + 1 < NB ? buf + 1 : 0;
       const f4 *cbn = lds + nbuf * buf_quads<PREC>();
       // chunk j+1: this pass or the first chunk of pass P+1
       const int2 o = c + 1 < nch0 ? frag_offsets<PREC, P>(wave, lane)
@@ -996,7 +996,7 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     Frag<PREC> F;
     {  // chunks 0 .. 2 (SPLIT) or 0 .. NBUF-2 of the pass sequence into
        // buffers 0 ..; SPLIT: then the fragments of chunk 0
-      constexpr int PRO = Pr::SPLIT ? 3 : (Pr::NBUF > 3 ? Pr::NBUF - 1 : 2);
+      constexpr int PRO = Pr::SPLIT ? Pr::NBUF : (Pr::NBUF > 3 ? Pr::NBUF - 1 : 2);
       const Src<PREC> s0 = pass_src<PREC, 0>(M, G, wave, lane);
 #pragma unroll
       for (int k = 0; k < PRO; ++k)
