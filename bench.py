@@ -70,11 +70,11 @@ def traffic_from_profiles(workload: str, kernel: str):
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
     or None when no summary exists for this kernel and workload."""
     # {kernel: figures} per kernel of the run, keyed by the template name
-    # (fused_triples_mixed_fp64 is fused_triples_mixed<0, false, false>:
-    # precision, tile export, narrowed contraction ranges), in the kernel's
-    # own summary or in the summary of all precisions
+    # (fused_triples_mixed_fp64 is fused_triples_mixed<0, 0, false>:
+    # precision, epilogue (0: the energy), narrowed contraction ranges), in
+    # the kernel's own summary or in the summary of all precisions
     base, _, prec = kernel.rpartition("_")
-    key = f"{base}<{ {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'bf16x6': 3}.get(prec, -1)}, false, false>"
+    key = f"{base}<{ {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'bf16x6': 3}.get(prec, -1)}, 0, false>"
     for name in (kernel, base + "_all_precisions"):
         path = os.path.join(ROOT, "profiles", name + "_pmc_" + workload + ".json")
         if not os.path.exists(path):
