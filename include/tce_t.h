@@ -386,7 +386,10 @@ double tce_t_batch_last_ms(const tce_t_batch *batch);
  * fused_triples_mixed on FP64 (16x16x4); cost_order 1 = costliest work
  * items first; tuned_ms[6] = candidate times of the tuning runs (family x
  * order in trial order, -1 where not tried, all -1 when the plan came from
- * the process-wide cache). */
+ * the process-wide cache).  The tuning runs cover the first eighth of every
+ * job's work items when the batch has at least 64 per CU (all items
+ * otherwise; TCE_T_BATCH_TUNE_FRACTION overrides), so their times are of
+ * that sample. */
 int tce_t_batch_plan(const tce_t_batch *batch, int precision, int *family, int *cost_order,
                      double *tuned_ms);
 /* The work-queue slice width the plan chose for fused_triples_mixed (the

@@ -12,10 +12,12 @@
 //              the segment order (as given; costliest items first).  For
 //              fused_triples_mixed the slice width of its work queues (the
 //              consecutive items an XCD takes at a time: 32, 64 or 128) is
-//              then tuned on the best of those.  Each candidate runs once,
-//              timed with HIP events; the fastest is kept and remembered per
-//              batch shape (the sorted list of job dimensions and tilings)
-//              for later batches in the process.
+//              then tuned on the best of those.  Each candidate runs once
+//              on the same sample (the first eighth of every job's items,
+//              or all of them for small batches), timed with HIP events; the
+//              fastest is kept and remembered per batch shape (the sorted
+//              list of job dimensions and tilings) for later batches in the
+//              process.
 //   graph      the chosen launch sequence is captured once into a hipGraph
 //              (kernel, reductions, device-to-host copies; no memset nodes)
 //              and replayed by every later run.
@@ -26,6 +28,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -102,8 +106,10 @@ std::vector<int> job_order(const Batch &B, int precision, bool cost_order) {
   return ord;
 }
 
-// Build the device table (job arguments, then segments) of a choice.
-int build_table(Batch &B, int precision, const Choice &c, Plan &P) {
+// Build the device table (job arguments, then segments) of a choice; frac < 1
+// keeps only the first ceil(frac * items) work items of every job (the
+// tuning sample: the same items under every candidate).
+int build_table(Batch &B, int precision, const Choice &c, Plan &P, double frac = 1.0) {
   // the 16x16 kernel (reduced precision, or FP64 with family WG16) takes MixedArgs
   const bool mixed = precision != TCE_T_FP64 || c.family == FAM_WG16;
   const bool wide = mixed || c.family == FAM_WG;
@@ -120,8 +126,9 @@ int build_table(Batch &B, int precision, const Choice &c, Plan &P) {
   int nseg = 0;
   for (size_t k = 0; k < n; ++k) {
     const Job &J = *B.jobs[ord[k]];
-    const long items = wide ? J.nitems8 : J.nitems;
-    if (items == 0) continue;
+    const long all = wide ? J.nitems8 : J.nitems;
+    if (all == 0) continue;
+    const long items = frac >= 1.0 ? all : std::max(1L, (long)std::ceil(frac * (double)all));
     char *ha = h.data() + (size_t)nseg * asz;
     if (mixed) {
       MixedArgs M = job_margs(J, precision);
@@ -232,11 +239,22 @@ int plan_batch(Batch &B, int precision) {
     std::vector<Choice> cands;
     for (int fam : fams)
       for (bool co : {true, false}) cands.push_back(Choice{fam, co});
+    // Candidates run on the first eighth of every job's items when that
+    // still gives the GPU 8 items per CU (the same sample under every
+    // candidate); a full run per candidate cost 6-7x the batch itself.
+    // TCE_T_BATCH_TUNE_FRACTION sets the fraction (0 < f <= 1).
+    long items8 = 0;
+    for (const Job *J : B.jobs) items8 += J->nitems8;
+    double frac = items8 >= 64L * B.cus ? 0.125 : 1.0;
+    if (const char *f = getenv("TCE_T_BATCH_TUNE_FRACTION")) {
+      const double x = atof(f);
+      if (x > 0.0 && x <= 1.0) frac = x;
+    }
     float best_ms = 0.f;
     int rc = set_slice(B, 5);
     if (rc != TCE_T_OK) return rc;
     for (size_t i = 0; i < cands.size(); ++i) {
-      rc = build_table(B, precision, cands[i], P);
+      rc = build_table(B, precision, cands[i], P, frac);
       if (rc != TCE_T_OK) return rc;
       float ms = 0.f;
       rc = time_plan(B, precision, P, ms);
@@ -247,7 +265,7 @@ int plan_batch(Batch &B, int precision) {
     // the work-queue slice width of the 16x16 kernel, on the best plan
     if (mixed || best.family == FAM_WG16) {
       P.slice_ms[0] = best_ms;
-      rc = build_table(B, precision, best, P);
+      rc = build_table(B, precision, best, P, frac);
       if (rc != TCE_T_OK) return rc;
       const Choice base = best;
       for (int k = 1; k < NSLICE; ++k) {
