@@ -242,25 +242,8 @@ class SpinSystem:
             L = int(h[0])
             return [(int(h[1 + i]), int(h[1 + L + i])) for i in range(L)]
 
-        y1h, y2h, f1h = T.y1_hash(), T.y2_hash(), T.f1_hash()
-        y1_blocks = []
-        for key, off in table(y1h):
-            h, p = key // nvab + 1, key % nvab + noab + 1
-            y1_blocks.append((off, d["Y1"][sl(h), sl(p)].ravel()))
-        y1 = np.zeros(max(sum(b.size for _, b in y1_blocks), 1))
-        for off, b in y1_blocks:
-            y1[off:off + b.size] = b
-        y2_blocks = []
-        for key, off in table(y2h):
-            p2 = key % nvab + noab + 1
-            q = key // nvab
-            p1 = q % nvab + noab + 1
-            q //= nvab
-            h8, h7 = q % noab + 1, q // noab + 1
-            y2_blocks.append((off, d["Y2"][sl(h7), sl(h8), sl(p1), sl(p2)].ravel()))
-        y2 = np.zeros(max(sum(b.size for _, b in y2_blocks), 1))
-        for off, b in y2_blocks:
-            y2[off:off + b.size] = b
+        y1, y1h, y2, y2h = self.y_blocks(d["Y1"], d["Y2"])
+        f1h = T.f1_hash()
         f_blocks = []
         for key, off in table(f1h):
             g5, g6 = key % N + 1, key // N + 1
@@ -271,6 +254,48 @@ class SpinSystem:
             f1[off:off + b.size] = b
         d.update(y1=y1, y1h=y1h, y2=y2, y2h=y2h, f1=f1, f1h=f1h)
         return d
+
+    def y_blocks(self, Y1, Y2):
+        """Dense Y1[i, a], Y2[i, j, a, b] laid out as the tce_y1_offset /
+        tce_y2_offset tables keep them: (y1, y1h, y2, y2h)."""
+        T = self.tiles
+        noab, nvab = T.noab, T.nvab
+        rng_, ofs = T.range, T.offset
+        no = self.no
+
+        def sl(b):
+            o = ofs[b - 1] - (no if b > noab else 0)
+            return slice(int(o), int(o + rng_[b - 1]))
+
+        def table(h):
+            L = int(h[0])
+            return [(int(h[1 + i]), int(h[1 + L + i])) for i in range(L)]
+
+        y1h, y2h = T.y1_hash(), T.y2_hash()
+        y1_blocks = []
+        for key, off in table(y1h):
+            h, p = key // nvab + 1, key % nvab + noab + 1
+            y1_blocks.append((off, Y1[sl(h), sl(p)].ravel()))
+        y1 = np.zeros(max(sum(b.size for _, b in y1_blocks), 1))
+        for off, b in y1_blocks:
+            y1[off:off + b.size] = b
+        y2_blocks = []
+        for key, off in table(y2h):
+            p2 = key % nvab + noab + 1
+            q = key // nvab
+            p1 = q % nvab + noab + 1
+            q //= nvab
+            h8, h7 = q % noab + 1, q // noab + 1
+            y2_blocks.append((off, Y2[sl(h7), sl(h8), sl(p1), sl(p2)].ravel()))
+        y2 = np.zeros(max(sum(b.size for _, b in y2_blocks), 1))
+        for off, b in y2_blocks:
+            y2[off:off + b.size] = b
+        return y1, y1h, y2, y2h
+
+    def y_equal_t(self):
+        """The amplitudes themselves as Lambda amplitudes: y(i a) = t(a i),
+        y(i j a b) = t(a b i j), in the y tables' layout."""
+        return self.y_blocks(self.T1.T, self.T2.transpose(2, 3, 0, 1))
 
     def dense_lambda(self, L, fock=True, y3f_sign=1.0):
         """Textbook Lambda-CCSD(T) energies over every spin-orbital sextuple,

@@ -157,3 +157,52 @@ def test_lambda_rejects_mismatched_jobs():
         with pytest.raises(RuntimeError, match="y1 lacks"):
             jc.upload_lambda(La["y1"], np.array([0], np.int64), La["y2"], La["y2h"],
                              c.blocks.v2, c.blocks.v2h)
+
+
+@pytest.mark.parametrize("path", ["fused", "tiles"])
+@pytest.mark.parametrize("prm", LAMBDA_GPU, ids=IDS)
+def test_lambda_with_y_equal_t_is_the_triples_energy(prm, path, monkeypatch):
+    """The amplitudes as Lambda amplitudes, no Fock term: the left tiles are
+    the right ones, so energy1 = E[T] and energy2 = E(T) of the same job,
+    task by task."""
+    for k, v in PATHS[path].items():
+        monkeypatch.setenv(k, v)
+    s = spin_systems.SpinSystem(**prm)
+    b = s.blocks
+    y1, y1h, y2, y2h = s.y_equal_t()
+    print(f"lambda y=t case {prm} path={path}", flush=True)
+    with TriplesJob(s.tiles) as right, TriplesJob(s.tiles) as left:
+        right.upload(b)
+        e = right.run()
+        te_t = right.task_energies()
+        left.upload_lambda(y1, y1h, y2, y2h, b.v2, b.v2h)
+        e1, e2, te = lambda_ccsd_t(right, left)
+    np.testing.assert_allclose([e1, e2], e, rtol=1e-12)
+    np.testing.assert_allclose(te, te_t, rtol=1e-11, atol=1e-15)
+
+
+@pytest.mark.timeout(600)
+def test_lambda_at_the_benchmark_size_is_the_triples_energy():
+    """no=40 (tiles of 20), nv=320 (tiles of 40), 480 tasks: with y = t and
+    no Fock term the fused Lambda passes give the job's own E[T] and E(T)
+    for every task, in FP64 and split-BF16 (3 terms).  The job's FP64 task
+    energies at this size are pinned against the reference's compiled
+    kernels (tests/golden/config2_tasks.json)."""
+    import bench
+    from nwchem_amd import tiles as tl
+
+    seed = 20261202
+    tm = tl.TileModel.synthetic(40, 320, 20, 40, seed)
+    b, y1, y1h, y2, y2h, _, _ = bench.lambda_inputs(tm, seed)  # y from the same blocks
+    print("lambda at no=40/nv=320: inputs ready", flush=True)
+    with TriplesJob(tm) as right, TriplesJob(tm) as left:
+        right.upload(b)
+        left.upload_lambda(y1, y1h, y2, y2h, b.v2, b.v2h)
+        for prec in (0, BF16X3):
+            e = right.run(prec)
+            te_t = right.task_energies()
+            print(f"lambda at no=40/nv=320: precision {prec} (T) done", flush=True)
+            e1, e2, te = lambda_ccsd_t(right, left, precision=prec)
+            print(f"lambda at no=40/nv=320: precision {prec} lambda done", flush=True)
+            np.testing.assert_allclose([e1, e2], e, rtol=1e-12)
+            np.testing.assert_allclose(te, te_t, rtol=1e-11, atol=1e-15)

@@ -189,6 +189,22 @@ def test_planted_mistakes_are_caught():
     assert not _close(e2, e1, rel=1e-6)  # the y1 term contributes to energy2 only
 
 
+@pytest.mark.parametrize("restricted", [False, True])
+@pytest.mark.parametrize("prm", SPIN_CASES)
+def test_lambda_with_y_equal_t_is_the_triples_energy(prm, restricted):
+    """With the amplitudes as Lambda amplitudes and no Fock term the left
+    tiles are the right ones: energy1 = E[T], energy2 = E(T), task by task,
+    against the (T) task loop of the oracle."""
+    s = SpinSystem(**prm, flip_symmetric=restricted, restricted=restricted)
+    b = s.blocks
+    y1, y1h, y2, y2h = s.y_equal_t()
+    lam = oracle.lambda_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, y1, y1h, y2, y2h, b.v2,
+                               b.v2h)
+    ref = oracle.ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h)
+    assert _close(lam[0], ref[0]) and _close(lam[1], ref[1]), (lam[:2], ref[:2])
+    np.testing.assert_allclose(lam[2], ref[2], rtol=1e-12, atol=1e-300)
+
+
 def test_pairings_agree_on_one_orbital_tiles_and_differ_otherwise():
     """With one orbital per tile the two element orders coincide, so the
     reference-order pairing equals the orbital pairing; with larger tiles it
