@@ -616,8 +616,9 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
         // NBUF buffers, AH = NBUF - 1 chunks ahead: the loads of chunk j+AH
         // (buffer (j + AH) % NBUF, which held chunk j-1 and is free since the
         // last barrier) go between the two halves of the MFMAs, so their
-        // address arithmetic issues while the matrix core works (measured
-        // 0.7% faster than before the MFMAs, 3% faster than after them)
+        // address arithmetic issues while the matrix core works (three
+        // buffers: 0.7% faster than before the MFMAs, 3% faster than after
+        // them; four buffers, FP64 at config 2: 8200 ms here, 8414 before)
         constexpr int AH = Pr::NBUF - 1;
         mma(I0(), I2(), f);
         if (j + AH < total)
