@@ -315,7 +315,13 @@ int tce_t_job_upload_lambda(tce_t_job *job, const double *y1, const long *k_y1_o
  *   energy1 += factor * tdoubles * ydoubles / D
  *   energy2 += factor * tdoubles * (ysingles + ydoubles) / D
  * task_energies (2 * ntasks doubles, or NULL) receives {energy1, energy2}
- * of each task in loop order.  Synchronous. */
+ * of each task in loop order.  Synchronous.  Device memory: the orbital
+ * pairing (same tiling on both jobs, not TCE_T_TILING_ORBITAL) runs two
+ * passes of the contraction kernel over the jobs' kernel tasks with a
+ * window of at most 16 GiB (a quarter of the free memory) allocated for
+ * the call; the reference order (and ORBITAL-tiled jobs) keep each job's
+ * tile buffer, 16 bytes per element of the largest task tile, as
+ * tce_t_job_triples_tiles does. */
 int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const double *f1,
                         const long *k_f1_offset, int pairing, int precision, double *energy1,
                         double *energy2, double *task_energies, long ntasks);
