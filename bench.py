@@ -593,13 +593,21 @@ def main() -> int:
             job.triples_tiles(t[3], t[4], t[5], t[0], t[1], t[2])
         tw = time.perf_counter() - t1
         ft = sum(n * 18.0 * (args.no + args.nv + 1) for n in elems)
+        # the same tiles left in HBM for a consumer running on the GPU
+        t1 = time.perf_counter()
+        for t in tk:
+            job.triples_tiles_device(t[3], t[4], t[5], t[0], t[1], t[2])
+        tdv = time.perf_counter() - t1
         tiles_out = {"tasks": len(tk), "seconds_per_task": tw / len(tk),
                      "elements_per_task": float(np.mean(elems)),
                      "task_flops_per_s": ft / tw,
-                     "host_bytes_per_s": 16.0 * sum(elems) / tw}
+                     "host_bytes_per_s": 16.0 * sum(elems) / tw,
+                     "device_seconds_per_task": tdv / len(tk),
+                     "device_task_flops_per_s": ft / tdv}
         print(f"[bench] task tiles: {tw / len(tk) * 1e3:.1f} ms per task "
               f"({np.mean(elems):.3g} elements, {ft / tw / 1e12:.1f} TF/s, "
-              f"{16.0 * sum(elems) / tw / 1e9:.1f} GB/s of tiles to the host)",
+              f"{16.0 * sum(elems) / tw / 1e9:.1f} GB/s of tiles to the host); left in HBM: "
+              f"{tdv / len(tk) * 1e3:.1f} ms per task ({ft / tdv / 1e12:.1f} TF/s)",
               file=sys.stderr, flush=True)
     job.close()
     lam = bench_lambda(tm, blocks, args.seed + 1, F_run, wall / args.steps) if args.lam else None

@@ -281,6 +281,16 @@ int tce_t_job_triples_tiles_6dts(tce_t_job *job, int precision, long h1b, long h
                                  long p4b, long p5b, long p6b, long istart, long istop,
                                  long jstart, long jstop, long maxp4, long maxp5,
                                  double *singles, double *doubles);
+/* The same tiles left in device memory, for consumers that run their own
+ * functional on the GPU (no copy over PCIe): *doubles and *singles point to
+ * the task's t3d and t3s in HBM, *elements (may be NULL) doubles each, in
+ * the layout of tce_t_job_triples_tiles (assigned, not accumulated).  The
+ * arrays belong to the job: valid until its next tile, Lambda or upload
+ * call, or its destruction; the tiles are complete when this returns.  No
+ * counterpart in the reference (its drivers only fill host buffers). */
+int tce_t_job_triples_tiles_device(tce_t_job *job, int precision, long h1b, long h2b, long h3b,
+                                   long p4b, long p5b, long p6b, const double **singles,
+                                   const double **doubles, long *elements);
 /* {E[T], E(T)} of each task in the order of ccsd_t_gpu.F's task loop, from
  * the last run.  out has 2*ntasks doubles. */
 int tce_t_job_task_energies(const tce_t_job *job, double *out, long ntasks);

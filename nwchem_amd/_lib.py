@@ -97,6 +97,8 @@ def _declare(L: C.CDLL) -> None:
     L.tce_t_job_run_restart.argtypes = [C.c_void_p, C.c_int, LP, DP, C.c_long, DP]
     L.tce_t_job_triples_tiles.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 6 + [DP, DP]
     L.tce_t_job_triples_tiles_6dts.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 12 + [DP, DP]
+    L.tce_t_job_triples_tiles_device.argtypes = [C.c_void_p, C.c_int] + [C.c_long] * 6 + [
+        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), LP]
     L.tce_t_job_upload_lambda.argtypes = [C.c_void_p, DP, LP, DP, LP, DP, LP]
     L.tce_t_lambda_ccsd_t.argtypes = [C.c_void_p, C.c_void_p, DP, LP, C.c_int, C.c_int, DP, DP,
                                       DP, C.c_long]

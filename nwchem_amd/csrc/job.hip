@@ -1382,6 +1382,30 @@ extern "C" int tce_t_job_triples_tiles(tce_t_job *job, int precision, long h1b, 
                        doubles);
 }
 
+extern "C" int tce_t_job_triples_tiles_device(tce_t_job *job, int precision, long h1b, long h2b,
+                                              long h3b, long p4b, long p5b, long p6b,
+                                              const double **singles, const double **doubles,
+                                              long *elements) {
+  if (!job || !singles || !doubles) return TCE_T_ERR_ARG;
+  Job &J = *reinterpret_cast<Job *>(job);
+  if (p4b < 1 || p5b < 1 || p4b > J.T.noab + J.T.nvab || p5b > J.T.noab + J.T.nvab) {
+    set_error("tile sextuple is not a task of the job");
+    return TCE_T_ERR_ARG;
+  }
+  const long r5 = J.T.rng(p5b);
+  TaskDev d;
+  double *dev = nullptr;
+  const int rc = tile_export_device(J, precision, h1b, h2b, h3b, p4b, p5b, p6b, 0, J.T.rng(p4b), 0,
+                                    r5, r5, d, &dev);
+  if (rc != TCE_T_OK) return rc;
+  HIPCHK(hipStreamSynchronize(J.stream), "fused_triples_tiles");
+  const long n = (long)d.r1 * d.r2 * d.r3 * d.r4 * d.r5 * d.r6;
+  *doubles = dev;
+  *singles = dev + n;
+  if (elements) *elements = n;
+  return TCE_T_OK;
+}
+
 extern "C" int tce_t_job_triples_tiles_6dts(tce_t_job *job, int precision, long h1b, long h2b,
                                             long h3b, long p4b, long p5b, long p6b, long istart,
                                             long istop, long jstart, long jstop, long maxp4,

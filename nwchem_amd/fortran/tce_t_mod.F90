@@ -122,6 +122,18 @@ module tce_t_mi355x
       real(c_double), intent(inout) :: singles(*), doubles(*)
     end function
 
+    ! the same tiles left in device memory (device addresses; valid until the
+    ! job's next tile, Lambda or upload call)
+    integer(c_int) function tce_t_job_triples_tiles_device(job, precision, h1b, h2b, h3b, &
+        p4b, p5b, p6b, singles, doubles, elements) bind(C, name="tce_t_job_triples_tiles_device")
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: job
+      integer(c_int), value :: precision
+      integer(c_long), value :: h1b, h2b, h3b, p4b, p5b, p6b
+      type(c_ptr), intent(out) :: singles, doubles
+      integer(c_long), intent(out) :: elements
+    end function
+
     integer(c_int) function tce_t_job_task_energies(job, out, ntasks) &
         bind(C, name="tce_t_job_task_energies")
       import :: c_int, c_long, c_double, c_ptr

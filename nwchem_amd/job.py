@@ -131,6 +131,18 @@ class TriplesJob:
               "tce_t_job_triples_tiles")
         return s, d
 
+    def triples_tiles_device(self, h1b: int, h2b: int, h3b: int, p4b: int, p5b: int, p6b: int,
+                             precision: int = FP64) -> tuple[int, int, int]:
+        """The same tiles left in HBM: (t3s device address, t3d device
+        address, elements), valid until the job's next tile, Lambda or
+        upload call."""
+        s, d, n = C.c_void_p(), C.c_void_p(), C.c_long(0)
+        check(_lib.lib().tce_t_job_triples_tiles_device(self._h, precision, h1b, h2b, h3b, p4b,
+                                                        p5b, p6b, C.byref(s), C.byref(d),
+                                                        C.byref(n)),
+              "tce_t_job_triples_tiles_device")
+        return s.value, d.value, n.value
+
     def triples_tiles_6dts(self, h1b: int, h2b: int, h3b: int, p4b: int, p5b: int, p6b: int,
                            istart: int, istop: int, jstart: int, jstop: int, maxp4: int,
                            maxp5: int, precision: int = FP64) -> tuple[np.ndarray, np.ndarray]:

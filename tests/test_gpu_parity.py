@@ -717,6 +717,31 @@ def test_triples_tiles_6dts_slices_equal_the_whole_tiles(precision):
             job.triples_tiles_6dts(h1, h2, h3, p4, p5, p6, 1, 3, 1, 1, 2, 1)
 
 
+def test_triples_tiles_left_on_the_device_equal_the_host_tiles():
+    """tce_t_job_triples_tiles_device: the task's tiles in HBM, read back
+    with hipMemcpy, equal what tce_t_job_triples_tiles hands the host, bit
+    for bit, in FP64 and split-BF16."""
+    import ctypes as C
+
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    tm, b, _ = system(7, 13, 3, 5, 11)
+    task = tm.tasks()[2]
+    print(f"device tiles of task {task}", flush=True)
+    with TriplesJob(tm) as job:
+        job.upload(b)
+        for prec in (0, 2):
+            hs, hd = job.triples_tiles(task[3], task[4], task[5], task[0], task[1], task[2], prec)
+            ps, pd, n = job.triples_tiles_device(task[3], task[4], task[5], task[0], task[1],
+                                                 task[2], prec)
+            assert n == hs.size
+            for ptr, want in ((ps, hs), (pd, hd)):
+                got = np.empty(n)
+                assert hip.hipMemcpy(got.ctypes.data, ptr, n * 8, 2) == 0  # device to host
+                assert np.array_equal(got, want.ravel()), prec
+
+
 def test_triples_tiles_accumulate_and_reject_non_tasks():
     """The C entry point adds into the caller's buffers (as the drivers do);
     a sextuple that is not a task, or a job without upload, is an error."""
