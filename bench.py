@@ -370,16 +370,29 @@ def bench_lambda(tm, blocks, seed: int, F_run: float, t_step: float):
     t0 = time.perf_counter()
     _, y1, y1h, y2, y2h, f1, f1h = lambda_inputs(tm, seed)
     t_in = time.perf_counter() - t0
+    modes = {}
     with TriplesJob(tm) as right, TriplesJob(tm) as left:
         right.upload(blocks)
         left.upload_lambda(y1, y1h, y2, y2h, blocks.v2, blocks.v2h)
         t1 = time.perf_counter()
         e1, e2, te = lambda_ccsd_t(right, left, f1, f1h, LAMBDA_ORBITAL)
         wall = time.perf_counter() - t1
+        # split-BF16: the first call converts both jobs' operands (untimed)
+        for name, prec in (("bf16x6", 3), ("bf16x3", 2)):
+            lambda_ccsd_t(right, left, f1, f1h, LAMBDA_ORBITAL, prec)
+            t1 = time.perf_counter()
+            m1, m2, _ = lambda_ccsd_t(right, left, f1, f1h, LAMBDA_ORBITAL, prec)
+            mw = time.perf_counter() - t1
+            modes[name] = {"seconds": mw, "speedup_vs_fp64": wall / mw, "energy1": m1,
+                           "energy2": m2, "dE_rel": max(abs(m1 - e1) / abs(e1),
+                                                        abs(m2 - e2) / abs(e2))}
+            print(f"[bench] lambda-ccsd(t) {name}: {mw:.2f} s ({wall / mw:.2f}x FP64), "
+                  f"relative dE {modes[name]['dE_rel']:.2e}", file=sys.stderr, flush=True)
     out = {"tasks": int(te.shape[0]), "seconds": wall, "energy1": e1, "energy2": e2,
            "flops": 2.0 * F_run, "tflops": 2.0 * F_run / wall / 1e12,
            "frac_of_fp64_peak": 2.0 * F_run / wall / 1e12 / PEAK_FP64_TFLOPS,
-           "vs_two_triples_energies": wall / (2.0 * t_step), "inputs_seconds": t_in}
+           "vs_two_triples_energies": wall / (2.0 * t_step), "inputs_seconds": t_in,
+           "precision_modes": modes}
     print(f"[bench] lambda-ccsd(t): {wall:.2f} s for {out['tasks']} tasks "
           f"({out['tflops']:.1f} TF/s, {wall / (2.0 * t_step):.3f}x two (T) energies), "
           f"energy1={e1:.12e} energy2={e2:.12e}", file=sys.stderr, flush=True)
