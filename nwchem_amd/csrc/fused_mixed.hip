@@ -576,11 +576,11 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
   const int nch0 = chunks_of<PREC, SP>(M, T, P).n;
-  // load sources of this pass; those of the next passes are formed only in
-  // the branches that need them (the pass's last chunks), which keeps the
-  // chunk loop's live registers to the accumulators, fragments and s0
-  const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
   if constexpr (Pr::SPLIT) {
+    // load sources of this pass; those of the next passes are formed only in
+    // the branches that need them (the pass's last chunks), which keeps the
+    // chunk loop's live registers to the accumulators, fragments and s0
+    const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
     constexpr int NB = Pr::NBUF;
     for (int c = 0; c < nch0; ++c, ++j) {
       const bool next = j + 1 < total;
@@ -605,8 +605,22 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
       buf = nbuf;
     }
   } else {
-    const int2 o = frag_offsets<PREC, P>(wave, lane);
+    // Split-BF16: the lane's LDS fragment offsets (and, with two buffers, the
+    // load sources) are formed again in every chunk from a lane index the
+    // compiler cannot see through.  Kept live across the loop they were
+    // spilled, and a reload's s_waitcnt also drains the chunk loads in flight
+    // (vmcnt counts them in issue order).  Config 2 kernel ms, kept live /
+    // formed per chunk (two boxes each): bf16x6 2865-2915 / 2736-2823, bf16x3
+    // 1598-1604 / 1589-1600, FP64 8200 / 8293 (FP64 keeps them live; a few
+    // of its passes reload them).
+    constexpr bool per_chunk = PREC != TCE_T_FP64;
+    constexpr bool src_per_chunk = per_chunk && Pr::NBUF < 3;
+    const Src<PREC> s0 = pass_src<PREC, P>(M, G, wave, lane);
+    const int2 o0 = frag_offsets<PREC, P>(wave, lane);
     for (int c = 0; c < nch0; ++c, ++j) {
+      int ln = lane;
+      if constexpr (per_chunk) asm volatile("" : "+v"(ln));
+      const int2 o = per_chunk ? frag_offsets<PREC, P>(wave, ln) : o0;
       const bool more = j + 2 < total;
       Frag<PREC> f;
       const f4 *cb = lds + buf * buf_quads<PREC>();
@@ -621,9 +635,14 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
         // them; four buffers, FP64 at config 2: 8200 ms here, 8414 before)
         constexpr int AH = Pr::NBUF - 1;
         mma(I0(), I2(), f);
-        if (j + AH < total)
-          load_ahead<PREC, SP, P, AH>(s0, M, T, G, wave, lane, c + AH, lds_base,
-                                      buf + AH < Pr::NBUF ? buf + AH : buf + AH - Pr::NBUF);
+        if (j + AH < total) {
+          const int nb = buf + AH < Pr::NBUF ? buf + AH : buf + AH - Pr::NBUF;
+          if constexpr (src_per_chunk)
+            load_ahead<PREC, SP, P, AH>(pass_src<PREC, P>(M, G, wave, ln), M, T, G, wave, ln,
+                                        c + AH, lds_base, nb);
+          else
+            load_ahead<PREC, SP, P, AH>(s0, M, T, G, wave, lane, c + AH, lds_base, nb);
+        }
         mma(I2(), I4(), f);
         // chunk j+1 landed; chunks j+2 .. j+AH (those issued) may be pending
         wait_pending<NLOAD, AH - 1>(total - j - 2);
@@ -637,7 +656,9 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
         mma(I2(), I4(), f);
         wait_vm<0>();  // chunk j+1 landed (no later chunk is in flight)
         block_barrier();
-        if (more) load_ahead<PREC, SP, P, 2>(s0, M, T, G, wave, lane, c + 2, lds_base, buf);
+        if (more)
+          load_ahead<PREC, SP, P, 2>(pass_src<PREC, P>(M, G, wave, ln), M, T, G, wave, ln, c + 2,
+                                     lds_base, buf);
         buf ^= 1;
       }
     }
