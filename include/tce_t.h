@@ -335,6 +335,37 @@ int tce_t_job_upload_lambda(tce_t_job *job, const double *y1, const long *k_y1_o
 int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const double *f1,
                         const long *k_f1_offset, int pairing, int precision, double *energy1,
                         double *energy2, double *task_energies, long ntasks);
+
+/* ---- CR-CCSD(T) (cr_ccsd_t.F) -------------------------------------------- */
+/* A CR-CCSD(T) job: d_t1/k_t1_offset and d_t2/k_t2_offset as for
+ * tce_t_job_upload, and the two intermediates cr_ccsd_t_N builds before its
+ * task loop (toggle 1, cr_ccsd_t.F:62-66): d_i1_1/k_i1_offset_1, i1(h11 p4
+ * h1 h2) under the table of OFFSET_cr_ccsd_t_N_1_1 (blocks [p4][h11][h1][h2],
+ * h1 tile <= h2 tile), and d_i1_2/k_i1_offset_2, i1(p4 p5 h1 p12) under
+ * OFFSET_cr_ccsd_t_N_2_1 (blocks [p4][p5][h1][p12], p4 tile <= p5 tile).  The
+ * job's t3 doubles are then the moment tile of cr_ccsd_t_N (toggle 2:
+ * -P(9) t(p4 p5 h1 h11) i1(h11 p6 h2 h3) - P(9) t(p4 p12 h1 h2) i1(p5 p6 h3
+ * p12)) and its singles the denominator tile of cr_ccsd_t_E (P(9) t(p4 p5
+ * h1 h2) t(p6 h3) - 2/3 P(9) t(p4 h1) i1(p5 p6 h2 h3), i1 = -1/4 P(4)
+ * t(p4 h1) t(p5 h2)).  Always packed on the device.  TCE_T_ERR_KEY if a
+ * table lacks a block the spin and irrep rules require. */
+int tce_t_job_upload_cr(tce_t_job *job, const double *t1, const long *k_t1_offset,
+                        const double *t2, const long *k_t2_offset, const double *i1_1,
+                        const long *k_i1_offset_1, const double *i1_2,
+                        const long *k_i1_offset_2);
+/* Replaces cr_ccsd_t(..., energy1, energy2) (cr_ccsd_t.F:1-269) after its
+ * intermediates are built: triples holds t1/t2/v2 (tce_t_job_upload), cr
+ * the CR job of the same tiles and tiling (not TCE_T_TILING_ORBITAL).  In
+ * FP64, over the tasks of the (T) loop with w = factor / D:
+ *   num1 = sum w * moment * tdoubles,  num2 = sum w * moment * (tsingles + tdoubles)
+ *   den1 = sum w * den * tdoubles,     den2 = sum w * den * (tsingles + tdoubles)
+ * den0 = the overlap of cr_ccsd_t_D with c = t, and
+ *   energy1 = num1 / (1 + den1 + den0), energy2 = num2 / (1 + den2 + den0).
+ * terms (5 doubles or NULL) receives num1, num2, den0, den1, den2.
+ * Synchronous; two passes of the contraction kernel with the window of
+ * tce_t_lambda_ccsd_t. */
+int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy1, double *energy2,
+                    double *terms);
 /* Host packer of tce_t_job_upload, for inspection and tests: the dense tensors
  * the job places in HBM, built from the same blocks and hash tables.  With
  * Ep = no. of virtuals and Mp = no. of occupieds rounded up to a multiple of 8:

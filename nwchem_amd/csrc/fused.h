@@ -107,6 +107,9 @@ struct MixedArgs {
   // the Fock hole-particle block F[h][p] (or null) the left-hand pass adds.
   const double *lam_y = nullptr, *lam_f = nullptr;
   long lam_item0 = 0;
+  // CR-CCSD(T) right-hand pass: the denominator partials (2 per wave, as
+  // f.partials holds the moment partials).
+  double *cr_den = nullptr;
 };
 
 // A launch covers a table of segments: work items [item0, item1) of the job
@@ -145,6 +148,7 @@ struct MixedArgsG {
   int Ek, Mk;
   gptr<const double> lam_y, lam_f;
   long lam_item0;
+  gptr<double> cr_den;
 };
 static_assert(sizeof(FusedArgsG) == sizeof(FusedArgs), "argument mirror");
 static_assert(sizeof(MixedArgsG) == sizeof(MixedArgs), "argument mirror");
@@ -185,6 +189,7 @@ __device__ __forceinline__ MixedArgsG load_args(const MixedArgs *p) {
   M.lam_y = (gptr<const double>)q->lam_y;
   M.lam_f = (gptr<const double>)q->lam_f;
   M.lam_item0 = q->lam_item0;
+  M.cr_den = (gptr<double>)q->cr_den;
   return M;
 }
 
@@ -415,6 +420,14 @@ int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int 
 // / D with the left-hand values read from args->lam_y at the same slots.
 int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int grid,
                         hipStream_t s, bool sparse, unsigned long long *sched, bool left);
+// The CR-CCSD(T) right-hand pass (FP64): the right-hand job's t3 doubles
+// (the moment M3 of cr_ccsd_t_N) and singles (the denominator tile of
+// cr_ccsd_t_E) against the left-hand job's t3 doubles and singles in the
+// window at args->lam_y: per wave factor * M3 * td / D and factor * M3 *
+// (td + ts) / D into args->f.partials, the same with the denominator tile
+// into args->cr_den.
+int launch_fused_cr_right(const MixedSegment *segs, long total, int grid, hipStream_t s,
+                          bool sparse, unsigned long long *sched);
 // Zeroed counters of the dynamic schedule for one launching stream (every
 // launch leaves them zero; launches that may overlap need their own), or
 // null on failure (error set).  Free with hipFree.

@@ -837,7 +837,7 @@ __device__ __forceinline__ void store_tiles(const AccT (&acc)[64], const double 
 
 // Epilogues of the item loop: each item's energy partials, one task's t3
 // tiles (store_tiles), or the two Lambda-CCSD(T) passes (lambda.hip).
-enum { EPI_ENERGY = 0, EPI_TILES = 1, EPI_LAMBDA_LEFT = 2, EPI_LAMBDA_RIGHT = 3 };
+enum { EPI_ENERGY = 0, EPI_TILES = 1, EPI_LAMBDA_LEFT = 2, EPI_LAMBDA_RIGHT = 3, EPI_CR_RIGHT = 4 };
 
 // Left-hand pass: the wave's y3 doubles (the accumulators plus the Fock
 // term P(c/ab) P(k/ij) y(a b i j) f(k c), y from the job's dense T2e,
@@ -941,6 +941,49 @@ __device__ __forceinline__ void lambda_right(const AccT (&acc)[64], const FusedA
         const double yd = in[r * 64 + lane], ys = in[(64 + r) * 64 + lane];
         e1 += ok ? dn * yd : 0.0;
         e2 += ok ? dn * (yd + ys) : 0.0;
+      }
+}
+
+// CR-CCSD(T) right-hand pass (cr_ccsd_t.F:150-205): with w = factor / D,
+// the right-hand job's doubles r (the moment) and singles q (the
+// denominator tile) against the left-hand t3 doubles td and singles ts of
+// the same elements: s0 += w r td, s1 += w r (td + ts), s2 += w q td,
+// s3 += w q (td + ts).
+template <class AccT>
+__device__ __forceinline__ void cr_right(const AccT (&acc)[64], const double *e,
+                                         const FusedArgsG &A, const TaskDev &T, int sa, int sb,
+                                         int sc, int la, int lb, int lc, int i0, int j0, int k0,
+                                         const ItemGeo &G, gptr<const double> in, int lane,
+                                         double (&s)[4]) {
+  s[0] = s[1] = s[2] = s[3] = 0.0;
+  if (!(la < T.r4 && lb < T.r5 && lc < T.r6)) return;
+  double sv[64];
+  singles_products(e, sa, sb, sc, sv);
+  const int no = A.no;
+  const double denom_0 =
+      -(A.evl[no + T.op4 + la] + A.evl[no + T.op5 + lb] + A.evl[no + T.op6 + lc]);
+  double eh[3][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    eh[0][u] = A.evl[hole(G.I, u)];
+    eh[1][u] = A.evl[hole(G.J, u)];
+    eh[2][u] = A.evl[hole(G.K, u)];
+  }
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const int r = 16 * ii + 4 * jj + kk;
+        const bool ok = i0 + ii < T.r1 && j0 + jj < T.r2 && k0 + kk < T.r3;
+        const double w = T.factor / (eh[0][ii] + eh[1][jj] + eh[2][kk] + denom_0);
+        const double rd = (double)acc[r] * w, rs = sv[r] * w;
+        const double td = in[r * 64 + lane], ts = in[(64 + r) * 64 + lane];
+        s[0] += ok ? rd * td : 0.0;
+        s[1] += ok ? rd * (td + ts) : 0.0;
+        s[2] += ok ? rs * td : 0.0;
+        s[3] += ok ? rs * (td + ts) : 0.0;
       }
 }
 
@@ -1052,6 +1095,18 @@ __global__ void __launch_bounds__(512) fused_triples_mixed(const MixedSegment *s
     } else if constexpr (EPI == EPI_LAMBDA_LEFT) {
       lambda_left(acc, epi, M, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, G,
                   A.partials + ((item - M.lam_item0) * 8 + wave) * 8192, lane);
+    } else if constexpr (EPI == EPI_CR_RIGHT) {
+      double cs[4];
+      cr_right(acc, epi, A, T, sa, sb, sc, a0 + sa, b0 + sb, c0 + sc, 4 * bi, 4 * bj, 4 * bk, G,
+               M.lam_y + ((item - M.lam_item0) * 8 + wave) * 8192, lane, cs);
+      wave_sum(cs[0], cs[1]);
+      wave_sum(cs[2], cs[3]);
+      if (lane == 0) {
+        A.partials[2 * (item * 8 + wave)] = cs[0];
+        A.partials[2 * (item * 8 + wave) + 1] = cs[1];
+        M.cr_den[2 * (item * 8 + wave)] = cs[2];
+        M.cr_den[2 * (item * 8 + wave) + 1] = cs[3];
+      }
     } else {
       double e4, e5;
       if constexpr (EPI == EPI_LAMBDA_RIGHT)
@@ -1105,6 +1160,14 @@ static int launch_prec(const MixedSegment *segs, int nseg, long total, int preci
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int EPI>
+static int launch_prec_fp64(const MixedSegment *segs, long total, int grid, bool sparse,
+                            hipStream_t s, unsigned long long *q) {
+  sparse ? launch_one<TCE_T_FP64, EPI, true>(segs, 1, total, grid, s, q)
+         : launch_one<TCE_T_FP64, EPI, false>(segs, 1, total, grid, s, q);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        hipStream_t s, bool sparse, unsigned long long *sched) {
   return launch_prec<EPI_ENERGY>(segs, nseg, total, precision, grid, sparse, s, sched);
@@ -1119,6 +1182,11 @@ int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int
                         hipStream_t s, bool sparse, unsigned long long *sched, bool left) {
   return left ? launch_prec<EPI_LAMBDA_LEFT>(segs, 1, total, precision, grid, sparse, s, sched)
               : launch_prec<EPI_LAMBDA_RIGHT>(segs, 1, total, precision, grid, sparse, s, sched);
+}
+
+int launch_fused_cr_right(const MixedSegment *segs, long total, int grid, hipStream_t s,
+                          bool sparse, unsigned long long *sched) {
+  return launch_prec_fp64<EPI_CR_RIGHT>(segs, total, grid, sparse, s, sched);
 }
 
 

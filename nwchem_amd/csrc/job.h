@@ -40,6 +40,7 @@ struct Job {
          *dS = nullptr, *devl = nullptr, *dpart = nullptr, *dtaskE = nullptr;
   TaskDev *dtasks = nullptr;
   bool uploaded = false;
+  bool cr = false;  // the last upload was a CR-CCSD(T) job (tce_t_job_upload_cr)
   long gen = 0;  // completed uploads: every upload replaces all device buffers
   float last_ms = 0.f;
   std::vector<double> taskE;
@@ -103,8 +104,13 @@ FusedArgs job_args(const Job &J, bool wide);
 MixedArgs job_margs(const Job &J, int precision);
 // Dense tensors of an upload built on the device from the canonical blocks
 // (pack.hip); the dense buffers must be allocated.  Returns a status.
+// PACK_CR: a CR-CCSD(T) job, Wd1 from the moment intermediate i1(h11 p4 h1
+// h2) in v2/v2h, Vd2 from i1(p4 p5 h1 p12) in x2/x2h (cr_ccsd_t_N.F), S from
+// t2 and t1 (the denominator tile of cr_ccsd_t_E.F).
+enum { PACK_TRIPLES = 0, PACK_CR = 1 };
 int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, const long *t2h,
-                   const double *v2, const long *v2h);
+                   const double *v2, const long *v2h, int kind = PACK_TRIPLES,
+                   const double *x2 = nullptr, const long *x2h = nullptr);
 // Free every device buffer of the job (it can be uploaded again).
 void job_release(Job &J);
 // Per-item cost weight (contraction chunks of an item) for scheduling.

@@ -744,9 +744,11 @@ static int pack_dense(const Tiles &T, long Ep, long Mp, const double *t1, const 
   return TCE_T_OK;
 }
 
-extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1h, const double *t2,
-                                const long *t2h, const double *v2, const long *v2h) {
-  if (!job || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
+// The upload of a (T) job (kind PACK_TRIPLES) or of a CR-CCSD(T) job
+// (PACK_CR: v2 and x2 hold the two moment intermediates).
+static int job_upload(tce_t_job *job, const double *t1, const long *t1h, const double *t2,
+                      const long *t2h, const double *v2, const long *v2h, int kind,
+                      const double *x2, const long *x2h) {
   Job &J = *reinterpret_cast<Job *>(job);
   const Tiles &T = J.T;
   // a new upload replaces the tensors and everything derived from them
@@ -766,7 +768,8 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
   // referenced ranges of the three files; TCE_T_PACK=host builds them with
   // the host packer instead (same values bit for bit) and copies them over.
   const char *pk = getenv("TCE_T_PACK");
-  if (pk && !strcmp(pk, "host")) {
+  J.cr = kind == PACK_CR;
+  if (pk && !strcmp(pk, "host") && kind == PACK_TRIPLES) {
     DenseHost D;
     const int rc = pack_dense(T, J.Ep, J.Mp, t1, t1h, t2, t2h, v2, v2h, D);
     if (rc != TCE_T_OK) return rc;
@@ -784,7 +787,7 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
         !alloc(J.dT2m, no * nv * nv * J.Mp, "T2m") || !alloc(J.dVd2, no * nv * nv * J.Ep, "Vd2") ||
         !alloc(J.dWd1, no * no * nv * J.Mp, "Wd1") || !alloc(J.dS, no * no * nv * nv, "S"))
       return TCE_T_ERR_HIP;
-    const int rc = pack_on_device(J, t1, t1h, t2, t2h, v2, v2h);
+    const int rc = pack_on_device(J, t1, t1h, t2, t2h, v2, v2h, kind, x2, x2h);
     if (rc != TCE_T_OK) return rc;
   }
   if (!up(J.devl, T.evl, "evl")) return TCE_T_ERR_HIP;
@@ -812,6 +815,31 @@ extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1
     J.dbound8 = J.dbound;
   }
   return TCE_T_OK;
+}
+
+extern "C" int tce_t_job_upload(tce_t_job *job, const double *t1, const long *t1h, const double *t2,
+                                const long *t2h, const double *v2, const long *v2h) {
+  if (!job || !t1 || !t1h || !t2 || !t2h || !v2 || !v2h) return TCE_T_ERR_ARG;
+  return job_upload(job, t1, t1h, t2, t2h, v2, v2h, PACK_TRIPLES, nullptr, nullptr);
+}
+
+// A CR-CCSD(T) job (cr_ccsd_t.F): t1 and t2 as a (T) job's, the doubles
+// operands from the two intermediates that cr_ccsd_t_N builds before its
+// task loop (toggle 1: d_i1_1 with k_i1_offset_1, i1(h11 p4 h1 h2); d_i1_2
+// with k_i1_offset_2, i1(p4 p5 h1 p12)).  Its t3 doubles are the moment
+// tile of cr_ccsd_t_N (toggle 2), its singles the denominator tile of
+// cr_ccsd_t_E; tce_t_cr_ccsd_t pairs them with a (T) job's.
+extern "C" int tce_t_job_upload_cr(tce_t_job *job, const double *t1, const long *k_t1_offset,
+                                   const double *t2, const long *k_t2_offset, const double *i1_1,
+                                   const long *k_i1_offset_1, const double *i1_2,
+                                   const long *k_i1_offset_2) {
+  if (!job || !t1 || !k_t1_offset || !t2 || !k_t2_offset || !i1_1 || !k_i1_offset_1 || !i1_2 ||
+      !k_i1_offset_2) {
+    set_error("tce_t_job_upload_cr: null argument");
+    return TCE_T_ERR_ARG;
+  }
+  return job_upload(job, t1, k_t1_offset, t2, k_t2_offset, i1_1, k_i1_offset_1, PACK_CR, i1_2,
+                    k_i1_offset_2);
 }
 
 // Room for one job's launch table: arguments and one segment, FP64 or

@@ -168,7 +168,10 @@ int fock_hole_particle(const Tiles &T, const double *f1, const long *f1h,
 // order, the right-hand pass forms the energy partials of the same items
 // from its t3 doubles and the window; the job's task reduction then gives
 // the per-task sums.  No tile leaves the registers in the tile layout.
-int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<double> &te) {
+// cr_den (CR-CCSD(T), FP64): the right-hand pass is the CR one; te gets the
+// moment sums, *te_den the denominator sums, per task.
+int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<double> &te,
+                 double *cr_den = nullptr, std::vector<double> *te_den = nullptr) {
   for (Job *J : {&JR, &JL}) {
     int rc = ensure_tasks8(*J);
     if (rc == TCE_T_OK && precision != TCE_T_FP64) rc = prepare_mixed(*J, precision);
@@ -214,33 +217,70 @@ int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<
     MixedArgs MR = job_margs(JR, precision);
     MR.lam_y = win;
     MR.lam_item0 = i0;
+    MR.cr_den = cr_den;
     const MixedSegment *sr = stage_mixed(JR, MR, i1, i0);
-    if (!sr || launch_fused_lambda(sr, i1 - i0, precision, grid, JR.stream, JR.sparse,
-                                   sched_use(JR.dsched), false) != 0) {
+    if (!sr || (cr_den ? launch_fused_cr_right(sr, i1 - i0, grid, JR.stream, JR.sparse,
+                                               sched_use(JR.dsched))
+                       : launch_fused_lambda(sr, i1 - i0, precision, grid, JR.stream, JR.sparse,
+                                             sched_use(JR.dsched), false)) != 0) {
       set_error("lambda right pass: launch failed");
       rc = TCE_T_ERR_HIP;
       break;
     }
     if (!hip_ok(hipStreamSynchronize(JR.stream), "lambda right pass")) rc = TCE_T_ERR_HIP;
   }
-  if (rc == TCE_T_OK) {
-    te.assign(2 * nt, 0.0);
-    if (launch_task_reduce(job_bounds(JR, true), (int)nt, 8, JR.dpart8, JR.dtaskE, JR.stream) != 0) {
+  // per-task sums of one partials buffer, in the order of the task list
+  auto reduce = [&](const double *part, std::vector<double> &out) {
+    out.assign(2 * nt, 0.0);
+    if (launch_task_reduce(job_bounds(JR, true), (int)nt, 8, part, JR.dtaskE, JR.stream) != 0) {
       set_error("task_reduce launch failed");
       rc = TCE_T_ERR_HIP;
-    } else if (!hip_ok(hipMemcpyAsync(te.data(), JR.dtaskE, 2 * nt * sizeof(double),
+    } else if (!hip_ok(hipMemcpyAsync(out.data(), JR.dtaskE, 2 * nt * sizeof(double),
                                       hipMemcpyDeviceToHost, JR.stream),
                        "D2H") ||
                !hip_ok(hipStreamSynchronize(JR.stream), "lambda task sums")) {
       rc = TCE_T_ERR_HIP;
     }
-  }
+  };
+  if (rc == TCE_T_OK) reduce(JR.dpart8, te);
+  if (rc == TCE_T_OK && cr_den) reduce(cr_den, *te_den);
   if (rc != TCE_T_OK) {  // nothing left in flight on either stream
     (void)hipStreamSynchronize(JL.stream);
     (void)hipStreamSynchronize(JR.stream);
   }
   (void)hipFree(win);
   return rc;
+}
+
+// The overlap den0 of cr_ccsd_t_D.F:4-7 with c = t (cr_ccsd_t.F:58-59):
+//   sum_(a i) t(a i)^2 + 1/4 sum_(a b i j) t(a b i j)^2
+//   + 1/2 sum_(a b i j) t(a i) t(b j) t(a b i j),
+// all spin-orbital sums, from the job's dense T1[a][i] and T2e[i][j][a][b]
+// (b padded to Ep).  One partial per block, summed on the host in order.
+constexpr int OV_BLOCKS = 1024, OV_THREADS = 256;
+__global__ void __launch_bounds__(OV_THREADS)
+    cr_overlap(const double *T1, const double *T2e, int no, int nv, long Ep, double *part) {
+  __shared__ double red[OV_THREADS];
+  const long stride = (long)gridDim.x * OV_THREADS, x0 = (long)blockIdx.x * OV_THREADS + threadIdx.x;
+  double acc = 0.0;
+  for (long x = x0; x < (long)nv * no; x += stride) acc += T1[x] * T1[x];
+  const long n2 = (long)no * no * nv * nv;
+  for (long x = x0; x < n2; x += stride) {
+    const int b = (int)(x % nv);
+    long r = x / nv;
+    const int a = (int)(r % nv);
+    r /= nv;
+    const int j = (int)(r % no), i = (int)(r / no);
+    const double t = T2e[(((long)i * no + j) * nv + a) * Ep + b];
+    acc += 0.25 * t * t + 0.5 * T1[(long)a * no + i] * T1[(long)b * no + j] * t;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = OV_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 }  // namespace
@@ -438,5 +478,79 @@ extern "C" int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const doub
   release();
   if (energy1) *energy1 = s1;
   if (energy2) *energy2 = s2;
+  return TCE_T_OK;
+}
+
+// cr_ccsd_t (cr_ccsd_t.F:1-269) from a (T) job and a CR job of the same
+// tiles: the task loop's four sums over the t3 tiles of the (T) job (left
+// pass) and the moment and denominator tiles of the CR job (right pass),
+// den0 from the CR job's t1 and t2, then energy1 = num1 / (1 + den1 +
+// den0), energy2 = num2 / (1 + den2 + den0) (cr_ccsd_t.F:254-258).
+extern "C" int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy1,
+                               double *energy2, double *terms) {
+  if (!triples || !cr || triples == cr) {
+    set_error("tce_t_cr_ccsd_t: two distinct jobs are needed");
+    return TCE_T_ERR_ARG;
+  }
+  Job &JL = *reinterpret_cast<Job *>(triples);
+  Job &JR = *reinterpret_cast<Job *>(cr);
+  if (!JR.uploaded || !JL.uploaded) {
+    set_error("tce_t_cr_ccsd_t before the uploads");
+    return TCE_T_ERR_STATE;
+  }
+  if (JL.cr || !JR.cr) {
+    set_error("tce_t_cr_ccsd_t: a (T) job (tce_t_job_upload) and a CR job (tce_t_job_upload_cr) are needed, in that order");
+    return TCE_T_ERR_ARG;
+  }
+  if (!same_tiles(JR.T, JL.T) || JR.device != JL.device || JR.tiling != JL.tiling) {
+    set_error("tce_t_cr_ccsd_t: the jobs' tiles, tilings (or devices) differ");
+    return TCE_T_ERR_ARG;
+  }
+  if (JR.tiling == TCE_T_TILING_ORBITAL) {
+    set_error("tce_t_cr_ccsd_t: not available for the orbital tiling");
+    return TCE_T_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(JR.device), "device");
+  const long nt = JR.ntask();
+  int rc = ensure_tasks8(JR);
+  if (rc != TCE_T_OK) return rc;
+  double *den = nullptr, *ov = nullptr;
+  HIPCHK(hipMalloc(&den, std::max(1L, JR.nitems8) * 16 * sizeof(double)), "CR partials");
+  if (!hip_ok(hipMalloc(&ov, OV_BLOCKS * sizeof(double)), "CR overlap")) {
+    (void)hipFree(den);
+    return TCE_T_ERR_HIP;
+  }
+  std::vector<double> te, td, hov(OV_BLOCKS);
+  rc = lambda_fused(JR, JL, nullptr, TCE_T_FP64, te, den, &td);
+  if (rc == TCE_T_OK) {
+    hipLaunchKernelGGL(cr_overlap, dim3(OV_BLOCKS), dim3(OV_THREADS), 0, JR.stream, JR.dT1,
+                       JR.dT2e, (int)JR.T.no, (int)JR.T.nv, JR.Ep, ov);
+    if (!hip_ok(hipGetLastError(), "cr_overlap launch") ||
+        !hip_ok(hipMemcpyAsync(hov.data(), ov, OV_BLOCKS * sizeof(double), hipMemcpyDeviceToHost,
+                               JR.stream),
+                "D2H") ||
+        !hip_ok(hipStreamSynchronize(JR.stream), "cr_overlap"))
+      rc = TCE_T_ERR_HIP;
+  }
+  (void)hipFree(den);
+  (void)hipFree(ov);
+  if (rc != TCE_T_OK) return rc;
+  double num1 = 0.0, num2 = 0.0, den1 = 0.0, den2 = 0.0, den0 = 0.0;
+  for (long t = 0; t < nt; ++t) {
+    num1 += te[2 * t];
+    num2 += te[2 * t + 1];
+    den1 += td[2 * t];
+    den2 += td[2 * t + 1];
+  }
+  for (double x : hov) den0 += x;
+  if (terms) {
+    terms[0] = num1;
+    terms[1] = num2;
+    terms[2] = den0;
+    terms[3] = den1;
+    terms[4] = den2;
+  }
+  if (energy1) *energy1 = num1 / (1.0 + den1 + den0);
+  if (energy2) *energy2 = num2 / (1.0 + den2 + den0);
   return TCE_T_OK;
 }

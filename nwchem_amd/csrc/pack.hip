@@ -111,7 +111,10 @@ __global__ void __launch_bounds__(PACK_THREADS)
 }
 
 // Wd1[h][g][p][m] (m padded to Mp) = <m p||h g> from v(H7 P6 H2 H3) with the
-// hole pair (H2, H3) sorted, sign -1 when swapped.
+// hole pair (H2, H3) sorted, sign -1 when swapped.  PM: the block holds
+// [P6][H7][H2][H3] instead, as cr_ccsd_t_N.F:749-752 leaves the moment
+// intermediate i1(h11 p4 h1 h2).
+template <bool PM>
 __global__ void __launch_bounds__(PACK_THREADS)
     pack_wd1(PackGeo G, long Mp, const long *tab, const double *src, double *out) {
   const long n = (long)G.no * G.no * G.nv * Mp;
@@ -134,10 +137,11 @@ __global__ void __launch_bounds__(PACK_THREADS)
       continue;
     }
     const bool sw = H2 > H3;
-    const long r6 = G.rng[P6], r2 = G.rng[H2], r3 = G.rng[H3];
+    const long r6 = G.rng[P6], r7 = G.rng[H7], r2 = G.rng[H2], r3 = G.rng[H3];
     const long lm = G.loc[m], lp = G.loc[G.no + p], lh = G.loc[h], lg = G.loc[g];
     const long ih = sw ? lg : lh, ig = sw ? lh : lg, c2 = sw ? r2 : r3;
-    const double v = src[off + ((lm * r6 + lp) * (sw ? r3 : r2) + ih) * c2 + ig];
+    const long pair = PM ? lp * r7 + lm : lm * r6 + lp;
+    const double v = src[off + (pair * (sw ? r3 : r2) + ih) * c2 + ig];
     out[x] = sw ? -v : v;
   }
 }
@@ -202,6 +206,27 @@ __global__ void __launch_bounds__(PACK_THREADS)
   }
 }
 
+// S of a CR-CCSD(T) job: S[h][g][p][q] = t(p q h g) + (t(p h) t(q g) -
+// t(p g) t(q h)) / 3 from the t2 blocks and the packed T1[p][h], so that the
+// job's singles P(9) t(a i) S(b c j k) are the denominator tile of
+// cr_ccsd_t_E.F:7-9, P(9) t(p4 p5 h1 h2) t(p6 h3) - 2/3 P(9) t(p4 h1)
+// i1(p5 p6 h2 h3) with i1 = -1/4 P(4) t(p4 h1) t(p5 h2) (the P(9) sums
+// agree after moving the single t to the front: even permutations).
+__global__ void __launch_bounds__(PACK_THREADS)
+    pack_s_cr(PackGeo G, const long *tab, const double *src, const double *T1, double *out) {
+  const long n = (long)G.no * G.no * G.nv * G.nv;
+  for (long x = gid(); x < n; x += gstride()) {
+    const int q = (int)(x % G.nv);
+    long r = x / G.nv;
+    const int p = (int)(r % G.nv);
+    r /= G.nv;
+    const int g = (int)(r % G.no), h = (int)(r / G.no);
+    const double ph = T1[(long)p * G.no + h], qg = T1[(long)q * G.no + g],
+                 pg = T1[(long)p * G.no + g], qh = T1[(long)q * G.no + h];
+    out[x] = t2_value(G, tab, src, p, q, h, g) + (ph * qg - pg * qh) / 3.0;
+  }
+}
+
 // Block tables of one upload: per tensor, the file offset of the canonical
 // block behind each tile tuple (-1: none), and the extent (doubles) that the
 // gather reads from it.
@@ -249,7 +274,9 @@ struct Runs {
 // referenced file ranges to the device, gathers on the job's stream.  The
 // dense buffers J.dT1 .. J.dS must be allocated; returns a status.
 int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, const long *t2h,
-                   const double *v2, const long *v2h) {
+                   const double *v2, const long *v2h, int kind, const double *x2,
+                   const long *x2h) {
+  const bool cr = kind == PACK_CR;
   const Tiles &T = J.T;
   const long no = T.no, nv = T.nv, noab = T.noab, nvab = T.nvab, N = noab + nvab;
   bool missing = false;
@@ -291,7 +318,10 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
           long g3 = H7, g4 = P6, g1 = std::min(H2, H3), g2 = std::max(H2, H3);
           const bool need = allowed4(T, g3, g4, g1, g2);
           restrict4(T, g3, g4, g1, g2);
-          wtab.add(look(v2h, vkey(g3, g4, g1, g2), need), T.rng(H7) * T.rng(P6) * T.rng(H2) * T.rng(H3));
+          // CR: i1(h11 p4 h1 h2) keyed as OFFSET_cr_ccsd_t_N_1_1 (cr_ccsd_t_N.F:800-830)
+          const long key = cr ? g2 - 1 + noab * (g1 - 1 + noab * (g3 - 1 + noab * (g4 - noab - 1)))
+                              : vkey(g3, g4, g1, g2);
+          wtab.add(look(v2h, key, need), T.rng(H7) * T.rng(P6) * T.rng(H2) * T.rng(H3));
         }
   // Vd2: v(P5 P6 H3 P7), (P5, P6) sorted
   Table vtab;
@@ -302,11 +332,18 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
           long g3 = std::min(P5, P6), g4 = std::max(P5, P6), g1 = H3, g2 = P7;
           const bool need = allowed4(T, g3, g4, g1, g2);
           restrict4(T, g3, g4, g1, g2);
-          vtab.add(look(v2h, vkey(g3, g4, g1, g2), need), T.rng(P5) * T.rng(P6) * T.rng(H3) * T.rng(P7));
+          // CR: i1(p4 p5 h1 p12) keyed as OFFSET_cr_ccsd_t_N_2_1, same block order
+          // [P5][P6][H3][P7] (cr_ccsd_t_N.F:3990-3996)
+          if (cr)
+            vtab.add(look(x2h, g2 - noab - 1 + nvab * (g1 - 1 + noab * (g4 - noab - 1 + nvab * (g3 - noab - 1))),
+                          need),
+                     T.rng(P5) * T.rng(P6) * T.rng(H3) * T.rng(P7));
+          else
+            vtab.add(look(v2h, vkey(g3, g4, g1, g2), need), T.rng(P5) * T.rng(P6) * T.rng(H3) * T.rng(P7));
         }
-  // S: v(P5 P6 H2 H3), both pairs sorted
+  // S: v(P5 P6 H2 H3), both pairs sorted (CR: from t2 and t1, pack_s_cr)
   Table stab;
-  for (long P5 = noab + 1; P5 <= N; ++P5)
+  for (long P5 = noab + 1; P5 <= (cr ? noab : N); ++P5)
     for (long P6 = noab + 1; P6 <= N; ++P6)
       for (long H2 = 1; H2 <= noab; ++H2)
         for (long H3 = 1; H3 <= noab; ++H3) {
@@ -321,17 +358,23 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
   }
 
   // the referenced ranges of each file, compacted on the device
-  Runs r1, r2, rv;
+  // (CR: Vd2's blocks from the second intermediate, file x)
+  Runs r1, r2, rv, rx;
   r1.build({&t1tab});
   r2.build({&t2tab});
-  rv.build({&wtab, &vtab, &stab});
+  if (cr) {
+    rv.build({&wtab});
+    rx.build({&vtab});
+  } else {
+    rv.build({&wtab, &vtab, &stab});
+  }
   for (long &o : t1tab.off)
     if (o >= 0) o = r1.remap(o);
   for (long &o : t2tab.off)
     if (o >= 0) o = r2.remap(o);
   for (Table *t : {&wtab, &vtab, &stab})
     for (long &o : t->off)
-      if (o >= 0) o = rv.remap(o);
+      if (o >= 0) o = (cr && t == &vtab) ? rx.remap(o) : rv.remap(o);
 
   // one device allocation: maps, tables, then the three compacted files
   std::vector<int> tile(T.norb), loc(T.norb), rng(N + 1, 0);
@@ -351,7 +394,8 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
     b_tabs += al(std::max<size_t>(1, tabs[k]->off.size()) * sizeof(long));
   }
   const size_t f1_at = b_maps + b_tabs, f2_at = f1_at + al(r1.total * 8),
-               fv_at = f2_at + al(r2.total * 8), bytes = fv_at + al(rv.total * 8);
+               fv_at = f2_at + al(r2.total * 8), fx_at = fv_at + al(rv.total * 8),
+               bytes = fx_at + al(rx.total * 8);
   std::vector<char> head(f1_at, 0);
   std::memcpy(head.data(), tile.data(), T.norb * sizeof(int));
   std::memcpy(head.data() + T.norb * sizeof(int), loc.data(), T.norb * sizeof(int));
@@ -415,6 +459,7 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
   stream_file(r1, t1, f1_at);
   stream_file(r2, t2, f2_at);
   stream_file(rv, v2, fv_at);
+  if (cr) stream_file(rx, x2, fx_at);
   if (rc == TCE_T_OK) {
     PackGeo G;
     G.tile = (const int *)d;
@@ -427,14 +472,20 @@ int pack_on_device(Job &J, const double *t1, const long *t1h, const double *t2, 
     const long *tb[5];
     for (int k = 0; k < 5; ++k) tb[k] = (const long *)(d + tab_at[k]);
     const double *f1 = (const double *)(d + f1_at), *f2 = (const double *)(d + f2_at),
-                 *fv = (const double *)(d + fv_at);
+                 *fv = (const double *)(d + fv_at), *fx = (const double *)(d + fx_at);
     const dim3 grid(PACK_BLOCKS), blk(PACK_THREADS);
     hipLaunchKernelGGL(pack_t1, grid, blk, 0, J.stream, G, tb[0], f1, J.dT1);
     hipLaunchKernelGGL(pack_t2e, grid, blk, 0, J.stream, G, J.Ep, tb[1], f2, J.dT2e);
     hipLaunchKernelGGL(pack_t2m, grid, blk, 0, J.stream, G, J.Mp, tb[1], f2, J.dT2m);
-    hipLaunchKernelGGL(pack_wd1, grid, blk, 0, J.stream, G, J.Mp, tb[2], fv, J.dWd1);
-    hipLaunchKernelGGL(pack_vd2, grid, blk, 0, J.stream, G, J.Ep, tb[3], fv, J.dVd2);
-    hipLaunchKernelGGL(pack_s, grid, blk, 0, J.stream, G, tb[4], fv, J.dS);
+    if (cr) {
+      hipLaunchKernelGGL(pack_wd1<true>, grid, blk, 0, J.stream, G, J.Mp, tb[2], fv, J.dWd1);
+      hipLaunchKernelGGL(pack_vd2, grid, blk, 0, J.stream, G, J.Ep, tb[3], fx, J.dVd2);
+      hipLaunchKernelGGL(pack_s_cr, grid, blk, 0, J.stream, G, tb[1], f2, J.dT1, J.dS);
+    } else {
+      hipLaunchKernelGGL(pack_wd1<false>, grid, blk, 0, J.stream, G, J.Mp, tb[2], fv, J.dWd1);
+      hipLaunchKernelGGL(pack_vd2, grid, blk, 0, J.stream, G, J.Ep, tb[3], fv, J.dVd2);
+      hipLaunchKernelGGL(pack_s, grid, blk, 0, J.stream, G, tb[4], fv, J.dS);
+    }
     if (!hip_ok(hipGetLastError(), "pack launch")) rc = TCE_T_ERR_HIP;
   }
   // the copies read the staging buffers (and `head`) until they complete

@@ -84,6 +84,20 @@ class TriplesJob:
         ptrs = [a.ctypes.data_as(DP if a.dtype == np.float64 else LP) for a in arrs]
         check(_lib.lib().tce_t_job_upload_lambda(self._h, *ptrs), "tce_t_job_upload_lambda")
 
+    def upload_cr(self, t1: np.ndarray, t1h: np.ndarray, t2: np.ndarray, t2h: np.ndarray,
+                  i1_1: np.ndarray, i1_1h: np.ndarray, i1_2: np.ndarray,
+                  i1_2h: np.ndarray) -> None:
+        """A CR-CCSD(T) job: t1, t2 and the two intermediates cr_ccsd_t_N
+        builds before its task loop (i1(h11 p4 h1 h2) with its
+        OFFSET_cr_ccsd_t_N_1_1 table, i1(p4 p5 h1 p12) with its
+        OFFSET_cr_ccsd_t_N_2_1 table).  The job's t3 doubles are the moment
+        tile, its singles the denominator tile of cr_ccsd_t_E."""
+        arrs = [np.ascontiguousarray(a, dtype=dt) for a, dt in
+                ((t1, np.float64), (t1h, np.int64), (t2, np.float64), (t2h, np.int64),
+                 (i1_1, np.float64), (i1_1h, np.int64), (i1_2, np.float64), (i1_2h, np.int64))]
+        ptrs = [a.ctypes.data_as(DP if a.dtype == np.float64 else LP) for a in arrs]
+        check(_lib.lib().tce_t_job_upload_cr(self._h, *ptrs), "tce_t_job_upload_cr")
+
     def run(self, precision: int = FP64) -> tuple[float, float]:
         e1, e2 = C.c_double(0.0), C.c_double(0.0)
         check(_lib.lib().tce_t_job_run(self._h, precision, C.byref(e1), C.byref(e2)), "tce_t_job_run")
@@ -254,6 +268,19 @@ def lambda_ccsd_t(right: TriplesJob, left: TriplesJob, f1: np.ndarray | None = N
                                          C.byref(e1), C.byref(e2), te.ctypes.data_as(DP),
                                          right.ntasks), "tce_t_lambda_ccsd_t")
     return e1.value, e2.value, te
+
+
+def cr_ccsd_t(triples: TriplesJob, cr: TriplesJob) -> tuple[float, float, dict]:
+    """cr_ccsd_t.F: (energy1, energy2, terms) from a (T) job (upload) and a
+    CR job of the same tiles (upload_cr); terms = num1, num2, den0, den1,
+    den2, with energy1 = num1 / (1 + den1 + den0) and energy2 = num2 /
+    (1 + den2 + den0).  FP64."""
+    e1, e2 = C.c_double(0.0), C.c_double(0.0)
+    terms = np.zeros(5)
+    check(_lib.lib().tce_t_cr_ccsd_t(triples._h, cr._h, C.byref(e1), C.byref(e2),
+                                     terms.ctypes.data_as(DP)), "tce_t_cr_ccsd_t")
+    return e1.value, e2.value, dict(zip(("num1", "num2", "den0", "den1", "den2"),
+                                        (float(x) for x in terms)))
 
 
 FAMILIES = {0: "fused_triples", 1: "fused_triples_wg", 2: "fused_triples_mixed",

@@ -338,3 +338,129 @@ class SpinSystem:
         e1 = float(np.sum(t3d * yd / D)) / 36.0
         e2 = float(np.sum(t3d * (ys + yd) / D)) / 36.0
         return e1, e2
+
+    # ---- CR-CCSD(T) inputs ----------------------------------------------
+
+    def i1_tables(self):
+        """The block tables of the two moment intermediates, as
+        OFFSET_cr_ccsd_t_N_1_1 (i1(h11 p4 h1 h2): loops p4, h11, h1, h2 >= h1;
+        key h2-1 + noab (h1-1 + noab (h11-1 + noab (p4-noab-1))), block
+        [p4][h11][h1][h2]) and OFFSET_cr_ccsd_t_N_2_1 (i1(p4 p5 h1 p12): loops
+        p4, p5 >= p4, h1, p12; key p12-noab-1 + nvab (h1-1 + noab (p5-noab-1 +
+        nvab (p4-noab-1))), block [p4][p5][h1][p12]) build them
+        (cr_ccsd_t_N.F:773-830, 4011-4070): spin and irrep conservation, no
+        all-beta block when restricted.  Returns ((hash, [(key, offset, tiles)])
+        for each)."""
+        T = self.tiles
+        noab, nvab = T.noab, T.nvab
+        N = noab + nvab
+        spin, sym, rng_ = T.spin, T.sym, T.range
+
+        def ok(a, b, c, d):
+            sp = spin[a - 1] + spin[b - 1] + spin[c - 1] + spin[d - 1]
+            return ((not T.restricted or sp != 8)
+                    and spin[a - 1] + spin[b - 1] == spin[c - 1] + spin[d - 1]
+                    and (sym[a - 1] ^ sym[b - 1] ^ sym[c - 1] ^ sym[d - 1]) == 0)
+
+        def build(entries):
+            L = len(entries)
+            h = np.zeros(1 + 2 * L, dtype=np.int64)
+            h[0] = L
+            out, size = [], 0
+            for n, (key, tiles) in enumerate(entries):
+                h[1 + n], h[1 + L + n] = key, size
+                out.append((key, size, tiles))
+                size += int(np.prod([rng_[b - 1] for b in tiles]))
+            return h, out, size
+
+        e1 = [(h2 - 1 + noab * (h1 - 1 + noab * (h11 - 1 + noab * (p4 - noab - 1))),
+               (p4, h11, h1, h2))
+              for p4 in range(noab + 1, N + 1) for h11 in range(1, noab + 1)
+              for h1 in range(1, noab + 1) for h2 in range(h1, noab + 1) if ok(h11, p4, h1, h2)]
+        e2 = [(p12 - noab - 1 + nvab * (h1 - 1 + noab * (p5 - noab - 1 + nvab * (p4 - noab - 1))),
+               (p4, p5, h1, p12))
+              for p4 in range(noab + 1, N + 1) for p5 in range(p4, N + 1)
+              for h1 in range(1, noab + 1) for p12 in range(noab + 1, N + 1) if ok(p4, p5, h1, p12)]
+        return build(e1), build(e2)
+
+    def i1_blocks(self, I1, I2):
+        """Dense I1[m, c, j, k] = i1(h11=m p4=c h1=j h2=k) and I2[b, c, k, e]
+        = i1(p4=b p5=c h1=k p12=e) as the two tables keep them: (i1_1,
+        i1_1h, i1_2, i1_2h)."""
+        T = self.tiles
+        noab = T.noab
+        rng_, ofs = T.range, T.offset
+        no = self.no
+
+        def sl(b):
+            o = ofs[b - 1] - (no if b > noab else 0)
+            return slice(int(o), int(o + rng_[b - 1]))
+
+        (h1, e1, n1), (h2, e2, n2) = self.i1_tables()
+        a1, a2 = np.zeros(max(n1, 1)), np.zeros(max(n2, 1))
+        for _, off, (p4, h11, hh1, hh2) in e1:
+            blk = I1[sl(h11), sl(p4), sl(hh1), sl(hh2)].transpose(1, 0, 2, 3).ravel()
+            a1[off:off + blk.size] = blk
+        for _, off, (p4, p5, hh1, p12) in e2:
+            blk = I2[sl(p4), sl(p5), sl(hh1), sl(p12)].ravel()
+            a2[off:off + blk.size] = blk
+        return a1, h1, a2, h2
+
+    def cr_inputs(self):
+        """Random moment intermediates with the symmetry cr_ccsd_t_N gives
+        them (antisymmetric in the pair the tables keep sorted) and their
+        blocks: dict(I1, I2, i1_1, i1_1h, i1_2, i1_2h)."""
+        mk = self._make
+        I1 = mk("ovoo", (1, 1, -1, -1), [(2, 3)], 0.05)
+        I2 = mk("vvov", (1, 1, -1, -1), [(0, 1)], 0.05)
+        a1, h1, a2, h2 = self.i1_blocks(I1, I2)
+        return dict(I1=I1, I2=I2, i1_1=a1, i1_1h=h1, i1_2=a2, i1_2h=h2)
+
+    def dense_cr(self, I1, I2, den_sign=1.0):
+        """Textbook CR-CCSD(T) over every spin-orbital sextuple, from the term
+        lists of cr_ccsd_t_N.F:8,26, cr_ccsd_t_E.F:7-9 and cr_ccsd_t_D.F:4-7
+        (c = t, cr_ccsd_t.F:58-59), with P(x/yz) as in dense_t3 and P(4) the
+        four signed exchanges of (a/b)(i/j):
+            m   = -P(c/ab) P(i/jk) sum_m t(a b i m) I1(m c j k)
+                  -P(a/bc) P(k/ij) sum_e t(a e i j) I2(b c k e)
+            q   = P(c/ab) P(k/ij) t(a b i j) t(c k) - 2/3 P(a/bc) P(i/jk) t(a i) x(b c j k),
+                  x(b c j k) = -1/4 P(4) t(b j) t(c k)
+            den0 = sum t(a i)^2 + 1/4 sum t(a b i j)^2 + 1/2 sum t(a i) t(b j) t(a b i j)
+        paired with the (T) tiles (cr_ccsd_t.F:150-205):
+            num1 = 1/36 sum m td / D, num2 = 1/36 sum m (ts + td) / D,
+            den1, den2 the same with q; energy_n = num_n / (1 + den_n + den0).
+        den_sign flips the second term of q (a planted mistake)."""
+        def tr(x, pa, ph):
+            return np.transpose(x, tuple(pa) + tuple(3 + q for q in ph))
+
+        id3 = ((0, 1, 2), 1.0)
+        p_a = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]
+        p_c = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]
+        h_k = [id3, ((2, 1, 0), -1.0), ((0, 2, 1), -1.0)]
+        h_i = [id3, ((1, 0, 2), -1.0), ((2, 1, 0), -1.0)]
+
+        def P(x, pp, hh):
+            return sum(sa * sh * tr(x, pa, ph) for pa, sa in pp for ph, sh in hh)
+
+        t1, t2 = self.T1, self.T2
+        m = P(-np.einsum("abim,mcjk->abcijk", t2, I1), p_c, h_i)
+        m = m + P(-np.einsum("aeij,bcke->abcijk", t2, I2), p_a, h_k)
+        tt = np.einsum("bj,ck->bcjk", t1, t1)
+        x = -0.25 * (tt - tt.transpose(1, 0, 2, 3) - tt.transpose(0, 1, 3, 2)
+                     + tt.transpose(1, 0, 3, 2))
+        q = P(np.einsum("abij,ck->abcijk", t2, t1), p_c, h_k)
+        q = q + den_sign * (-2.0 / 3.0) * P(np.einsum("ai,bcjk->abcijk", t1, x), p_a, h_i)
+        den0 = (float(np.sum(t1 * t1)) + 0.25 * float(np.sum(t2 * t2))
+                + 0.5 * float(np.einsum("ai,bj,abij->", t1, t1, t2)))
+        ts, td = self.dense_t3()
+        eo, ev = self.eo, self.ev
+        D = (-ev[:, None, None, None, None, None] - ev[None, :, None, None, None, None]
+             - ev[None, None, :, None, None, None] + eo[None, None, None, :, None, None]
+             + eo[None, None, None, None, :, None] + eo[None, None, None, None, None, :])
+        out = dict(num1=float(np.sum(m * td / D)) / 36.0,
+                   num2=float(np.sum(m * (ts + td) / D)) / 36.0,
+                   den1=float(np.sum(q * td / D)) / 36.0,
+                   den2=float(np.sum(q * (ts + td) / D)) / 36.0, den0=den0)
+        out["energy1"] = out["num1"] / (1.0 + out["den1"] + den0)
+        out["energy2"] = out["num2"] / (1.0 + out["den2"] + den0)
+        return out
