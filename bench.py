@@ -23,7 +23,9 @@ contractions in FP32 and in 3-term split-BF16, each timed the same way on the
 same resident inputs, with energies/s, the speedup over FP64 and |dE| in
 micro-Eh against the FP64 result of the run; and the Lambda-CCSD(T)
 energies of lambda_ccsd_t.F over the same tiles (`--lambda`), a second
-seeded amplitude set taken as the Lambda amplitudes.
+seeded amplitude set taken as the Lambda amplitudes; and CR-CCSD(T)
+(`--cr`), with the bare integrals as cr_ccsd_t_N's intermediates so that its
+moment sums equal E[T] and E(T) of the run.
 """
 from __future__ import annotations
 
@@ -399,6 +401,87 @@ def bench_lambda(tm, blocks, seed: int, F_run: float, t_step: float):
     return out
 
 
+def cr_identity_inputs(tm, blocks):
+    """The two moment intermediates of cr_ccsd_t_N set to their first terms,
+    the bare integrals: i1(h11 p4 h1 h2) = v(h11 p4 h1 h2) and i1(p4 p5 h1
+    p12) = v(p4 p5 h1 p12), copied from the job's v2 blocks into the tables
+    of OFFSET_cr_ccsd_t_N_1_1 (blocks [p4][h11][h1][h2]) and _2_1 (blocks
+    [p4][p5][h1][p12]).  The moment tile is then the (T) doubles tile, so
+    num1 = E[T] and num2 = E(T) at any size."""
+    noab, r = tm.noab, tm.range
+    nvab = len(r) - noab
+    N = noab + nvab
+    spin, sym = tm.spin, tm.sym
+
+    def table(h):
+        n = int(h[0])
+        return dict((int(h[1 + i]), int(h[1 + n + i])) for i in range(n))
+
+    def ok(a, b, c, d):
+        sp = spin[a - 1] + spin[b - 1] + spin[c - 1] + spin[d - 1]
+        return ((not tm.restricted or sp != 8)
+                and spin[a - 1] + spin[b - 1] == spin[c - 1] + spin[d - 1]
+                and (sym[a - 1] ^ sym[b - 1] ^ sym[c - 1] ^ sym[d - 1]) == 0)
+
+    v2 = table(blocks.v2h)
+
+    def vkey(a, b, c, d):
+        return d - 1 + N * (c - 1 + N * (b - 1 + N * (a - 1)))
+
+    def build(entries, src_key, perm):
+        keys, offs, size = [], [], 0
+        for key, tl4 in entries:
+            keys.append(key)
+            offs.append(size)
+            size += int(np.prod([r[x - 1] for x in tl4]))
+        out = np.zeros(max(size, 1))
+        for (key, tl4), off in zip(entries, offs):
+            shp = [int(r[x - 1]) for x in src_key(tl4)]
+            n = int(np.prod(shp))
+            so = v2[vkey(*src_key(tl4))]
+            out[off:off + n] = blocks.v2[so:so + n].reshape(shp).transpose(perm).ravel()
+        h = np.array([len(keys)] + keys + offs, dtype=np.int64)
+        return out, h
+
+    e1 = [(h2 - 1 + noab * (h1 - 1 + noab * (h11 - 1 + noab * (p4 - noab - 1))), (p4, h11, h1, h2))
+          for p4 in range(noab + 1, N + 1) for h11 in range(1, noab + 1)
+          for h1 in range(1, noab + 1) for h2 in range(h1, noab + 1) if ok(h11, p4, h1, h2)]
+    e2 = [(p12 - noab - 1 + nvab * (h1 - 1 + noab * (p5 - noab - 1 + nvab * (p4 - noab - 1))),
+           (p4, p5, h1, p12))
+          for p4 in range(noab + 1, N + 1) for p5 in range(p4, N + 1)
+          for h1 in range(1, noab + 1) for p12 in range(noab + 1, N + 1) if ok(p4, p5, h1, p12)]
+    i1_1, i1_1h = build(e1, lambda t: (t[1], t[0], t[2], t[3]), (1, 0, 2, 3))
+    i1_2, i1_2h = build(e2, lambda t: t, (0, 1, 2, 3))
+    return i1_1, i1_1h, i1_2, i1_2h
+
+
+def bench_cr(tm, blocks, e_bt: float, e_pt: float, t_step: float):
+    """tce_t_cr_ccsd_t over every task of the job (FP64): the left pass over
+    the (T) job, the CR right pass over the CR job (moment and denominator
+    tiles), with the bare integrals as intermediates so that num1 and num2
+    are the job's E[T] and E(T); reported against two (T) energies' time."""
+    from nwchem_amd import TriplesJob
+    from nwchem_amd.job import cr_ccsd_t
+
+    t0 = time.perf_counter()
+    i1_1, i1_1h, i1_2, i1_2h = cr_identity_inputs(tm, blocks)
+    t_in = time.perf_counter() - t0
+    with TriplesJob(tm) as tj, TriplesJob(tm) as cj:
+        tj.upload(blocks)
+        cj.upload_cr(blocks.t1, blocks.t1h, blocks.t2, blocks.t2h, i1_1, i1_1h, i1_2, i1_2h)
+        t1 = time.perf_counter()
+        e1, e2, terms = cr_ccsd_t(tj, cj)
+        wall = time.perf_counter() - t1
+    rel = max(abs(terms["num1"] - e_bt) / abs(e_bt), abs(terms["num2"] - e_pt) / abs(e_pt))
+    out = {"seconds": wall, "energy1": e1, "energy2": e2, "terms": terms,
+           "vs_two_triples_energies": wall / (2.0 * t_step), "inputs_seconds": t_in,
+           "num_vs_triples_rel": rel}
+    print(f"[bench] cr-ccsd(t): {wall:.2f} s ({wall / (2.0 * t_step):.3f}x two (T) energies), "
+          f"num1/num2 against E[T]/E(T): rel. diff {rel:.2e}, energy1={e1:.12e} "
+          f"energy2={e2:.12e}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
                 rates: dict | None = None) -> dict:
     """Serving: a Poisson stream (rate lam) of requests for a pool of seeded
@@ -513,6 +596,8 @@ def main() -> int:
                     help="time tce_t_job_triples_tiles on this many tasks (0: skip)")
     ap.add_argument("--lambda", dest="lam", type=int, default=1,
                     help="also time tce_t_lambda_ccsd_t over the whole job (1) or skip it (0)")
+    ap.add_argument("--cr", type=int, default=1,
+                    help="also time tce_t_cr_ccsd_t over the whole job (1) or skip it (0)")
     ap.add_argument("--modes", default="fp32,bf16x3,bf16x6",
                     help="reduced-precision modes to time after FP64 (comma list; '' for none)")
     args = ap.parse_args()
@@ -624,6 +709,7 @@ def main() -> int:
               file=sys.stderr, flush=True)
     job.close()
     lam = bench_lambda(tm, blocks, args.seed + 1, F_run, wall / args.steps) if args.lam else None
+    cr = bench_cr(tm, blocks, e_bt, e_pt, wall / args.steps) if args.cr else None
     reftile = None
     if args.reference_tiling:
         # the same job with the reference's tiles as kernel tasks: the work the
@@ -709,6 +795,7 @@ def main() -> int:
         "precision_modes": modes or None,
         "task_tiles": tiles_out,
         "lambda_ccsd_t": lam,
+        "cr_ccsd_t": cr,
         "batch": batch,
         "batch_small_jobs": batch_small,
         "serving": serve,

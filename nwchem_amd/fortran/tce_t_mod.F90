@@ -168,6 +168,28 @@ module tce_t_mi355x
       integer(c_long), value :: ntasks
     end function
 
+    ! A CR-CCSD(T) job: d_t1, d_t2 and the two intermediates cr_ccsd_t_N
+    ! builds with toggle 1 (d_i1_1/k_i1_offset_1, d_i1_2/k_i1_offset_2)
+    integer(c_int) function tce_t_job_upload_cr(job, t1, k_t1_offset, t2, k_t2_offset, &
+                                                i1_1, k_i1_offset_1, i1_2, k_i1_offset_2) &
+        bind(C, name="tce_t_job_upload_cr")
+      import :: c_int, c_long, c_double, c_ptr
+      type(c_ptr), value :: job
+      real(c_double), intent(in) :: t1(*), t2(*), i1_1(*), i1_2(*)
+      integer(c_long), intent(in) :: k_t1_offset(*), k_t2_offset(*)
+      integer(c_long), intent(in) :: k_i1_offset_1(*), k_i1_offset_2(*)
+    end function
+
+    ! energy1, energy2 of cr_ccsd_t from a (T) job and a CR job; terms(5) =
+    ! num1, num2, den0, den1, den2
+    integer(c_int) function tce_t_cr_ccsd_t(triples, cr, energy1, energy2, terms) &
+        bind(C, name="tce_t_cr_ccsd_t")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: triples, cr
+      real(c_double), intent(out) :: energy1, energy2
+      real(c_double), intent(out) :: terms(*)
+    end function
+
     integer(c_int) function tce_t_job_destroy(job) bind(C, name="tce_t_job_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: job
