@@ -5,7 +5,8 @@
 ! E[T], E(T) once more from the task tiles the library hands back, summed
 ! here as the tiles' other consumers (cr_ccsd_t.F, lambda_ccsd_t.F) do;
 ! then the Lambda-CCSD(T) energies (lambda_ccsd_t.F) of a second seeded
-! amplitude set taken as the Lambda amplitudes.
+! amplitude set taken as the Lambda amplitudes; then CR-CCSD(T)
+! (cr_ccsd_t.F) with the bare integrals as its two moment intermediates.
 ! Usage: tce_t_fdemo [no nv tile_o tile_v]   (default 8 32 8 16)
 !        tce_t_fdemo intorb   (host only: the orbital-form integral input of a
 !        closed-shell tiling through tce_t_v2_alpha_offset and
@@ -141,6 +142,7 @@ program tce_t_fdemo
   end do
   write(*, '(A,ES24.16,A,ES24.16)') 'tiles E[T]=', ta, ' E(T)=', tb
   call lambda_demo()
+  call cr_demo()
   rc = tce_t_job_destroy(job)
 
 contains
@@ -217,6 +219,104 @@ contains
     write(*, '(A,ES24.16,A,ES24.16)') 'lambda energy1=', l1, ' energy2=', l2
     rc = tce_t_job_destroy(left)
   end subroutine lambda_demo
+
+  ! CR-CCSD(T) with the bare integrals as cr_ccsd_t_N's two intermediates:
+  ! i1(h11 p4 h1 h2) = v(h11 p4 h1 h2) under the table of
+  ! OFFSET_cr_ccsd_t_N_1_1 (blocks [p4][h11][h1][h2]) and i1(p4 p5 h1 p12) =
+  ! v(p4 p5 h1 p12) under OFFSET_cr_ccsd_t_N_2_1 (blocks [p4][p5][h1][p12]),
+  ! copied from the v2 blocks (every block is allowed: one spin, one irrep).
+  ! The moment sums num1, num2 are then E[T], E(T) of the job above.
+  subroutine cr_demo()
+    integer(c_long), allocatable :: hi1(:), hi2(:)
+    real(c_double), allocatable :: i1a(:), i1b(:)
+    integer(c_long) :: la, lb, sa, sb, k, off, vo, m, a, ii, jj, nb
+    integer(c_long) :: p4, p5, h11, hh1, hh2, p12, ra, rm, ri, rj
+    type(c_ptr) :: crj
+    real(c_double) :: c1, c2, terms(5)
+    la = nvab * noab * noab * (noab + 1) / 2
+    lb = nvab * (nvab + 1) / 2 * noab * nvab
+    allocate(hi1(2 * la + 1), hi2(2 * lb + 1))
+    hi1(1) = la
+    hi2(1) = lb
+    k = 0
+    sa = 0
+    do p4 = noab + 1, n
+      do h11 = 1, noab
+        do hh1 = 1, noab
+          do hh2 = hh1, noab
+            k = k + 1
+            hi1(1 + k) = hh2 - 1 + noab * (hh1 - 1 + noab * (h11 - 1 + noab * (p4 - noab - 1)))
+            hi1(1 + la + k) = sa
+            sa = sa + range(p4) * range(h11) * range(hh1) * range(hh2)
+          end do
+        end do
+      end do
+    end do
+    k = 0
+    sb = 0
+    do p4 = noab + 1, n
+      do p5 = p4, n
+        do hh1 = 1, noab
+          do p12 = noab + 1, n
+            k = k + 1
+            hi2(1 + k) = p12 - noab - 1 + nvab * (hh1 - 1 + noab * (p5 - noab - 1 + nvab * (p4 - noab - 1)))
+            hi2(1 + lb + k) = sb
+            sb = sb + range(p4) * range(p5) * range(hh1) * range(p12)
+          end do
+        end do
+      end do
+    end do
+    allocate(i1a(max(sa, 1_c_long)), i1b(max(sb, 1_c_long)))
+    k = 0
+    do p4 = noab + 1, n
+      do h11 = 1, noab
+        do hh1 = 1, noab
+          do hh2 = hh1, noab
+            k = k + 1
+            off = hi1(1 + la + k)
+            vo = tce_t_hash(hv, hh2 - 1 + n * (hh1 - 1 + n * (p4 - 1 + n * (h11 - 1))))
+            ra = range(p4)
+            rm = range(h11)
+            ri = range(hh1)
+            rj = range(hh2)
+            do a = 0, ra - 1
+              do m = 0, rm - 1
+                do ii = 0, ri - 1
+                  do jj = 0, rj - 1
+                    i1a(off + ((a * rm + m) * ri + ii) * rj + jj + 1) = &
+                        v2(vo + ((m * ra + a) * ri + ii) * rj + jj + 1)
+                  end do
+                end do
+              end do
+            end do
+          end do
+        end do
+      end do
+    end do
+    k = 0
+    do p4 = noab + 1, n
+      do p5 = p4, n
+        do hh1 = 1, noab
+          do p12 = noab + 1, n
+            k = k + 1
+            off = hi2(1 + lb + k)
+            vo = tce_t_hash(hv, p12 - 1 + n * (hh1 - 1 + n * (p5 - 1 + n * (p4 - 1))))
+            nb = range(p4) * range(p5) * range(hh1) * range(p12)
+            i1b(off + 1:off + nb) = v2(vo + 1:vo + nb)
+          end do
+        end do
+      end do
+    end do
+    rc = tce_t_job_create(tiles, crj)
+    if (rc /= TCE_T_OK) stop 13
+    rc = tce_t_job_upload_cr(crj, t1, h1, t2, h2, i1a, hi1, i1b, hi2)
+    if (rc /= TCE_T_OK) stop 14
+    rc = tce_t_cr_ccsd_t(job, crj, c1, c2, terms)
+    if (rc /= TCE_T_OK) stop 15
+    write(*, '(A,5ES24.16)') 'cr terms=', terms
+    write(*, '(A,ES24.16,A,ES24.16)') 'cr energy1=', c1, ' energy2=', c2
+    rc = tce_t_job_destroy(crj)
+  end subroutine cr_demo
 
   ! Closed-shell tiling as tce_tile.F lays it out with intorb: alpha holes
   ! (3, 2), beta holes, alpha particles (4, 5, 3), beta particles; k_b2am

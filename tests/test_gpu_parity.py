@@ -265,6 +265,17 @@ def test_fortran_caller_matches_oracle():
     want = oracle.lambda_ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, y1, y1h, y2, y2h, b.v2, b.v2h, f1,
                                 f1h, 0)
     np.testing.assert_allclose([float(lam.group(1)), float(lam.group(2))], want[:2], rtol=1e-11)
+    # CR-CCSD(T) from Fortran with the bare integrals as the intermediates (the
+    # caller lays out the two i1 tables): num1, num2 are E[T], E(T); the energies
+    # follow from the five printed terms as cr_ccsd_t.F:254-258 forms them
+    cr = re.search(r"cr terms=\s*(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)", out)
+    ce = re.search(r"cr energy1=\s*(\S+)\s+energy2=\s*(\S+)", out)
+    assert cr and ce, out
+    num1, num2, den0, den1, den2 = (float(x) for x in cr.groups())
+    assert abs(num1 - e1) <= tol(e1) and abs(num2 - e2) <= tol(e2)
+    np.testing.assert_allclose([float(ce.group(1)), float(ce.group(2))],
+                               [num1 / (1.0 + den1 + den0), num2 / (1.0 + den2 + den0)],
+                               rtol=1e-14)
 
 
 # ---- golden vectors (reference kernels, tests/golden) and spin/irrep/restricted tilings
