@@ -537,3 +537,147 @@ def lambda_ccsd_t(tiles: Tiles, t1, t1h, t2, t2h, y1, y1h, y2, y2h, v2, v2h, f1=
         te[n, 0] = f * np.sum(td * yd / D)
         te[n, 1] = f * np.sum(td * (ys + yd) / D)
     return float(te[:, 0].sum()), float(te[:, 1].sum()), te
+
+
+# ---- CR-CCSD(T) -------------------------------------------------------------
+
+def _dense_t1(tiles: Tiles, t1, t1h):
+    """T[a, i] over every orbital from t1-layout blocks ([p][h]; all-beta
+    blocks from their alpha image in a restricted run)."""
+    L = lib()
+    noab, nvab = tiles.noab, tiles.nvab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    nv = int(r[noab:].sum())
+    T = np.zeros((nv, no))
+    sp, sy = tiles.spin, tiles.sym
+
+    def sl(b):
+        s = int(o[b - 1]) - (no if b > noab else 0)
+        return slice(s, s + int(r[b - 1]))
+
+    for p in range(noab + 1, noab + nvab + 1):
+        for h in range(1, noab + 1):
+            if sp[p - 1] != sp[h - 1] or sy[p - 1] ^ sy[h - 1]:
+                continue
+            q = (p, h)
+            if tiles.restricted and sp[p - 1] + sp[h - 1] == 4:
+                q = (int(tiles.alpha[p - 1]), int(tiles.alpha[h - 1]))
+            off = L.ora_hash(_lp(t1h), q[1] - 1 + noab * (q[0] - noab - 1))
+            if off < 0:
+                raise ValueError("t1 lacks a block")
+            n = int(r[p - 1] * r[h - 1])
+            T[sl(p), sl(h)] = t1[off:off + n].reshape(int(r[p - 1]), int(r[h - 1]))
+    return T
+
+
+def cr_operands(tiles: Tiles, t1, t1h, t2, t2h, v2h, i1_1, i1_1h, i1_2, i1_2h):
+    """The operands of cr_ccsd_t.F's extra tiles as a v2-layout array over the
+    table v2h, so that ccsd_t_singles / ccsd_t_doubles over it give them:
+      <h p||h h> block (h11, p4, h1, h2): i1(h11 p4 h1 h2) of cr_ccsd_t_N
+        (table OFFSET_cr_ccsd_t_N_1_1, block [p4][h11][h1][h2] after the sorts
+        of cr_ccsd_t_N.F:739-752);
+      <p p||h p> block (p4, p5, h1, p12): i1(p4 p5 h1 p12) (OFFSET_cr_ccsd_t_N_2_1,
+        block [p4][p5][h1][p12], cr_ccsd_t_N.F:3990-3996);
+      <p p||h h> block: S = t(p q h g) + (t(p h) t(q g) - t(p g) t(q h)) / 3,
+        the denominator of cr_ccsd_t_E.F:7-9 (P(9) t t2 - 2/3 P(9) t i1,
+        i1 = -1/4 P(4) t t) with its single t moved to the front.
+    Doubles over the array are the moment tile (cr_ccsd_t_N.F:8,26),
+    singles the denominator tile."""
+    L = lib()
+    noab, nvab = tiles.noab, tiles.nvab
+    N = noab + nvab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    T1 = _dense_t1(tiles, t1, t1h)
+    T2 = _dense_y2(tiles, t2, t2h)
+    S = T2 + (np.einsum("ph,qg->pqhg", T1, T1) - np.einsum("pg,qh->pqhg", T1, T1)) / 3.0
+
+    def sl(b):
+        s = int(o[b - 1]) - (no if b > noab else 0)
+        return slice(s, s + int(r[b - 1]))
+
+    n = int(v2h[0])
+    ent = [(int(v2h[1 + k]), int(v2h[1 + n + k])) for k in range(n)]
+    size = max([off + int(np.prod([r[b - 1] for b in _vtiles(key, N)])) for key, off in ent] + [1])
+    out = np.zeros(size)
+    for key, off in ent:
+        g3, g4, g1, g2 = _vtiles(key, N)
+        shp = tuple(int(r[b - 1]) for b in (g3, g4, g1, g2))
+        m = int(np.prod(shp))
+        if g3 <= noab < g4 and g1 <= noab and g2 <= noab:
+            k1 = g2 - 1 + noab * (g1 - 1 + noab * (g3 - 1 + noab * (g4 - noab - 1)))
+            s1 = L.ora_hash(_lp(i1_1h), k1)
+            if s1 < 0:
+                raise ValueError("i1(h11 p4 h1 h2) lacks a block")
+            blk = i1_1[s1:s1 + m].reshape(shp[1], shp[0], shp[2], shp[3]).transpose(1, 0, 2, 3)
+        elif g3 > noab and g4 > noab and g1 <= noab < g2:
+            k2 = g2 - noab - 1 + nvab * (g1 - 1 + noab * (g4 - noab - 1 + nvab * (g3 - noab - 1)))
+            s2 = L.ora_hash(_lp(i1_2h), k2)
+            if s2 < 0:
+                raise ValueError("i1(p4 p5 h1 p12) lacks a block")
+            blk = i1_2[s2:s2 + m].reshape(shp)
+        elif g3 > noab and g4 > noab and g1 <= noab and g2 <= noab:
+            blk = S[sl(g3), sl(g4), sl(g1), sl(g2)]
+        else:
+            continue
+        out[off:off + m] = blk.ravel()
+    return out
+
+
+def _vtiles(key: int, N: int):
+    """(g3, g4, g1, g2) of a tce_mo2e_offset key g2-1 + N (g1-1 + N (g4-1 + N (g3-1)))."""
+    g2 = key % N + 1
+    key //= N
+    g1 = key % N + 1
+    key //= N
+    return key // N + 1, key % N + 1, g1, g2
+
+
+def cr_ccsd_t(tiles: Tiles, t1, t1h, t2, t2h, v2, v2h, i1_1, i1_1h, i1_2, i1_2h):
+    """cr_ccsd_t.F:70-258 after its intermediates exist: per task of the (T)
+    loop the (T) tiles ts, td (ccsd_t_singles_l / ccsd_t_doubles_l), the
+    moment m and the denominator tile q (cr_operands), w = factor / D, and
+      num1 += w m td, num2 += w m (ts + td), den1 += w q td, den2 += w q (ts + td);
+    den0 = sum t(a i)^2 + 1/4 sum t(a b i j)^2 + 1/2 sum t(a i) t(b j) t(a b i j)
+    (cr_ccsd_t_D.F:4-7 with c = t); energy_n = num_n / (1 + den_n + den0).
+    Returns (energy1, energy2, dict of the five terms)."""
+    vc = cr_operands(tiles, t1, t1h, t2, t2h, v2h, i1_1, i1_1h, i1_2, i1_2h)
+    noab = tiles.noab
+    r, o = tiles.range, tiles.offset
+    no = int(r[:noab].sum())
+    ev = tiles.evl
+
+    def orb(b):
+        s = int(o[b - 1]) - (no if b > noab else 0)
+        return np.arange(s, s + int(r[b - 1]))
+
+    num1 = num2 = den1 = den2 = 0.0
+    for q in _task_list(tiles):
+        p4, p5, p6, h1, h2, h3 = q
+        ts, td = task_tiles(tiles, t1, t1h, t2, t2h, v2, v2h, q)
+        qd, m = task_tiles(tiles, t1, t1h, t2, t2h, vc, v2h, q)
+        A, B, Cc, I, J, K = (orb(b) for b in q)
+        D = (-ev[no + A][:, None, None, None, None, None] - ev[no + B][None, :, None, None, None, None]
+             - ev[no + Cc][None, None, :, None, None, None] + ev[I][None, None, None, :, None, None]
+             + ev[J][None, None, None, None, :, None] + ev[K][None, None, None, None, None, :])
+        f = 2.0 if tiles.restricted else 1.0
+        if p4 == p5 == p6:
+            f /= 6.0
+        elif p4 == p5 or p5 == p6:
+            f /= 2.0
+        if h1 == h2 == h3:
+            f /= 6.0
+        elif h1 == h2 or h2 == h3:
+            f /= 2.0
+        num1 += f * np.sum(m * td / D)
+        num2 += f * np.sum(m * (ts + td) / D)
+        den1 += f * np.sum(qd * td / D)
+        den2 += f * np.sum(qd * (ts + td) / D)
+    T1 = _dense_t1(tiles, t1, t1h)
+    T2 = _dense_y2(tiles, t2, t2h)
+    den0 = (float(np.sum(T1 * T1)) + 0.25 * float(np.sum(T2 * T2))
+            + 0.5 * float(np.einsum("ai,bj,abij->", T1, T1, T2)))
+    terms = dict(num1=float(num1), num2=float(num2), den0=den0, den1=float(den1), den2=float(den2))
+    return (terms["num1"] / (1.0 + terms["den1"] + den0),
+            terms["num2"] / (1.0 + terms["den2"] + den0), terms)

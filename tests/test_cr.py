@@ -149,3 +149,47 @@ def test_overlap_closed_form():
     assert s.dense_cr(s.W, s.V)["den0"] == pytest.approx(0.25 * float(np.sum(t2 * t2)), rel=1e-14)
     s.T1, s.T2 = t1, np.zeros_like(t2)
     assert s.dense_cr(s.W, s.V)["den0"] == pytest.approx(float(np.sum(t1 * t1)), rel=1e-14)
+
+
+@pytest.mark.parametrize("prm", SYSTEMS, ids=IDS)
+def test_oracle_task_loop_equals_the_textbook(prm):
+    """oracle.cr_ccsd_t (the task loop of cr_ccsd_t.F over per-task tiles,
+    the intermediates in their own tables) against the dense textbook."""
+    import oracle
+
+    s = spin_systems.SpinSystem(**prm)
+    I = s.cr_inputs()
+    b = s.blocks
+    e1, e2, terms = oracle.cr_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                                     I["i1_1"], I["i1_1h"], I["i1_2"], I["i1_2h"])
+    want = s.dense_cr(I["I1"], I["I2"])
+    for k, v in terms.items():
+        assert v == pytest.approx(want[k], rel=1e-12), k
+    assert e1 == pytest.approx(want["energy1"], rel=1e-12)
+    assert e2 == pytest.approx(want["energy2"], rel=1e-12)
+
+
+def test_planted_i1_block_order_mistake_is_caught():
+    """i1(h11 p4 h1 h2) laid out [h11][p4][h1][h2] (the v2 order) instead of
+    [p4][h11][h1][h2]: the moment sums leave the textbook."""
+    import oracle
+
+    s = spin_systems.SpinSystem(**SYSTEMS[0])
+    I = s.cr_inputs()
+    b = s.blocks
+    (_, e1, _), _ = s.i1_tables()
+    T = s.tiles
+    no = s.no
+
+    def sl(t):
+        o = int(T.offset[t - 1]) - (no if t > T.noab else 0)
+        return slice(o, o + int(T.range[t - 1]))
+
+    bad = I["i1_1"].copy()
+    for _, off, (p4, h11, h1, h2) in e1:
+        blk = I["I1"][sl(h11), sl(p4), sl(h1), sl(h2)].ravel()
+        bad[off:off + blk.size] = blk
+    _, _, terms = oracle.cr_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                                   bad, I["i1_1h"], I["i1_2"], I["i1_2h"])
+    want = s.dense_cr(I["I1"], I["I2"])
+    assert terms["num1"] != pytest.approx(want["num1"], rel=1e-6)

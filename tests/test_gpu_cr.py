@@ -1,7 +1,8 @@
 """CR-CCSD(T) on the GPU (tce_t_job_upload_cr, tce_t_cr_ccsd_t) against the
 dense textbook of tests/spin_systems.py (written from the term lists of
 cr_ccsd_t_N.F, cr_ccsd_t_E.F and cr_ccsd_t_D.F and pinned in
-tests/test_cr.py), and against the library's own (T) energies when the
+tests/test_cr.py), against the oracle's task loop (oracle.cr_ccsd_t, equal to
+the textbook in tests/test_cr.py), and against the library's own (T) energies when the
 intermediates are the bare integrals (the moment tile is then the (T)
 doubles tile: num1 = E[T], num2 = E(T)).
 
@@ -14,6 +15,7 @@ import sys
 
 import pytest
 
+import oracle
 from nwchem_amd import TriplesJob
 from nwchem_amd.job import TILING_ORBITAL, TILING_REFERENCE, cr_ccsd_t
 
@@ -54,6 +56,12 @@ def test_cr_matches_the_dense_textbook(prm):
     for k in KEYS:
         assert _close(terms[k], want[k]), (k, terms[k], want[k])
     assert _close(e1, want["energy1"]) and _close(e2, want["energy2"])
+    b = s.blocks
+    o1, o2, ot = oracle.cr_ccsd_t(s.oracle_tiles(), b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h,
+                                  I["i1_1"], I["i1_1h"], I["i1_2"], I["i1_2h"])
+    for k in KEYS:
+        assert _close(terms[k], ot[k]), (k, terms[k], ot[k])
+    assert _close(e1, o1) and _close(e2, o2)
 
 
 @pytest.mark.parametrize("prm", CR_GPU, ids=IDS)
