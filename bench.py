@@ -456,7 +456,8 @@ def cr_identity_inputs(tm, blocks):
 
 
 def bench_cr(tm, blocks, e_bt: float, e_pt: float, t_step: float):
-    """tce_t_cr_ccsd_t over every task of the job (FP64): the left pass over
+    """tce_t_cr_ccsd_t over every task of the job (FP64, then 6-term
+    split-BF16): the left pass over
     the (T) job, the CR right pass over the CR job (moment and denominator
     tiles), with the bare integrals as intermediates so that num1 and num2
     are the job's E[T] and E(T); reported against two (T) energies' time."""
@@ -472,10 +473,19 @@ def bench_cr(tm, blocks, e_bt: float, e_pt: float, t_step: float):
         t1 = time.perf_counter()
         e1, e2, terms = cr_ccsd_t(tj, cj)
         wall = time.perf_counter() - t1
+        # 6-term split-BF16: the first call converts both jobs' operands (untimed)
+        cr_ccsd_t(tj, cj, 3)
+        t1 = time.perf_counter()
+        m1, m2, _ = cr_ccsd_t(tj, cj, 3)
+        mw = time.perf_counter() - t1
     rel = max(abs(terms["num1"] - e_bt) / abs(e_bt), abs(terms["num2"] - e_pt) / abs(e_pt))
+    bf = {"seconds": mw, "speedup_vs_fp64": wall / mw, "energy1": m1, "energy2": m2,
+          "dE_rel": max(abs(m1 - e1) / abs(e1), abs(m2 - e2) / abs(e2))}
+    print(f"[bench] cr-ccsd(t) bf16x6: {mw:.2f} s ({wall / mw:.2f}x FP64), relative dE "
+          f"{bf['dE_rel']:.2e}", file=sys.stderr, flush=True)
     out = {"seconds": wall, "energy1": e1, "energy2": e2, "terms": terms,
            "vs_two_triples_energies": wall / (2.0 * t_step), "inputs_seconds": t_in,
-           "num_vs_triples_rel": rel}
+           "num_vs_triples_rel": rel, "precision_modes": {"bf16x6": bf}}
     print(f"[bench] cr-ccsd(t): {wall:.2f} s ({wall / (2.0 * t_step):.3f}x two (T) energies), "
           f"num1/num2 against E[T]/E(T): rel. diff {rel:.2e}, energy1={e1:.12e} "
           f"energy2={e2:.12e}", file=sys.stderr, flush=True)

@@ -355,8 +355,10 @@ int tce_t_job_upload_cr(tce_t_job *job, const double *t1, const long *k_t1_offse
                         const long *k_i1_offset_2);
 /* Replaces cr_ccsd_t(..., energy1, energy2) (cr_ccsd_t.F:1-269) after its
  * intermediates are built: triples holds t1/t2/v2 (tce_t_job_upload), cr
- * the CR job of the same tiles and tiling (not TCE_T_TILING_ORBITAL).  In
- * FP64, over the tasks of the (T) loop with w = factor / D:
+ * the CR job of the same tiles and tiling (not TCE_T_TILING_ORBITAL).  The
+ * d1/d2 contractions of both passes run in `precision` (TCE_T_FP64 as the
+ * reference), the sums in FP64; over the tasks of the (T) loop with
+ * w = factor / D:
  *   num1 = sum w * moment * tdoubles,  num2 = sum w * moment * (tsingles + tdoubles)
  *   den1 = sum w * den * tdoubles,     den2 = sum w * den * (tsingles + tdoubles)
  * den0 = the overlap of cr_ccsd_t_D with c = t, and
@@ -364,8 +366,8 @@ int tce_t_job_upload_cr(tce_t_job *job, const double *t1, const long *k_t1_offse
  * terms (5 doubles or NULL) receives num1, num2, den0, den1, den2.
  * Synchronous; two passes of the contraction kernel with the window of
  * tce_t_lambda_ccsd_t. */
-int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy1, double *energy2,
-                    double *terms);
+int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, int precision, double *energy1,
+                    double *energy2, double *terms);
 /* Host packer of tce_t_job_upload, for inspection and tests: the dense tensors
  * the job places in HBM, built from the same blocks and hash tables.  With
  * Ep = no. of virtuals and Mp = no. of occupieds rounded up to a multiple of 8:

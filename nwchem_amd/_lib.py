@@ -101,7 +101,7 @@ def _declare(L: C.CDLL) -> None:
         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), LP]
     L.tce_t_job_upload_lambda.argtypes = [C.c_void_p, DP, LP, DP, LP, DP, LP]
     L.tce_t_job_upload_cr.argtypes = [C.c_void_p, DP, LP, DP, LP, DP, LP, DP, LP]
-    L.tce_t_cr_ccsd_t.argtypes = [C.c_void_p, C.c_void_p, DP, DP, DP]
+    L.tce_t_cr_ccsd_t.argtypes = [C.c_void_p, C.c_void_p, C.c_int, DP, DP, DP]
     L.tce_t_lambda_ccsd_t.argtypes = [C.c_void_p, C.c_void_p, DP, LP, C.c_int, C.c_int, DP, DP,
                                       DP, C.c_long]
     L.tce_t_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]

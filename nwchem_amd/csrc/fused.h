@@ -420,14 +420,14 @@ int launch_fused_tiles(const MixedSegment *segs, long total, int precision, int 
 // / D with the left-hand values read from args->lam_y at the same slots.
 int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int grid,
                         hipStream_t s, bool sparse, unsigned long long *sched, bool left);
-// The CR-CCSD(T) right-hand pass (FP64): the right-hand job's t3 doubles
+// The CR-CCSD(T) right-hand pass (in `precision`): the right-hand job's t3 doubles
 // (the moment M3 of cr_ccsd_t_N) and singles (the denominator tile of
 // cr_ccsd_t_E) against the left-hand job's t3 doubles and singles in the
 // window at args->lam_y: per wave factor * M3 * td / D and factor * M3 *
 // (td + ts) / D into args->f.partials, the same with the denominator tile
 // into args->cr_den.
-int launch_fused_cr_right(const MixedSegment *segs, long total, int grid, hipStream_t s,
-                          bool sparse, unsigned long long *sched);
+int launch_fused_cr_right(const MixedSegment *segs, long total, int precision, int grid,
+                          hipStream_t s, bool sparse, unsigned long long *sched);
 // Zeroed counters of the dynamic schedule for one launching stream (every
 // launch leaves them zero; launches that may overlap need their own), or
 // null on failure (error set).  Free with hipFree.

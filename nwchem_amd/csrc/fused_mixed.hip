@@ -1160,14 +1160,6 @@ static int launch_prec(const MixedSegment *segs, int nseg, long total, int preci
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int EPI>
-static int launch_prec_fp64(const MixedSegment *segs, long total, int grid, bool sparse,
-                            hipStream_t s, unsigned long long *q) {
-  sparse ? launch_one<TCE_T_FP64, EPI, true>(segs, 1, total, grid, s, q)
-         : launch_one<TCE_T_FP64, EPI, false>(segs, 1, total, grid, s, q);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
 int launch_fused_mixed(const MixedSegment *segs, int nseg, long total, int precision, int grid,
                        hipStream_t s, bool sparse, unsigned long long *sched) {
   return launch_prec<EPI_ENERGY>(segs, nseg, total, precision, grid, sparse, s, sched);
@@ -1184,9 +1176,9 @@ int launch_fused_lambda(const MixedSegment *segs, long total, int precision, int
               : launch_prec<EPI_LAMBDA_RIGHT>(segs, 1, total, precision, grid, sparse, s, sched);
 }
 
-int launch_fused_cr_right(const MixedSegment *segs, long total, int grid, hipStream_t s,
-                          bool sparse, unsigned long long *sched) {
-  return launch_prec_fp64<EPI_CR_RIGHT>(segs, total, grid, sparse, s, sched);
+int launch_fused_cr_right(const MixedSegment *segs, long total, int precision, int grid,
+                          hipStream_t s, bool sparse, unsigned long long *sched) {
+  return launch_prec<EPI_CR_RIGHT>(segs, 1, total, precision, grid, sparse, s, sched);
 }
 
 

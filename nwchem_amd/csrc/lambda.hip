@@ -168,7 +168,7 @@ int fock_hole_particle(const Tiles &T, const double *f1, const long *f1h,
 // order, the right-hand pass forms the energy partials of the same items
 // from its t3 doubles and the window; the job's task reduction then gives
 // the per-task sums.  No tile leaves the registers in the tile layout.
-// cr_den (CR-CCSD(T), FP64): the right-hand pass is the CR one; te gets the
+// cr_den (CR-CCSD(T)): the right-hand pass is the CR one; te gets the
 // moment sums, *te_den the denominator sums, per task.
 int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<double> &te,
                  double *cr_den = nullptr, std::vector<double> *te_den = nullptr) {
@@ -219,7 +219,7 @@ int lambda_fused(Job &JR, Job &JL, const double *dF, int precision, std::vector<
     MR.lam_item0 = i0;
     MR.cr_den = cr_den;
     const MixedSegment *sr = stage_mixed(JR, MR, i1, i0);
-    if (!sr || (cr_den ? launch_fused_cr_right(sr, i1 - i0, grid, JR.stream, JR.sparse,
+    if (!sr || (cr_den ? launch_fused_cr_right(sr, i1 - i0, precision, grid, JR.stream, JR.sparse,
                                                sched_use(JR.dsched))
                        : launch_fused_lambda(sr, i1 - i0, precision, grid, JR.stream, JR.sparse,
                                              sched_use(JR.dsched), false)) != 0) {
@@ -484,10 +484,11 @@ extern "C" int tce_t_lambda_ccsd_t(tce_t_job *right, tce_t_job *left, const doub
 // cr_ccsd_t (cr_ccsd_t.F:1-269) from a (T) job and a CR job of the same
 // tiles: the task loop's four sums over the t3 tiles of the (T) job (left
 // pass) and the moment and denominator tiles of the CR job (right pass),
-// den0 from the CR job's t1 and t2, then energy1 = num1 / (1 + den1 +
-// den0), energy2 = num2 / (1 + den2 + den0) (cr_ccsd_t.F:254-258).
-extern "C" int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy1,
-                               double *energy2, double *terms) {
+// their d1/d2 contractions in `precision`, den0 (FP64) from the CR job's t1
+// and t2, then energy1 = num1 / (1 + den1 + den0), energy2 = num2 / (1 +
+// den2 + den0) (cr_ccsd_t.F:254-258).
+extern "C" int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, int precision,
+                               double *energy1, double *energy2, double *terms) {
   if (!triples || !cr || triples == cr) {
     set_error("tce_t_cr_ccsd_t: two distinct jobs are needed");
     return TCE_T_ERR_ARG;
@@ -510,6 +511,10 @@ extern "C" int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy
     set_error("tce_t_cr_ccsd_t: not available for the orbital tiling");
     return TCE_T_ERR_ARG;
   }
+  if (!valid_precision(precision)) {
+    set_error("unsupported precision mode");
+    return TCE_T_ERR_ARG;
+  }
   HIPCHK(hipSetDevice(JR.device), "device");
   const long nt = JR.ntask();
   int rc = ensure_tasks8(JR);
@@ -521,7 +526,7 @@ extern "C" int tce_t_cr_ccsd_t(tce_t_job *triples, tce_t_job *cr, double *energy
     return TCE_T_ERR_HIP;
   }
   std::vector<double> te, td, hov(OV_BLOCKS);
-  rc = lambda_fused(JR, JL, nullptr, TCE_T_FP64, te, den, &td);
+  rc = lambda_fused(JR, JL, nullptr, precision, te, den, &td);
   if (rc == TCE_T_OK) {
     hipLaunchKernelGGL(cr_overlap, dim3(OV_BLOCKS), dim3(OV_THREADS), 0, JR.stream, JR.dT1,
                        JR.dT2e, (int)JR.T.no, (int)JR.T.nv, JR.Ep, ov);

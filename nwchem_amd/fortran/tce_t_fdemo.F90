@@ -311,7 +311,7 @@ contains
     if (rc /= TCE_T_OK) stop 13
     rc = tce_t_job_upload_cr(crj, t1, h1, t2, h2, i1a, hi1, i1b, hi2)
     if (rc /= TCE_T_OK) stop 14
-    rc = tce_t_cr_ccsd_t(job, crj, c1, c2, terms)
+    rc = tce_t_cr_ccsd_t(job, crj, TCE_T_FP64, c1, c2, terms)
     if (rc /= TCE_T_OK) stop 15
     write(*, '(A,5ES24.16)') 'cr terms=', terms
     write(*, '(A,ES24.16,A,ES24.16)') 'cr energy1=', c1, ' energy2=', c2

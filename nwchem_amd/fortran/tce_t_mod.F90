@@ -182,10 +182,11 @@ module tce_t_mi355x
 
     ! energy1, energy2 of cr_ccsd_t from a (T) job and a CR job; terms(5) =
     ! num1, num2, den0, den1, den2
-    integer(c_int) function tce_t_cr_ccsd_t(triples, cr, energy1, energy2, terms) &
+    integer(c_int) function tce_t_cr_ccsd_t(triples, cr, precision, energy1, energy2, terms) &
         bind(C, name="tce_t_cr_ccsd_t")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: triples, cr
+      integer(c_int), value :: precision
       real(c_double), intent(out) :: energy1, energy2
       real(c_double), intent(out) :: terms(*)
     end function

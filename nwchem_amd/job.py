@@ -270,15 +270,16 @@ def lambda_ccsd_t(right: TriplesJob, left: TriplesJob, f1: np.ndarray | None = N
     return e1.value, e2.value, te
 
 
-def cr_ccsd_t(triples: TriplesJob, cr: TriplesJob) -> tuple[float, float, dict]:
+def cr_ccsd_t(triples: TriplesJob, cr: TriplesJob,
+              precision: int = FP64) -> tuple[float, float, dict]:
     """cr_ccsd_t.F: (energy1, energy2, terms) from a (T) job (upload) and a
     CR job of the same tiles (upload_cr); terms = num1, num2, den0, den1,
     den2, with energy1 = num1 / (1 + den1 + den0) and energy2 = num2 /
-    (1 + den2 + den0).  FP64."""
+    (1 + den2 + den0).  The d1/d2 contractions run in `precision`."""
     e1, e2 = C.c_double(0.0), C.c_double(0.0)
     terms = np.zeros(5)
-    check(_lib.lib().tce_t_cr_ccsd_t(triples._h, cr._h, C.byref(e1), C.byref(e2),
-                                     terms.ctypes.data_as(DP)), "tce_t_cr_ccsd_t")
+    check(_lib.lib().tce_t_cr_ccsd_t(triples._h, cr._h, int(precision), C.byref(e1),
+                                     C.byref(e2), terms.ctypes.data_as(DP)), "tce_t_cr_ccsd_t")
     return e1.value, e2.value, dict(zip(("num1", "num2", "den0", "den1", "den2"),
                                         (float(x) for x in terms)))
 

@@ -8,7 +8,11 @@ doubles tile: num1 = E[T], num2 = E(T)).
 
 Tolerance: FP64, |dx| <= 1e-10 |x| + 1e-15 for every sum and energy (the
 device sums each task in fixed-order blocks, the textbook over dense
-tensors in numpy's order).
+tensors in numpy's order).  Reduced precision, against FP64: the t3, moment
+and denominator tiles carry the d1/d2 product error of the mode, so
+|dx| <= tol |x| with tol 1e-6 for FP32 and BF16X6 (products to 2^-24) and
+3e-5 for BF16X3 (the dropped lo*lo term), as for Lambda-CCSD(T); den0 is
+FP64 in every mode.
 """
 import os
 import sys
@@ -17,7 +21,7 @@ import pytest
 
 import oracle
 from nwchem_amd import TriplesJob
-from nwchem_amd.job import TILING_ORBITAL, TILING_REFERENCE, cr_ccsd_t
+from nwchem_amd.job import BF16X3, BF16X6, FP32, TILING_ORBITAL, TILING_REFERENCE, cr_ccsd_t
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import spin_systems  # noqa: E402
@@ -102,3 +106,22 @@ def test_cr_rejects_wrong_jobs():
         cj.upload_cr(b.t1, b.t1h, b.t2, b.t2h, I["i1_1"], I["i1_1h"], I["i1_2"], I["i1_2h"])
         with pytest.raises(RuntimeError, match="orbital tiling"):
             cr_ccsd_t(tj, cj)
+
+
+@pytest.mark.parametrize("mode", [(FP32, 1e-6), (BF16X6, 1e-6), (BF16X3, 3e-5)],
+                         ids=["fp32", "bf16x6", "bf16x3"])
+def test_cr_precision_modes_against_fp64(mode):
+    prec, tol = mode
+    s = spin_systems.SpinSystem(**CR_GPU[1])
+    I = s.cr_inputs()
+    b = s.blocks
+    print(f"cr precision {prec}: no={s.no} nv={s.nv}", flush=True)
+    with TriplesJob(s.tiles) as tj, TriplesJob(s.tiles) as cj:
+        tj.upload(b)
+        cj.upload_cr(b.t1, b.t1h, b.t2, b.t2h, I["i1_1"], I["i1_1h"], I["i1_2"], I["i1_2h"])
+        f1, f2, ft = cr_ccsd_t(tj, cj)
+        m1, m2, mt = cr_ccsd_t(tj, cj, prec)
+    assert mt["den0"] == ft["den0"]
+    for k in ("num1", "num2", "den1", "den2"):
+        assert abs(mt[k] - ft[k]) <= tol * abs(ft[k]), (k, mt[k], ft[k])
+    assert abs(m1 - f1) <= tol * abs(f1) and abs(m2 - f2) <= tol * abs(f2)
