@@ -208,8 +208,8 @@ def ragged(n: int, ts: int) -> list:
     return r
 
 
-def config1(threads: int) -> dict | None:
-    """Config 1 (no=8 in one tile, nv=32 in tiles of 16, 4 tasks) whole: the
+def small_case(threads: int) -> dict | None:
+    """The small case (no=8 in one tile, nv=32 in tiles of 16, 4 tasks) whole: the
     reference CPU path (its compiled Fortran kernels under the ccsd_t_gpu.F
     task loop at usedevice = 0) on one core and on `threads` OpenMP threads,
     and the GPU job on the same inputs."""
@@ -244,7 +244,7 @@ def config1(threads: int) -> dict | None:
             em = j.run(p)
             out["precision_modes_dE_uEh"][name] = [(em[0] - e[0]) * 1e6, (em[1] - e[1]) * 1e6]
     out["dE_gpu_vs_cpu"] = [e[0] - out["cpu_1core"]["E_bt"], e[1] - out["cpu_1core"]["E_pt"]]
-    print(f"[bench] config 1: reference CPU {out['cpu_1core']['seconds']:.2f} s on 1 core, "
+    print(f"[bench] no=8/nv=32: reference CPU {out['cpu_1core']['seconds']:.2f} s on 1 core, "
           f"{out['cpu_all_cores']['seconds']:.2f} s on {out['cpu_all_cores']['threads']} threads; "
           f"GPU {out['gpu']['seconds'] * 1e3:.2f} ms; dE {out['dE_gpu_vs_cpu']}; reduced precision "
           f"dE (uEh) {out['precision_modes_dE_uEh']}", file=sys.stderr, flush=True)
@@ -557,7 +557,7 @@ def bench_serve(nreq: int, lam: float, npool: int, seed: int, slice_ms: float,
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(lat.mean()), "wall_s": wall, "energies_per_s": nreq / wall,
            "arrival_rate_per_s": arrival_rate, "completion_rate_per_s": nreq / wall,
-           "offered_load_at_config2_rates": demand,
+           "offered_load_at_no40_nv320_rates": demand,
            "queue_wait_first_quarter_p50_s": float(np.percentile(wait[:q], 50)),
            "queue_wait_last_quarter_p50_s": float(np.percentile(wait[-q:], 50)),
            "fp64_equiv_tflops": F / wall / 1e12, "slices": st["slices"], "prep_s": t_prep,
@@ -585,7 +585,7 @@ def main() -> int:
                     help="OpenMP threads of the CPU baseline (0: all physical cores of this "
                          "process's affinity, capped by OMP_NUM_THREADS when set)")
     ap.add_argument("--batch-jobs", type=int, default=64,
-                    help="batch of independent jobs (config 4: 64; 0: skip)")
+                    help="batch of independent jobs (64 by default; 0: skip)")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="timed batch runs per precision (graph replays)")
     ap.add_argument("--batch-small-jobs", type=int, default=128,
@@ -766,13 +766,13 @@ def main() -> int:
     serve_low = bench_serve(args.serve_low, args.serve_low_rate, args.serve_pool, args.seed,
                             args.slice_ms, rates if len(rates) == 2 else None) if args.serve_low else None
     cpu = None
-    cfg1 = None
+    small = None
     if args.cpu_baseline:
         threads = args.cpu_threads or cpu_threads_default()
         print(f"[bench] cpu baseline on {threads} threads ({host_cpu()})", file=sys.stderr, flush=True)
         del blocks
         cpu = cpu_baseline(tm, threads)
-        cfg1 = config1(threads)
+        small = small_case(threads)
         if cpu:
             print(f"[bench] cpu baseline {cpu['value']*1e3:.2f} GFLOP/s ({cpu['seconds']:.1f}s)",
                   file=sys.stderr, flush=True)
@@ -801,7 +801,7 @@ def main() -> int:
         "roofline": roofline,
         "reference_tiling": reftile,
         "cpu_baseline": cpu,
-        "config1": cfg1,
+        "no8_nv32": small,
         "precision_modes": modes or None,
         "task_tiles": tiles_out,
         "lambda_ccsd_t": lam,

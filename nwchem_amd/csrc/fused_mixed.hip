@@ -63,7 +63,7 @@ struct Prec<TCE_T_FP64> {
   static constexpr int KC = 8;     // contraction values per chunk
   static constexpr int NPART = 1;  // operand parts
   static constexpr int ES = 8;     // bytes per element
-  // LDS chunk buffers (24 KB each), chunks loaded NBUF - 1 ahead.  Config 2
+  // LDS chunk buffers (24 KB each), chunks loaded NBUF - 1 ahead.  At no=40/nv=320
   // (no=40, nv=320), kernel ms: 3 buffers 8283, 4 buffers 8176, 5 buffers 8251.
   static constexpr int NBUF = 4;
   static constexpr bool SPLIT = false;  // chunk loop: see pass_step
@@ -74,7 +74,7 @@ struct Prec<TCE_T_FP32> {
   static constexpr int KC = 16;
   static constexpr int NPART = 1;
   static constexpr int ES = 4;
-  static constexpr int NBUF = 3;  // config 2 kernel ms: 3 buffers 4310, 4 buffers 4349
+  static constexpr int NBUF = 3;  // kernel ms at no=40/nv=320: 3 buffers 4310, 4 buffers 4349
   static constexpr bool SPLIT = true;  // measured 0.7% faster than the direct loop
   typedef float acc;
 };
@@ -609,7 +609,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
     // load sources) are formed again in every chunk from a lane index the
     // compiler cannot see through.  Kept live across the loop they were
     // spilled, and a reload's s_waitcnt also drains the chunk loads in flight
-    // (vmcnt counts them in issue order).  Config 2 kernel ms, kept live /
+    // (vmcnt counts them in issue order).  Kernel ms at no=40/nv=320, kept live /
     // formed per chunk (two boxes each): bf16x6 2865-2915 / 2736-2823, bf16x3
     // 1598-1604 / 1589-1600, FP64 8200 / 8293 (FP64 keeps them live; a few
     // of its passes reload them).
@@ -632,7 +632,7 @@ __device__ __forceinline__ void pass_step(typename Prec<PREC>::acc (&acc)[64], c
         // last barrier) go between the two halves of the MFMAs, so their
         // address arithmetic issues while the matrix core works (three
         // buffers: 0.7% faster than before the MFMAs, 3% faster than after
-        // them; four buffers, FP64 at config 2: 8200 ms here, 8414 before)
+        // them; four buffers, FP64 at no=40/nv=320: 8200 ms here, 8414 before)
         constexpr int AH = Pr::NBUF - 1;
         mma(I0(), I2(), f);
         if (j + AH < total) {

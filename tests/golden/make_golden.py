@@ -2,10 +2,10 @@
 
     python3 tests/golden/make_golden.py        (needs oracle/_ref built from
                                                 the reference tree)
-    python3 tests/golden/make_golden.py --config2
+    python3 tests/golden/make_golden.py --no40-nv320
                                                (the same at the full benchmark
                                                 size; ~2 min per task on 8 cores)
-    python3 tests/golden/make_golden.py --precision [--with-config2]
+    python3 tests/golden/make_golden.py --precision [--with-no40-nv320]
                                                (reduced-precision references;
                                                 the benchmark-job tasks take
                                                 about an hour on 8 cores)
@@ -21,7 +21,7 @@ energies.npz  E[T], E(T) and every task's pair for the systems in SYSTEMS,
               arithmetic, plus SHA-256 digests of the input blocks and hash
               tables, so a test can tell an input change from a result change.
 
-config2_tasks.json  (--config2) {E[T], E(T)} of eight tasks of the
+no40_nv320_tasks.json  (--no40-nv320) {E[T], E(T)} of eight tasks of the
               benchmark job (no=40 in tiles of 20, nv=320 in tiles of 40,
               seed 20261202, 480 tasks): the first and last task, tasks whose
               three particle (and hole) tiles are all equal, two equal, all
@@ -38,7 +38,7 @@ precision.npz / precision.json  (--precision) for the systems of
               (oracle.arith, tests/precision_model.py: products exact, FP64
               accumulation), totals and per task, with the accumulation
               bounds of tests/precision_model.py (per task and total); and
-              for two tasks of the benchmark job (CONFIG2_PRECISION_TASKS)
+              for two tasks of the benchmark job (NO40_NV320_PRECISION_TASKS)
               the same per-task values.
 
 The inputs of the synthetic systems come from the library's Philox generator
@@ -69,7 +69,7 @@ KERNEL_SHAPES = [(2, 3, 1, 4, 3, 2, 3), (3, 1, 2, 2, 5, 3, 2)]
 
 # name -> (kind, parameters)
 SYSTEMS = {
-    "cfg1_no8_nv32": ("synthetic", dict(no=8, nv=32, to=8, tv=16, seed=1234)),
+    "no8_nv32": ("synthetic", dict(no=8, nv=32, to=8, tv=16, seed=1234)),
     "no7_nv13_to3_tv5": ("synthetic", dict(no=7, nv=13, to=3, tv=5, seed=11)),
     "no5_nv9_to2_tv4": ("synthetic", dict(no=5, nv=9, to=2, tv=4, seed=12)),
     "no10_nv22_to4_tv6": ("synthetic", dict(no=10, nv=22, to=4, tv=6, seed=13)),
@@ -126,10 +126,10 @@ def digest(*arrays) -> str:
     return h.hexdigest()
 
 
-CONFIG2 = dict(no=40, nv=320, to=20, tv=40, seed=20261202)
+NO40_NV320 = dict(no=40, nv=320, to=20, tv=40, seed=20261202)
 # (p4b, p5b, p6b, h1b, h2b, h3b) of the fixture tasks (particles are tiles
 # 3..10, holes 1..2)
-CONFIG2_TASKS = [
+NO40_NV320_TASKS = [
     (3, 3, 3, 1, 1, 1),     # first task: all tiles equal in both spaces
     (10, 10, 10, 2, 2, 2),  # last task: all equal in both spaces
     (7, 7, 7, 1, 2, 2),     # particles all equal, two holes equal
@@ -141,9 +141,9 @@ CONFIG2_TASKS = [
 ]
 
 
-PRECISION_SYSTEMS = ["cfg1_no8_nv32", "no7_nv13_to3_tv5", "no5_nv9_to2_tv4", "no9_nv41_ragged",
+PRECISION_SYSTEMS = ["no8_nv32", "no7_nv13_to3_tv5", "no5_nv9_to2_tv4", "no9_nv41_ragged",
                      "spin_sym_unrestricted", "spin_flip_restricted"]
-CONFIG2_PRECISION_TASKS = [(3, 3, 3, 1, 1, 1), (3, 5, 9, 1, 1, 2)]
+NO40_NV320_PRECISION_TASKS = [(3, 3, 3, 1, 1, 1), (3, 5, 9, 1, 1, 2)]
 
 
 def precision() -> int:
@@ -176,18 +176,18 @@ def precision() -> int:
     if os.path.exists(path):  # keep the benchmark-job entries of an earlier run
         with open(path) as f:
             old = json.load(f)
-        if "config2" in old:
-            meta["config2"] = old["config2"]
+        if "no40_nv320" in old:
+            meta["no40_nv320"] = old["no40_nv320"]
     with open(path, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
         f.write("\n")
-    if "--with-config2" not in sys.argv:
+    if "--with-no40-nv320" not in sys.argv:
         return 0
     # two tasks of the benchmark job
-    ot, b, tm = build_system("synthetic", CONFIG2)
+    ot, b, tm = build_system("synthetic", NO40_NV320)
     tasks = [tuple(int(x) for x in t) for t in tm.tasks()]
-    meta["config2"] = {"params": CONFIG2, "inputs_sha256": digest(b.t1, b.t2, b.v2), "tasks": []}
-    for t in CONFIG2_PRECISION_TASKS:
+    meta["no40_nv320"] = {"params": NO40_NV320, "inputs_sha256": digest(b.t1, b.t2, b.v2), "tasks": []}
+    for t in NO40_NV320_PRECISION_TASKS:
         k = tasks.index(t)
         entry = {"index": k, "tiles": list(t)}
         t0 = time.time()
@@ -203,17 +203,17 @@ def precision() -> int:
             entry[mode] = {"E_bt": float(te[0, 0]).hex(), "E_pt": float(te[0, 1]).hex(),
                            "E_bt_dec": float(te[0, 0]), "E_pt_dec": float(te[0, 1]),
                            "bound_bt": float(bnd[0]), "bound_pt": float(bnd[1])}
-            print(f"config2 task {k} {t} {mode}: {te[0]} bound {bnd} ({time.time() - t0:.0f} s)",
+            print(f"no40_nv320 task {k} {t} {mode}: {te[0]} bound {bnd} ({time.time() - t0:.0f} s)",
                   flush=True)
         del s_, d_, a_
-        meta["config2"]["tasks"].append(entry)
+        meta["no40_nv320"]["tasks"].append(entry)
         with open(path, "w") as f:  # after every task: partial results survive
             json.dump(meta, f, indent=1, sort_keys=True)
             f.write("\n")
     return 0
 
 
-def config2(threads: int | None = None) -> int:
+def no40_nv320(threads: int | None = None) -> int:
     """Golden task energies of the full-size benchmark job (see above)."""
     import time
 
@@ -221,18 +221,18 @@ def config2(threads: int | None = None) -> int:
 
     if threads:
         os.environ["OMP_NUM_THREADS"] = str(threads)
-    ot, b, tm = build_system("synthetic", CONFIG2)
+    ot, b, tm = build_system("synthetic", NO40_NV320)
     tasks = [tuple(int(x) for x in t) for t in tm.tasks()]
     be = oracle.ref_backend("par")
-    path = os.path.join(HERE, "config2_tasks.json")
-    meta = {"params": CONFIG2, "tasks_total": len(tasks),
+    path = os.path.join(HERE, "no40_nv320_tasks.json")
+    meta = {"params": NO40_NV320, "tasks_total": len(tasks),
             "inputs_sha256": digest(b.t1, b.t2, b.v2),
             "tables_sha256": digest(b.t1h, b.t2h, b.v2h, tm.range, tm.evl),
             "kernels": "oracle/_ref/libsdt_ref_par.so (ccsd_t_kernels_omp.F, -O2 "
                        "-ffp-contract=off -fopenmp)",
             "task_order": "ccsd_t_gpu.F:87-112 (index into the job's task list)",
             "tasks": []}
-    for t in CONFIG2_TASKS:
+    for t in NO40_NV320_TASKS:
         k = tasks.index(t)
         t0 = time.time()
         e1, e2, te = oracle.ccsd_t(ot, b.t1, b.t1h, b.t2, b.t2h, b.v2, b.v2h, backend=be,
@@ -254,8 +254,8 @@ def config2(threads: int | None = None) -> int:
 def main() -> int:
     import oracle
 
-    if "--config2" in sys.argv:
-        return config2()
+    if "--no40-nv320" in sys.argv:
+        return no40_nv320()
     if "--precision" in sys.argv:
         return precision()
 

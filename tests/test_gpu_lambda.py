@@ -187,7 +187,7 @@ def test_lambda_at_the_benchmark_size_is_the_triples_energy():
     no Fock term the fused Lambda passes give the job's own E[T] and E(T)
     for every task, in FP64 and split-BF16 (3 terms).  The job's FP64 task
     energies at this size are pinned against the reference's compiled
-    kernels (tests/golden/config2_tasks.json)."""
+    kernels (tests/golden/no40_nv320_tasks.json)."""
     import bench
     from nwchem_amd import tiles as tl
 

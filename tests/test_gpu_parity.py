@@ -34,7 +34,7 @@ def tol(e):
 # operand representation within the accumulation bound of
 # tests/precision_model.py (tests/test_gpu_precision.py); here, where a
 # case is one of those fixture systems, the same references apply.
-FIXTURE_OF_CASE = {(8, 32, 8, 16, 1234): "cfg1_no8_nv32", (7, 13, 3, 5, 11): "no7_nv13_to3_tv5",
+FIXTURE_OF_CASE = {(8, 32, 8, 16, 1234): "no8_nv32", (7, 13, 3, 5, 11): "no7_nv13_to3_tv5",
                    (5, 9, 2, 4, 12): "no5_nv9_to2_tv4", (9, 41, 0, 0, 15): "no9_nv41_ragged"}
 
 
@@ -67,7 +67,7 @@ def assert_parity(gpu_e, gpu_te, ref):
 
 CASES = [
     # (no, nv, tile_o, tile_v, seed, ranges_o, ranges_v)
-    (8, 32, 8, 16, 1234, None, None),     # config 1 of the benchmark plan
+    (8, 32, 8, 16, 1234, None, None),     # the small case, no=8/nv=32
     (7, 13, 3, 5, 11, None, None),        # ranges not multiples of 4
     (5, 9, 2, 4, 12, None, None),
     (10, 22, 4, 6, 13, None, None),
@@ -192,7 +192,7 @@ def legacy_backend(request):
 
 
 @pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[5]],
-                         ids=["cfg1", "ragged_small", "ragged_tiles"])
+                         ids=["no8_nv32", "ragged_small", "ragged_tiles"])
 def test_legacy_surface_matches_oracle(case, legacy_backend):
     """ccsd_t_gpu.F's usedevice=1 call sequence (dev_mem_*, sd_t_*_cuda_,
     compute_en_, dev_release_, finalizememmodule_) against libtce_t.so, on
@@ -631,7 +631,7 @@ def test_empty_system_job_batch_and_server():
                 if p == 0:
                     assert abs(e[1, 0] - e1) <= tol(e1)
                 else:
-                    tot, _, _, bnd = precision_fixture("cfg1_no8_nv32", "bf16x3")
+                    tot, _, _, bnd = precision_fixture("no8_nv32", "bf16x3")
                     assert abs(e[1, 0] - tot[0]) <= bnd[0]
     finally:
         for j in jobs:
@@ -924,17 +924,17 @@ def test_triples_tiles_larger_than_the_host_staging_chunk():
 C2 = None
 
 
-def config2_inputs():
+def no40_nv320_inputs():
     """Inputs of the benchmark job, checked against the digest the golden
-    task energies were computed from (tests/golden/config2_tasks.json)."""
+    task energies were computed from (tests/golden/no40_nv320_tasks.json)."""
     global C2
     if C2 is None:
-        with open(os.path.join(GOLD, "config2_tasks.json")) as f:
+        with open(os.path.join(GOLD, "no40_nv320_tasks.json")) as f:
             meta = json.load(f)
         prm = meta["params"]
         tm = tl.TileModel.synthetic(prm["no"], prm["nv"], prm["to"], prm["tv"], prm["seed"])
         b = tl.synth_blocks(tm, prm["seed"])
-        print("config 2 inputs generated", flush=True)
+        print("no=40/nv=320 inputs generated", flush=True)
         assert mg.digest(b.t1, b.t2, b.v2) == meta["inputs_sha256"], "inputs changed"
         C2 = (tm, b, meta)
     return C2
@@ -949,16 +949,16 @@ def summation_tol(n, e_bt, e_pt):
     return 6.0 * np.sqrt(n) * 2.0 ** -53 * 2.0 * max(abs(e_bt), abs(e_pt))
 
 
-def test_config2_fp64_tasks_match_reference():
+def test_no40_nv320_fp64_tasks_match_reference():
     """Eight tasks of the benchmark job (first, last, equal / two equal /
     distinct tiles) against the energies of the reference's compiled kernels
     under the ccsd_t_gpu.F task loop, for the default sub-tiles and the
     reference tiles."""
     from nwchem_amd import TILING_REFERENCE
-    tm, b, meta = config2_inputs()
+    tm, b, meta = no40_nv320_inputs()
     got = {}
     for name, tiling in (("subtiles", None), ("reference tiles", TILING_REFERENCE)):
-        print(f"config 2: {name}", flush=True)
+        print(f"no=40/nv=320: {name}", flush=True)
         with (TriplesJob(tm) if tiling is None else TriplesJob(tm, tiling=tiling)) as job:
             job.upload(b)
             e = job.run()
@@ -980,15 +980,15 @@ def test_config2_fp64_tasks_match_reference():
 
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
-def test_config2_reduced_precision_tasks(mode):
+def test_no40_nv320_reduced_precision_tasks(mode):
     """The benchmark job in reduced precision: two tasks against the oracle in
     the same representation (tests/golden/precision.json) within the
     accumulation bound of tests/precision_model.py; |dE| against the FP64
     reference printed for all eight fixture tasks."""
     import precision_model as pm
-    tm, b, meta = config2_inputs()
+    tm, b, meta = no40_nv320_inputs()
     with open(os.path.join(GOLD, "precision.json")) as f:
-        pmeta = json.load(f)["config2"]
+        pmeta = json.load(f)["no40_nv320"]
     assert pmeta["inputs_sha256"] == meta["inputs_sha256"]
     with TriplesJob(tm) as job:
         job.upload(b)
@@ -1006,7 +1006,7 @@ def test_config2_reduced_precision_tasks(mode):
         assert np.all(d <= bnd), (t, te[t["index"]], em, bnd)
 
 
-def test_config2_task_through_the_per_call_surface(legacy_backend):
+def test_no40_nv320_task_through_the_per_call_surface(legacy_backend):
     """The first task of the benchmark job through the reference's per-call
     GPU surface in ccsd_t_gpu.F order (the drop-in of sd_t_total.cu: host
     block fetch and sort per call, operands copied per call, t3 accumulated
@@ -1014,7 +1014,7 @@ def test_config2_task_through_the_per_call_surface(legacy_backend):
     that task; the time per task is printed (what the binary drop-in costs
     next to the whole-job path)."""
     import time
-    tm, b, meta = config2_inputs()
+    tm, b, meta = no40_nv320_inputs()
     ot = oracle.Tiles(tm.range, tm.noab, tm.evl)
     L = _lib.lib()
     be = oracle.device_backend(L)
@@ -1027,17 +1027,17 @@ def test_config2_task_through_the_per_call_surface(legacy_backend):
     assert L.tce_t_legacy_status() == 0
     ref = np.array([float.fromhex(t["E_bt"]), float.fromhex(t["E_pt"])])
     d = np.abs(got[2][0] - ref)
-    print(f"per-call surface ({['direct', 'ttgt'][legacy_backend]}), config-2 task {k} {t['tiles']}: "
+    print(f"per-call surface ({['direct', 'ttgt'][legacy_backend]}), no=40/nv=320 task {k} {t['tiles']}: "
           f"{dt:.2f} s, |dE| = {d}", flush=True)
     assert np.all(d <= summation_tol(t["elements"], *ref))
 
 
-def test_config2_orbital_tiling_agrees():
+def test_no40_nv320_orbital_tiling_agrees():
     """The orbital tiling decomposes the benchmark job's sum into other items
     (pieces of the whole spaces, other factors and padding): E[T] and E(T)
     agree with the sub-tiled job to 1e-12 relative."""
     from nwchem_amd import TILING_ORBITAL
-    tm, b, meta = config2_inputs()
+    tm, b, meta = no40_nv320_inputs()
     with TriplesJob(tm) as job:
         job.upload(b)
         e = job.run()

@@ -113,7 +113,7 @@ def test_abs_mode_bounds_every_element():
         assert np.all(a >= np.abs(d) * (1 - 1e-12)) and a.max() > 0
 
 
-@pytest.mark.parametrize("name", ["cfg1_no8_nv32", "no5_nv9_to2_tv4", "spin_flip_restricted"])
+@pytest.mark.parametrize("name", ["no8_nv32", "no5_nv9_to2_tv4", "spin_flip_restricted"])
 def test_precision_references_are_current(name):
     ot, b, tm = _system(name)
     with np.load(os.path.join(GOLD, "precision.npz")) as z:
@@ -123,7 +123,7 @@ def test_precision_references_are_current(name):
             assert np.array_equal(z[f"{name}_{mode}_tasks"], te)
 
 
-@pytest.mark.parametrize("name", ["cfg1_no8_nv32", "no9_nv41_ragged"])
+@pytest.mark.parametrize("name", ["no8_nv32", "no9_nv41_ragged"])
 def test_bound_fails_a_two_term_split(name):
     ot, b, tm = _system(name)
     with np.load(os.path.join(GOLD, "precision.npz")) as z:
@@ -133,9 +133,9 @@ def test_bound_fails_a_two_term_split(name):
 
 
 def test_bound_fails_a_truncated_lo():
-    """At config 1 a truncated lo moves some task's energy beyond its bound
+    """At no=8/nv=32 a truncated lo moves some task's energy beyond its bound
     (it also fails the bit-exact split check of the GPU tests)."""
-    name = "cfg1_no8_nv32"
+    name = "no8_nv32"
     ot, b, tm = _system(name)
     with np.load(os.path.join(GOLD, "precision.npz")) as z:
         te, tb = z[f"{name}_bf16x3_tasks"], z[f"{name}_bf16x3_task_bounds"]
